@@ -1,0 +1,139 @@
+/*
+ * smolyax_amd — C ABI of the MI355X evaluation engine for barycentric Smolyak interpolants.
+ *
+ * The reference library (JoWestermann/smolyax) has no FFI; its boundary is the Python class
+ * SmolyakBarycentricInterpolator (src/smolyax/interpolation.py:15-390) and the per-group operator
+ * it jit/vmaps (interpolation.py:241-248).  Every entry point below names the reference function
+ * whose work it replaces.  Plain pointers and sizes only; device pointers are borrowed (the caller,
+ * e.g. torch, owns device memory); no entry point allocates device memory or synchronises the
+ * stream; return value 0 = success, < 0 = error (message via sbi_last_error(), thread-local).
+ */
+#ifndef SMOLYAX_AMD_H
+#define SMOLYAX_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----------------------------------------------------------------------------------------------
+ * Error codes and diagnostics
+ * -------------------------------------------------------------------------------------------- */
+enum {
+  SBI_OK = 0,
+  SBI_ERR_INVALID = -1,   /* bad argument / descriptor */
+  SBI_ERR_HIP = -2,       /* a HIP runtime call failed */
+  SBI_ERR_UNSUPPORTED = -3,
+  SBI_ERR_CAPACITY = -4   /* a size exceeds a documented limit */
+};
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* sbi_last_error(void);
+/* Library version string. */
+const char* sbi_version(void);
+
+/* ----------------------------------------------------------------------------------------------
+ * Multi-index sets (host, exact).  Bit-identical restatements of the numba kernels of
+ * src/smolyax/indices.py: same traversal order, same sequence of IEEE double operations.
+ * `k` is the anisotropy vector (length d), `t` the threshold.
+ * -------------------------------------------------------------------------------------------- */
+
+/* indices.py:72-118  indexset_cardinality(k, t) */
+int64_t sbi_indexset_cardinality(const double* k, int64_t d, double t);
+/* indices.py:121-168  smolyak_coefficient(k, d, rem_t, parity) */
+int64_t sbi_smolyak_coefficient(const double* k, int64_t d, double rem_t, int32_t parity);
+/* indices.py:262-289  nodeset_cardinality(k, t, nested) (non-nested branch: indices.py:175-211) */
+int64_t sbi_nodeset_cardinality(const double* k, int64_t d, double t, int32_t nested);
+/* indices.py:292-355  find_approximate_threshold(k, m, nested, max_iter, accuracy) */
+double sbi_find_approximate_threshold(const double* k, int64_t d, int64_t m, int32_t nested,
+                                      int32_t max_iter, double accuracy);
+
+/* indices.py:20-69  indexset(k, t).  Two-call pattern: with dims == NULL only the sizes are
+ * returned (*n_nu = |Lambda|, *n_entries = total number of (dim, deg) pairs).  With buffers,
+ * fills offsets[n_nu+1] (CSR into dims/degs) in the reference's DFS output order. */
+int sbi_indexset(const double* k, int64_t d, double t, int64_t* n_nu, int64_t* n_entries,
+                 int64_t* offsets, int32_t* dims, int32_t* degs);
+
+/* indices.py:214-259  non_zero_indices_and_zetas(k, t): multi-indices with zeta != 0 in the
+ * reference's discovery order (its per-n lists are this order filtered by n).  Same two-call
+ * pattern; zetas[n_nu] receives the Smolyak coefficients. */
+int sbi_nonzero_indices_and_zetas(const double* k, int64_t d, double t, int64_t* n_nu,
+                                  int64_t* n_entries, int64_t* offsets, int32_t* dims,
+                                  int32_t* degs, int64_t* zetas);
+
+/* ----------------------------------------------------------------------------------------------
+ * Evaluation plans.  A plan is the packed device layout of one interpolant (built by the host
+ * from set_f's packing, interpolation.py:140-235).  All pointers in the descriptor are DEVICE
+ * pointers owned by the caller and must outlive the plan; the plan copies only scalars.
+ * -------------------------------------------------------------------------------------------- */
+
+/* Barycentric combination-technique layout (the reference's algorithm, ragged instead of padded).
+ * Blocks are the multi-indices nu != 0 with zeta_nu != 0; each block's factors are its active
+ * dims in ascending order; the LAST factor of a block is "streamed", the others are "prefix"
+ * factors whose 1D bases live in an on-chip table.  Blocks are sorted by last slot (runs). */
+typedef struct sbi_bary_desc {
+  int64_t d_in, d_out;
+  int32_t n_slots;          /* distinct (dim, deg) pairs */
+  int32_t dmax1;            /* max deg + 1 over slots (row stride of slot_nodes / slot_w) */
+  int32_t n_prefix_vals;    /* sum over prefix slots of (deg + 1): on-chip table rows */
+  int32_t n_prefix_slots;
+  int32_t n_runs;           /* runs of consecutive blocks sharing their last slot */
+  int32_t n_blocks;
+  int32_t max_n;            /* max active dims per block */
+  int32_t _pad;
+  int64_t k_raw;            /* sum over blocks of prod(deg_i + 1) */
+  const int32_t* slot_dim;       /* [n_slots] */
+  const int32_t* slot_deg;       /* [n_slots] */
+  const double* slot_nodes;      /* [n_slots * dmax1] node values xi_mu */
+  const double* slot_w;          /* [n_slots * dmax1] first-form weights 1/prod_{nu!=mu}(xi_mu-xi_nu) */
+  const double* slot_q;          /* [n_slots * dmax1] quadrature weights (integral) */
+  const int32_t* prefix_slot;    /* [n_prefix_slots] slot id of each prefix slot */
+  const int32_t* prefix_off;     /* [n_prefix_slots] first table row of each prefix slot */
+  const int32_t* run_slot;       /* [n_runs] last slot of the run */
+  const int32_t* run_start;      /* [n_runs + 1] first block of each run */
+  const int32_t* blk_n;          /* [n_blocks] active dims */
+  const int32_t* blk_fac;        /* [n_blocks + 1] CSR start into fac_* (prefix factors only) */
+  const int32_t* fac_row;        /* prefix factor: first table row of its slot */
+  const int32_t* fac_deg;        /* prefix factor degree */
+  const int32_t* fac_slot;       /* prefix factor slot id */
+  const int32_t* blk_last_slot;  /* [n_blocks] streamed (last) slot of each block, -1 if none */
+  const int64_t* blk_foff;       /* [n_blocks + 1] offset of the block's F rows (rows of d_out) */
+  const double* F;               /* [k_raw * d_out] zeta-scaled values, row-major, block entries
+                                    in C order over (prefix factors..., last factor) */
+  const double* offset;          /* [d_out] zeta_0 * f(xi_0) */
+  const int32_t* wave_run;       /* [n_split + 1] run ranges per wave (cost-balanced) */
+  int32_t n_split;               /* waves per point tile */
+  int32_t _pad2;
+  const int32_t* dim_blk;        /* [d_in + 1] CSR: blocks containing each dim (jacobian) */
+  const int32_t* dim_blk_idx;    /* block ids, and the factor position in dim_blk_pos */
+  const int32_t* dim_blk_pos;    /*   (0..n-1, n-1 = last factor) */
+} sbi_bary_desc;
+
+typedef struct sbi_plan sbi_plan;
+
+/* Create / destroy a barycentric plan (host metadata only, validates the descriptor). */
+int sbi_bary_plan_create(const sbi_bary_desc* desc, sbi_plan** out);
+int sbi_plan_destroy(sbi_plan* plan);
+
+/* interpolation.py:264-304 __call__ with barycentric.py:69-123 per block:
+ * y[P, d_out] (FP64) = offset + sum_blocks zeta * tensor-product interpolant at x[P, ldx].
+ * x, y device pointers; stream = hipStream_t (NULL = default stream). */
+int sbi_bary_eval_f64(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx, double* y,
+                      void* stream);
+
+/* interpolation.py:306-345 gradient with barycentric.py:126-229: dense Jacobian
+ * J[P, d_out, d_in] (FP64), NaN where x_j equals a node of a slot of dim j (reference
+ * semantics, barycentric.py:152-154). */
+int sbi_bary_jacobian_f64(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx,
+                          double* J, void* stream);
+
+/* interpolation.py:347-390 integral: Q[d_out] = offset + sum_blocks zeta <F_nu, (x) q>.
+ * `partial` is caller-provided device workspace of sbi_bary_integral_workspace(plan) doubles. */
+int64_t sbi_bary_integral_workspace(const sbi_plan* plan);
+int sbi_bary_integral_f64(const sbi_plan* plan, double* Q, double* partial, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SMOLYAX_AMD_H */

@@ -1,0 +1,93 @@
+"""ctypes binding of libsmolyax_amd.so (the C ABI declared in include/smolyax_amd.h).
+
+The library is loaded after torch so that its HIP runtime dependency resolves to the runtime torch
+already mapped.  There is no fallback: a missing library raises at import time.
+"""
+
+import ctypes
+import os
+
+import torch  # noqa: F401  (maps torch's HIP runtime first)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsmolyax_amd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it with `python -m smolyax_amd.build` "
+        "(hipcc --offload-arch=gfx950); smolyax_amd has no CPU fallback."
+    )
+
+lib = ctypes.CDLL(LIB_PATH)
+
+c_i32, c_i64, c_f64, c_vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+P_f64 = ctypes.POINTER(ctypes.c_double)
+P_i64 = ctypes.POINTER(ctypes.c_int64)
+P_i32 = ctypes.POINTER(ctypes.c_int32)
+
+
+class BaryDesc(ctypes.Structure):
+    _fields_ = [
+        ("d_in", c_i64), ("d_out", c_i64),
+        ("n_slots", c_i32), ("dmax1", c_i32), ("n_prefix_vals", c_i32), ("n_prefix_slots", c_i32),
+        ("n_runs", c_i32), ("n_blocks", c_i32), ("max_n", c_i32), ("_pad", c_i32),
+        ("k_raw", c_i64),
+        ("slot_dim", c_vp), ("slot_deg", c_vp), ("slot_nodes", c_vp), ("slot_w", c_vp),
+        ("slot_q", c_vp), ("prefix_slot", c_vp), ("prefix_off", c_vp), ("run_slot", c_vp),
+        ("run_start", c_vp), ("blk_n", c_vp), ("blk_fac", c_vp), ("fac_row", c_vp),
+        ("fac_deg", c_vp), ("fac_slot", c_vp), ("blk_last_slot", c_vp), ("blk_foff", c_vp),
+        ("F", c_vp), ("offset", c_vp), ("wave_run", c_vp), ("n_split", c_i32), ("_pad2", c_i32),
+        ("dim_blk", c_vp), ("dim_blk_idx", c_vp), ("dim_blk_pos", c_vp),
+    ]
+
+
+def _sig(name, restype, argtypes):
+    fn = getattr(lib, name)
+    fn.restype = restype
+    fn.argtypes = argtypes
+    return fn
+
+
+# name -> (restype, argtypes); every symbol declared in include/smolyax_amd.h
+SIGNATURES = {
+    "sbi_last_error": (ctypes.c_char_p, []),
+    "sbi_version": (ctypes.c_char_p, []),
+    "sbi_indexset_cardinality": (c_i64, [P_f64, c_i64, c_f64]),
+    "sbi_smolyak_coefficient": (c_i64, [P_f64, c_i64, c_f64, c_i32]),
+    "sbi_nodeset_cardinality": (c_i64, [P_f64, c_i64, c_f64, c_i32]),
+    "sbi_find_approximate_threshold": (c_f64, [P_f64, c_i64, c_i64, c_i32, c_i32, c_f64]),
+    "sbi_indexset": (c_i32, [P_f64, c_i64, c_f64, P_i64, P_i64, P_i64, P_i32, P_i32]),
+    "sbi_nonzero_indices_and_zetas": (c_i32, [P_f64, c_i64, c_f64, P_i64, P_i64, P_i64, P_i32,
+                                             P_i32, P_i64]),
+    "sbi_bary_plan_create": (c_i32, [ctypes.POINTER(BaryDesc), ctypes.POINTER(c_vp)]),
+    "sbi_plan_destroy": (c_i32, [c_vp]),
+    "sbi_bary_eval_f64": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp]),
+    "sbi_bary_jacobian_f64": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp]),
+    "sbi_bary_integral_workspace": (c_i64, [c_vp]),
+    "sbi_bary_integral_f64": (c_i32, [c_vp, c_vp, c_vp, c_vp]),
+}
+
+fn = {name: _sig(name, r, a) for name, (r, a) in SIGNATURES.items()}
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = fn["sbi_last_error"]().decode(errors="replace")
+        raise NativeError(f"{what} failed (code {rc}): {msg}")
+
+
+def f64_ptr(a):
+    """numpy float64 array -> ctypes pointer (array must stay alive)."""
+    return a.ctypes.data_as(P_f64)
+
+
+def i64_ptr(a):
+    return a.ctypes.data_as(P_i64)
+
+
+def i32_ptr(a):
+    return a.ctypes.data_as(P_i32)

@@ -1,0 +1,530 @@
+// Barycentric combination-technique kernels (FP64) for gfx950.
+//
+// Math (reference: src/smolyax/barycentric.py:34-123, interpolation.py:264-304):
+//   y(x) = offset + sum_{blocks b} zeta_b sum_{mu <= nu_b} F_b[mu] prod_i Lhat^{(i)}_{mu_i}(x_{s_i})
+// where Lhat is the normalised 1D barycentric basis of a slot (dim, deg).  The normalised basis is
+// evaluated in the division-free first (modified Lagrange) form
+//   Lhat_mu(x) = w_mu * prod_{nu != mu} (x - xi_nu),   w_mu = 1 / prod_{nu != mu} (xi_mu - xi_nu),
+// which equals b_mu / sum(b) of the reference up to rounding; when x hits a node exactly the
+// result is the exact one-hot vector, as in barycentric.py:62-65.
+//
+// Layout and schedule (one tile = 64 points = one point per lane of a wave64):
+//   * every block's factors are its active dims in ascending order; the last (highest-dim)
+//     factor is "streamed", the others are "prefix" factors;
+//   * the 1D bases of all prefix slots for the tile's 64 points are computed once into LDS
+//     (row-major [table row][64 lanes], conflict-free ds_read_b64);
+//   * blocks are sorted by their last slot, so each run of blocks shares one streamed basis that
+//     is computed once per run into registers; all block metadata and F values are wave-uniform
+//     and arrive through scalar loads, so 64 points share every F read;
+//   * the 8 waves of a workgroup split the runs (cost-balanced on the host) and reduce their
+//     partial sums through LDS in a fixed order (deterministic, no atomics).
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "../../include/smolyax_amd.h"
+#include "common.h"
+
+namespace sbi {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+}  // namespace sbi
+
+#define SBI_HIP_CHECK(expr)                                                              \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess) {                                                              \
+      sbi::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));                 \
+      return SBI_ERR_HIP;                                                                \
+    }                                                                                    \
+  } while (0)
+
+namespace {
+
+constexpr int kTile = 64;       // points per tile (one per lane)
+constexpr int kWaves = 8;       // waves per workgroup sharing one tile
+constexpr int kLMax = 12;       // max (deg + 1) of a streamed factor held in registers
+constexpr int kMaxLds = 160 * 1024;
+
+// ------------------------------------------------------------------------------------------
+// 1D basis helpers
+// ------------------------------------------------------------------------------------------
+
+// Streamed factor: first-form basis of degree `deg` (< kLMax) into registers L[0..kLMax).
+// Entries beyond deg are zero.  Exact one-hot at nodes.
+__device__ __forceinline__ void basis_regs(double x, const double* __restrict__ xi,
+                                           const double* __restrict__ w, int deg,
+                                           double (&L)[kLMax]) {
+  double dlt[kLMax];
+#pragma unroll
+  for (int m = 0; m < kLMax; ++m) dlt[m] = (m <= deg) ? (x - xi[m]) : 1.0;
+  double suf[kLMax];
+  double s = 1.0;
+#pragma unroll
+  for (int m = kLMax - 1; m >= 0; --m) {
+    suf[m] = s;
+    s *= dlt[m];
+  }
+  double pre = 1.0;
+#pragma unroll
+  for (int m = 0; m < kLMax; ++m) {
+    double v = (m <= deg) ? w[m] * pre * suf[m] : 0.0;
+    if (m <= deg && dlt[m] == 0.0) v = 1.0;
+    L[m] = v;
+    pre *= dlt[m];
+  }
+}
+
+// Prefix slot: first-form basis of degree `deg` (any) written to LDS rows row0..row0+deg for this
+// lane.  Two passes: suffix products stored, then the forward pass finishes each value.
+__device__ __forceinline__ void basis_lds(double x, const double* __restrict__ xi,
+                                          const double* __restrict__ w, int deg,
+                                          double* __restrict__ tab, int row0, int lane) {
+  double s = 1.0;
+  for (int m = deg; m >= 0; --m) {
+    tab[(row0 + m) * kTile + lane] = s;
+    s *= (x - xi[m]);
+  }
+  double pre = 1.0;
+  int hit = -1;
+  for (int m = 0; m <= deg; ++m) {
+    const double dm = x - xi[m];
+    if (dm == 0.0) hit = m;
+    tab[(row0 + m) * kTile + lane] *= w[m] * pre;
+    pre *= dm;
+  }
+  if (hit >= 0) {
+    for (int m = 0; m <= deg; ++m) tab[(row0 + m) * kTile + lane] = (m == hit) ? 1.0 : 0.0;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Evaluation kernel
+// ------------------------------------------------------------------------------------------
+
+// Inner contraction over the streamed factor for OC outputs of one F row group:
+//   out[o] = sum_{m <= dl} F[(row + m) * d_out + o] * Ll[m]
+template <int OC>
+__device__ __forceinline__ void row_dot(const double* __restrict__ Frow, int64_t d_out, int oc_n,
+                                        int dl, const double (&Ll)[kLMax], double (&out)[OC]) {
+#pragma unroll
+  for (int o = 0; o < OC; ++o) out[o] = 0.0;
+#pragma unroll
+  for (int m = 0; m < kLMax; ++m) {
+    if (m <= dl) {
+#pragma unroll
+      for (int o = 0; o < OC; ++o) {
+        if (o < oc_n) out[o] = fma(Frow[m * d_out + o], Ll[m], out[o]);
+      }
+    }
+  }
+}
+
+template <int OC>
+__global__ __launch_bounds__(kTile* kWaves) void bary_eval_kernel(
+    sbi_bary_desc d, const double* __restrict__ x, int64_t P, int64_t ldx,
+    double* __restrict__ y) {
+  extern __shared__ double lds[];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t p_raw = (int64_t)blockIdx.x * kTile + lane;
+  const bool valid = p_raw < P;
+  const int64_t p = valid ? p_raw : (P - 1);
+  const int oc0 = blockIdx.y * OC;
+  const int oc_n = (int)min<int64_t>(OC, d.d_out - oc0);
+  const double* __restrict__ xp = x + p * ldx;
+
+  // 1. prefix table
+  for (int s = wave; s < d.n_prefix_slots; s += kWaves) {
+    const int slot = d.prefix_slot[s];
+    const int row = d.prefix_off[s];
+    const int deg = d.slot_deg[slot];
+    const double xv = xp[d.slot_dim[slot]];
+    basis_lds(xv, d.slot_nodes + (int64_t)slot * d.dmax1, d.slot_w + (int64_t)slot * d.dmax1, deg,
+              lds, row, lane);
+  }
+  __syncthreads();
+
+  double acc[OC];
+#pragma unroll
+  for (int o = 0; o < OC; ++o) acc[o] = 0.0;
+
+  const int r_begin = d.wave_run[wave];
+  const int r_end = d.wave_run[wave + 1];
+  for (int r = r_begin; r < r_end; ++r) {
+    const int ls = d.run_slot[r];
+    double Ll[kLMax];
+    int dl;
+    if (ls >= 0) {
+      dl = d.slot_deg[ls];
+      const double xv = xp[d.slot_dim[ls]];
+      basis_regs(xv, d.slot_nodes + (int64_t)ls * d.dmax1, d.slot_w + (int64_t)ls * d.dmax1, dl,
+                 Ll);
+    } else {
+      dl = 0;
+#pragma unroll
+      for (int m = 0; m < kLMax; ++m) Ll[m] = (m == 0) ? 1.0 : 0.0;
+    }
+    const int b_end = d.run_start[r + 1];
+    for (int b = d.run_start[r]; b < b_end; ++b) {
+      const int f0 = d.blk_fac[b];
+      const int npre = d.blk_fac[b + 1] - f0;
+      const double* __restrict__ Fb = d.F + d.blk_foff[b] * d.d_out + oc0;
+      const int64_t rs = (int64_t)(dl + 1) * d.d_out;  // stride between prefix entries
+      double inner[OC];
+      if (npre == 0) {
+        row_dot<OC>(Fb, d.d_out, oc_n, dl, Ll, inner);
+#pragma unroll
+        for (int o = 0; o < OC; ++o) acc[o] += inner[o];
+      } else if (npre == 1) {
+        const int r1 = d.fac_row[f0], d1 = d.fac_deg[f0];
+        for (int m1 = 0; m1 <= d1; ++m1) {
+          const double p1 = lds[(r1 + m1) * kTile + lane];
+          row_dot<OC>(Fb + m1 * rs, d.d_out, oc_n, dl, Ll, inner);
+#pragma unroll
+          for (int o = 0; o < OC; ++o) acc[o] = fma(p1, inner[o], acc[o]);
+        }
+      } else if (npre == 2) {
+        const int r1 = d.fac_row[f0], d1 = d.fac_deg[f0];
+        const int r2 = d.fac_row[f0 + 1], d2 = d.fac_deg[f0 + 1];
+        int e = 0;
+        for (int m1 = 0; m1 <= d1; ++m1) {
+          const double p1 = lds[(r1 + m1) * kTile + lane];
+          for (int m2 = 0; m2 <= d2; ++m2, ++e) {
+            const double p2 = p1 * lds[(r2 + m2) * kTile + lane];
+            row_dot<OC>(Fb + e * rs, d.d_out, oc_n, dl, Ll, inner);
+#pragma unroll
+            for (int o = 0; o < OC; ++o) acc[o] = fma(p2, inner[o], acc[o]);
+          }
+        }
+      } else if (npre == 3) {
+        const int r1 = d.fac_row[f0], d1 = d.fac_deg[f0];
+        const int r2 = d.fac_row[f0 + 1], d2 = d.fac_deg[f0 + 1];
+        const int r3 = d.fac_row[f0 + 2], d3 = d.fac_deg[f0 + 2];
+        int e = 0;
+        for (int m1 = 0; m1 <= d1; ++m1) {
+          const double p1 = lds[(r1 + m1) * kTile + lane];
+          for (int m2 = 0; m2 <= d2; ++m2) {
+            const double p2 = p1 * lds[(r2 + m2) * kTile + lane];
+            for (int m3 = 0; m3 <= d3; ++m3, ++e) {
+              const double p3 = p2 * lds[(r3 + m3) * kTile + lane];
+              row_dot<OC>(Fb + e * rs, d.d_out, oc_n, dl, Ll, inner);
+#pragma unroll
+              for (int o = 0; o < OC; ++o) acc[o] = fma(p3, inner[o], acc[o]);
+            }
+          }
+        }
+      } else {
+        // generic odometer: product recomputed per prefix entry
+        int64_t nent = 1;
+        for (int i = 0; i < npre; ++i) nent *= (d.fac_deg[f0 + i] + 1);
+        for (int64_t e = 0; e < nent; ++e) {
+          int64_t rem = e;
+          double pw = 1.0;
+          for (int i = npre - 1; i >= 0; --i) {
+            const int di1 = d.fac_deg[f0 + i] + 1;
+            const int mi = (int)(rem % di1);
+            rem /= di1;
+            pw *= lds[(d.fac_row[f0 + i] + mi) * kTile + lane];
+          }
+          row_dot<OC>(Fb + e * rs, d.d_out, oc_n, dl, Ll, inner);
+#pragma unroll
+          for (int o = 0; o < OC; ++o) acc[o] = fma(pw, inner[o], acc[o]);
+        }
+      }
+    }
+  }
+
+  // 2. deterministic cross-wave reduction (table area is reused)
+  __syncthreads();
+#pragma unroll
+  for (int o = 0; o < OC; ++o) lds[(wave * OC + o) * kTile + lane] = acc[o];
+  __syncthreads();
+  if (wave == 0 && valid) {
+#pragma unroll
+    for (int o = 0; o < OC; ++o) {
+      if (o < oc_n) {
+        double s = d.offset[oc0 + o];
+        for (int w = 0; w < kWaves; ++w) s += lds[(w * OC + o) * kTile + lane];
+        y[p * d.d_out + oc0 + o] = s;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Jacobian kernel (parity path of `gradient`): one wave per (tile, dim j); loops over the blocks
+// that contain j; every (p, o, j) entry is written by exactly one lane.
+// ------------------------------------------------------------------------------------------
+constexpr int kJMaxN = 8;    // max active dims per block supported by the jacobian kernel
+constexpr int kJDm = 16;     // max (deg + 1) per factor in the jacobian kernel
+
+// Basis (first form) and, for the differentiated factor, its derivative
+//   dL_mu = w_mu sum_{k != mu} prod_{nu != mu, k} (x - xi_nu)
+// into LDS rows [f][m][lane].  The derivative is NaN if x is a node of the slot, which is the
+// reference's result there (barycentric.py:152-154 turns a 0 denominator into NaN).
+__device__ __forceinline__ void basis_and_deriv_lds(double x, const double* __restrict__ xi,
+                                                    const double* __restrict__ w, int deg,
+                                                    bool want_d, double* __restrict__ Lrow,
+                                                    double* __restrict__ dLrow, int lane) {
+  int hit = -1;
+  for (int m = 0; m <= deg; ++m)
+    if (x - xi[m] == 0.0) hit = m;
+  for (int m = 0; m <= deg; ++m) {
+    double v = w[m];
+    for (int q = 0; q <= deg; ++q)
+      if (q != m) v *= (x - xi[q]);
+    Lrow[m * kTile + lane] = (hit >= 0) ? (m == hit ? 1.0 : 0.0) : v;
+    if (want_d) {
+      double dv = 0.0;
+      for (int k = 0; k <= deg; ++k) {
+        if (k == m) continue;
+        double pr = w[m];
+        for (int q = 0; q <= deg; ++q)
+          if (q != m && q != k) pr *= (x - xi[q]);
+        dv += pr;
+      }
+      dLrow[m * kTile + lane] = (hit >= 0) ? NAN : dv;
+    }
+  }
+}
+
+__global__ __launch_bounds__(64) void bary_jacobian_kernel(sbi_bary_desc d,
+                                                           const double* __restrict__ x, int64_t P,
+                                                           int64_t ldx, double* __restrict__ J) {
+  __shared__ double Ls[kJMaxN * kJDm * kTile];
+  __shared__ double dLs[kJDm * kTile];
+  const int lane = threadIdx.x;
+  const int j = blockIdx.y;
+  const int64_t p_raw = (int64_t)blockIdx.x * kTile + lane;
+  const bool valid = p_raw < P;
+  const int64_t p = valid ? p_raw : (P - 1);
+  const double* __restrict__ xp = x + p * ldx;
+  const int64_t dout = d.d_out;
+
+  if (valid)
+    for (int64_t o = 0; o < dout; ++o) J[(p * dout + o) * d.d_in + j] = 0.0;
+
+  for (int q = d.dim_blk[j]; q < d.dim_blk[j + 1]; ++q) {
+    const int b = d.dim_blk_idx[q];
+    const int pos = d.dim_blk_pos[q];
+    const int f0 = d.blk_fac[b];
+    const int npre = d.blk_fac[b + 1] - f0;
+    const int ls = d.blk_last_slot[b];
+    const int nf = npre + (ls >= 0 ? 1 : 0);
+    int64_t nent = 1;
+    for (int i = 0; i < nf; ++i) {
+      const int s = (i < npre) ? d.fac_slot[f0 + i] : ls;
+      const int deg = d.slot_deg[s];
+      nent *= (deg + 1);
+      basis_and_deriv_lds(xp[d.slot_dim[s]], d.slot_nodes + (int64_t)s * d.dmax1,
+                          d.slot_w + (int64_t)s * d.dmax1, deg, i == pos,
+                          Ls + i * kJDm * kTile, dLs, lane);
+    }
+    // entries in C order over factors (last factor fastest)
+    const double* __restrict__ Fb = d.F + d.blk_foff[b] * dout;
+    for (int64_t e = 0; e < nent; ++e) {
+      int64_t rem = e;
+      double wgt = 1.0;
+      for (int i = nf - 1; i >= 0; --i) {
+        const int s = (i < npre) ? d.fac_slot[f0 + i] : ls;
+        const int di1 = d.slot_deg[s] + 1;
+        const int mi = (int)(rem % di1);
+        rem /= di1;
+        wgt *= (i == pos) ? dLs[mi * kTile + lane] : Ls[(i * kJDm + mi) * kTile + lane];
+      }
+      if (valid)
+        for (int64_t o = 0; o < dout; ++o) J[(p * dout + o) * d.d_in + j] += wgt * Fb[e * dout + o];
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Integral: partial[c][o] = sum over blocks of chunk c of <F_b[:, o], (x)q>, then a fixed-order
+// sum over chunks.  F is streamed once, coalesced across o.
+// ------------------------------------------------------------------------------------------
+constexpr int kIntChunks = 512;
+constexpr int kIntThreads = 256;
+
+__global__ __launch_bounds__(kIntThreads) void bary_integral_partial_kernel(
+    sbi_bary_desc d, double* __restrict__ partial) {
+  const int c = blockIdx.x;
+  const int64_t o = (int64_t)blockIdx.y * kIntThreads + threadIdx.x;
+  const int nb = d.n_blocks;
+  const int b0 = (int)((int64_t)nb * c / kIntChunks);
+  const int b1 = (int)((int64_t)nb * (c + 1) / kIntChunks);
+  double acc = 0.0;
+  for (int b = b0; b < b1; ++b) {
+    const int f0 = d.blk_fac[b];
+    const int npre = d.blk_fac[b + 1] - f0;
+    const int ls = d.blk_last_slot[b];
+    const int nf = npre + (ls >= 0 ? 1 : 0);
+    int64_t nent = 1;
+    for (int i = 0; i < npre; ++i) nent *= (d.fac_deg[f0 + i] + 1);
+    if (ls >= 0) nent *= (d.slot_deg[ls] + 1);
+    const double* __restrict__ Fb = d.F + d.blk_foff[b] * d.d_out;
+    for (int64_t e = 0; e < nent; ++e) {
+      int64_t rem = e;
+      double wgt = 1.0;
+      for (int i = nf - 1; i >= 0; --i) {
+        const int s = (i < npre) ? d.fac_slot[f0 + i] : ls;
+        const int di1 = d.slot_deg[s] + 1;
+        const int mi = (int)(rem % di1);
+        rem /= di1;
+        wgt *= d.slot_q[(int64_t)s * d.dmax1 + mi];
+      }
+      if (o < d.d_out) acc = fma(wgt, Fb[e * d.d_out + o], acc);
+    }
+  }
+  if (o < d.d_out) partial[(int64_t)c * d.d_out + o] = acc;
+}
+
+__global__ __launch_bounds__(kIntThreads) void bary_integral_final_kernel(
+    int64_t d_out, const double* __restrict__ offset, const double* __restrict__ partial,
+    double* __restrict__ Q) {
+  const int64_t o = (int64_t)blockIdx.x * kIntThreads + threadIdx.x;
+  if (o >= d_out) return;
+  double s = offset[o];
+  for (int c = 0; c < kIntChunks; ++c) s += partial[(int64_t)c * d_out + o];
+  Q[o] = s;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------------
+struct sbi_plan {
+  int kind;  // 1 = barycentric
+  sbi_bary_desc bary;
+  size_t eval_lds_bytes;
+};
+
+extern "C" {
+
+const char* sbi_last_error(void) { return sbi::g_err.c_str(); }
+const char* sbi_version(void) { return "smolyax_amd 0.1 (gfx950)"; }
+
+int sbi_bary_plan_create(const sbi_bary_desc* desc, sbi_plan** out) {
+  if (!desc || !out) {
+    sbi::set_error("sbi_bary_plan_create: null argument");
+    return SBI_ERR_INVALID;
+  }
+  const sbi_bary_desc& d = *desc;
+  if (d.d_in <= 0 || d.d_out <= 0 || d.n_slots < 0 || d.n_blocks < 0 || d.n_runs < 0 ||
+      d.dmax1 <= 0 || !d.F || !d.offset || !d.wave_run || !d.run_start || !d.blk_fac ||
+      !d.blk_foff) {
+    sbi::set_error("sbi_bary_plan_create: invalid descriptor");
+    return SBI_ERR_INVALID;
+  }
+  if (d.n_split != kWaves) {
+    sbi::set_error("sbi_bary_plan_create: n_split must equal the waves per workgroup (8)");
+    return SBI_ERR_INVALID;
+  }
+  if (d.dmax1 < kLMax) {
+    sbi::set_error("sbi_bary_plan_create: slot rows must be padded to at least 12 entries");
+    return SBI_ERR_INVALID;
+  }
+  const size_t table = (size_t)d.n_prefix_vals * kTile * sizeof(double);
+  const size_t red = (size_t)kWaves * 8 * kTile * sizeof(double);
+  const size_t lds = table > red ? table : red;
+  if (lds > (size_t)kMaxLds) {
+    sbi::set_error("sbi_bary_plan_create: prefix basis table exceeds LDS (" +
+                   std::to_string(d.n_prefix_vals) + " rows > 320)");
+    return SBI_ERR_CAPACITY;
+  }
+  sbi_plan* p = new sbi_plan();
+  p->kind = 1;
+  p->bary = d;
+  p->eval_lds_bytes = lds;
+  *out = p;
+  return SBI_OK;
+}
+
+int sbi_plan_destroy(sbi_plan* plan) {
+  delete plan;
+  return SBI_OK;
+}
+
+}  // extern "C"
+
+template <int OC>
+static int launch_eval(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx, double* y,
+                       hipStream_t st) {
+  const sbi_bary_desc& d = plan->bary;
+  dim3 grid((unsigned)((P + kTile - 1) / kTile), (unsigned)((d.d_out + OC - 1) / OC));
+  static std::once_flag once;
+  std::call_once(once, [] {
+    (void)hipFuncSetAttribute((const void*)bary_eval_kernel<OC>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
+  });
+  hipLaunchKernelGGL(bary_eval_kernel<OC>, grid, dim3(kTile * kWaves), plan->eval_lds_bytes, st, d,
+                     x, P, ldx, y);
+  SBI_HIP_CHECK(hipGetLastError());
+  return SBI_OK;
+}
+
+extern "C" int sbi_bary_eval_f64(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx,
+                                 double* y, void* stream) {
+  if (!plan || plan->kind != 1 || !x || !y || P < 0 || ldx < plan->bary.d_in) {
+    sbi::set_error("sbi_bary_eval_f64: invalid argument");
+    return SBI_ERR_INVALID;
+  }
+  if (P == 0) return SBI_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t dout = plan->bary.d_out;
+  if (dout == 1) return launch_eval<1>(plan, x, P, ldx, y, st);
+  if (dout == 2) return launch_eval<2>(plan, x, P, ldx, y, st);
+  if (dout <= 4) return launch_eval<4>(plan, x, P, ldx, y, st);
+  return launch_eval<8>(plan, x, P, ldx, y, st);
+}
+
+extern "C" int sbi_bary_jacobian_f64(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx,
+                                     double* J, void* stream) {
+  if (!plan || plan->kind != 1 || !x || !J || P < 0 || ldx < plan->bary.d_in ||
+      !plan->bary.blk_last_slot || !plan->bary.dim_blk) {
+    sbi::set_error("sbi_bary_jacobian_f64: invalid argument");
+    return SBI_ERR_INVALID;
+  }
+  if (P == 0) return SBI_OK;
+  const sbi_bary_desc& d = plan->bary;
+  dim3 grid((unsigned)((P + kTile - 1) / kTile), (unsigned)d.d_in);
+  if (d.max_n > kJMaxN || d.dmax1 > kJDm) {
+    sbi::set_error("sbi_bary_jacobian_f64: supports at most 8 active dims and degree 15");
+    return SBI_ERR_UNSUPPORTED;
+  }
+  hipLaunchKernelGGL(bary_jacobian_kernel, grid, dim3(64), 0, (hipStream_t)stream, d, x, P, ldx,
+                     J);
+  SBI_HIP_CHECK(hipGetLastError());
+  return SBI_OK;
+}
+
+extern "C" int64_t sbi_bary_integral_workspace(const sbi_plan* plan) {
+  if (!plan) return -1;
+  return (int64_t)kIntChunks * plan->bary.d_out;
+}
+
+extern "C" int sbi_bary_integral_f64(const sbi_plan* plan, double* Q, double* partial,
+                                     void* stream) {
+  if (!plan || plan->kind != 1 || !Q || !partial || !plan->bary.blk_last_slot || !plan->bary.slot_q) {
+    sbi::set_error("sbi_bary_integral_f64: invalid argument");
+    return SBI_ERR_INVALID;
+  }
+  const sbi_bary_desc& d = plan->bary;
+  hipStream_t st = (hipStream_t)stream;
+  dim3 g1(kIntChunks, (unsigned)((d.d_out + kIntThreads - 1) / kIntThreads));
+  hipLaunchKernelGGL(bary_integral_partial_kernel, g1, dim3(kIntThreads), 0, st, d, partial);
+  SBI_HIP_CHECK(hipGetLastError());
+  dim3 g2((unsigned)((d.d_out + kIntThreads - 1) / kIntThreads));
+  hipLaunchKernelGGL(bary_integral_final_kernel, g2, dim3(kIntThreads), 0, st, d.d_out, d.offset,
+                     partial, Q);
+  SBI_HIP_CHECK(hipGetLastError());
+  return SBI_OK;
+}

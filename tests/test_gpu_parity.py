@@ -1,0 +1,173 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the FP64 oracle.
+
+Tolerances: FP64 evaluation, gradient and integral agree with the oracle to <= 1e-12 relative
+(max |diff| / max |y|).  The oracle follows the reference's formula; the kernels use the first
+(modified Lagrange) form, so agreement is to rounding, amplified by the Smolyak coefficients.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import indices as OI
+from oracle import nodes as ON
+from oracle import smolyak as OS
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def build_pair(d, d_out, m, f, kind="leja", seed=0, custom=False):
+    from smolyax_amd import nodes as SN
+    from smolyax_amd.interpolation import SmolyakBarycentricInterpolator
+
+    rng = np.random.default_rng(seed)
+    k = np.log((2 + np.arange(d)) / np.log(2))
+    if kind == "leja":
+        if custom:
+            dom = np.sort(rng.random((d, 2)), axis=1)
+            dom[:, 1] += 0.1
+            ong, sng = ON.leja(domains=dom), SN.Leja(domains=dom)
+        else:
+            ong, sng = ON.leja(dim=d), SN.Leja(dim=d)
+    else:
+        if custom:
+            mean, sc = rng.standard_normal(d), rng.random(d) + 0.3
+            ong, sng = ON.gauss_hermite(mean=mean, scaling=sc), SN.GaussHermite(mean, sc)
+        else:
+            ong, sng = ON.gauss_hermite(dim=d), SN.GaussHermite(dim=d)
+    t = OI.find_approximate_threshold(k, m, ong.is_nested)
+    ref = OS.Interpolant(d_out=d_out, node_gen=ong, k=k, t=t, f=f)
+    ip = SmolyakBarycentricInterpolator(d_out=d_out, node_gen=sng, k=k, t=t, f=f)
+    return ref, ip, ong
+
+
+def sample(ong, P, seed):
+    rng = np.random.default_rng(seed)
+    if ong.is_nested:
+        x = rng.uniform(-1, 1, (P, ong.dim))
+    else:
+        x = rng.standard_normal((P, ong.dim)) / np.sqrt(2)
+    return np.stack([g.scale(x[:, j]) for j, g in enumerate(ong)], axis=1)
+
+
+CASES = [
+    # d, d_out, m, kind, custom
+    (1, 1, 9, "leja", False),
+    (3, 2, 60, "leja", True),
+    (5, 3, 150, "gh", False),
+    (4, 5, 120, "gh", True),
+    (10, 1, 1000, "leja", False),
+    (8, 9, 400, "leja", False),
+    (30, 2, 600, "leja", False),
+]
+
+
+@pytest.mark.parametrize("d,d_out,m,kind,custom", CASES)
+def test_eval_parity(gpu, d, d_out, m, kind, custom):
+    rng = np.random.default_rng(d * 100 + d_out)
+    A = rng.standard_normal((d_out, d)) / (np.arange(d) + 1.0)
+    f = lambda x: np.cos(A @ x + 0.2) + 0.1 * (A @ x) ** 2
+    ref, ip, ong = build_pair(d, d_out, m, f, kind, seed=d, custom=custom)
+    x = sample(ong, 333, seed=7)
+    y = ip(x).cpu().numpy()
+    yr = ref(x)
+    assert y.shape == yr.shape == (333, d_out)
+    assert rel(y, yr) <= TOL, rel(y, yr)
+    # single point (d_in,) input promoted to (1, d_in)
+    y1 = ip(x[5]).cpu().numpy()
+    assert y1.shape == (1, d_out)
+    assert rel(y1, yr[5:6]) <= TOL
+
+
+def test_eval_at_grid_nodes(gpu):
+    """One-hot branch: at sparse-grid nodes the interpolant reproduces f."""
+    d, d_out = 4, 2
+    f = lambda x: np.array([np.exp(np.sum(x) / 3), np.sin(x[0] - x[3])])
+    ref, ip, ong = build_pair(d, d_out, 80, f)
+    k = np.log((2 + np.arange(d)) / np.log(2))
+    t = OI.find_approximate_threshold(k, 80, True)
+    lam = OI.indexset(k, t)
+    xi = ong[0](12)
+    x = np.array([[xi[dict(nu).get(j, 0)] for j in range(d)] for nu in lam])
+    y = ip(x).cpu().numpy()
+    assert rel(y, ref(x)) <= TOL
+    fx = np.array([f(r) for r in x])
+    assert np.max(np.abs(y - fx)) < 1e-12
+
+
+@pytest.mark.parametrize("d,d_out,m,kind,custom", CASES[:6])
+def test_gradient_parity(gpu, d, d_out, m, kind, custom):
+    rng = np.random.default_rng(d)
+    A = rng.standard_normal((d_out, d))
+    f = lambda x: np.tanh(A @ x / d)
+    ref, ip, ong = build_pair(d, d_out, m, f, kind, seed=d + 1, custom=custom)
+    x = sample(ong, 70, seed=11)
+    J = ip.gradient(x).cpu().numpy()
+    Jr = ref.gradient(x)
+    assert J.shape == Jr.shape == (70, d_out, d)
+    assert rel(J, Jr) <= 1e-11, rel(J, Jr)
+
+
+def test_gradient_nan_at_nodes(gpu):
+    d = 3
+    f = lambda x: np.array([x[0] * x[1] + x[2] ** 2])
+    ref, ip, ong = build_pair(d, 1, 30, f)
+    x = np.array([[0.3, 0.0, -0.2], [0.25, 0.5, 0.125]])
+    J = ip.gradient(x).cpu().numpy()
+    Jr = ref.gradient(x)
+    assert np.array_equal(np.isnan(J), np.isnan(Jr))
+    assert np.isnan(J[0, 0, 1]) and not np.isnan(J[1]).any()
+    ok = ~np.isnan(Jr)
+    assert rel(J[ok], Jr[ok]) <= 1e-12
+
+
+@pytest.mark.parametrize("d,d_out,m,kind,custom", CASES)
+def test_integral_parity(gpu, d, d_out, m, kind, custom):
+    rng = np.random.default_rng(d + 5)
+    A = rng.standard_normal((d_out, d))
+    f = lambda x: np.exp(-0.3 * (A @ x) ** 2)
+    ref, ip, ong = build_pair(d, d_out, m, f, kind, seed=d + 2, custom=custom)
+    Q = ip.integral().cpu().numpy()
+    Qr = ref.integral()
+    assert Q.shape == (d_out,)
+    assert rel(Q, Qr) <= TOL, rel(Q, Qr)
+
+
+def test_config1_parity(gpu):
+    """Plain FP64 CPU-runnable workload: d_in=10, |Lambda|~1000, 1024 points from seed 0."""
+    from smolyax_amd import workloads as W
+
+    d = 10
+    f = W.f_rational(d)
+    ref, ip, ong = build_pair(d, 1, 1000, f)
+    x = W.points_uniform(1024, d, seed=0)
+    assert rel(ip(x).cpu().numpy(), ref(x)) <= TOL
+
+
+def test_f_evals_reuse(gpu):
+    """set_f returns / accepts the reference's f_evals dict; reuse makes no new calls."""
+    from smolyax_amd import nodes as SN
+    from smolyax_amd.interpolation import SmolyakBarycentricInterpolator
+
+    d = 5
+    k = np.log((2 + np.arange(d)) / np.log(2))
+    calls = []
+    f = lambda x: calls.append(1) or np.sum(x ** 2)
+    for ng in (SN.Leja(dim=d), SN.GaussHermite(dim=d)):
+        t = OI.find_approximate_threshold(k, 100, ng.is_nested)
+        ip = SmolyakBarycentricInterpolator(d_out=1, node_gen=ng, k=k, t=t)
+        calls.clear()
+        fe = ip.set_f(f=f)
+        assert len(calls) == ip.n_f_evals_new == ip.n_f_evals
+        ip2 = SmolyakBarycentricInterpolator(d_out=1, node_gen=ng, k=k, t=t)
+        calls.clear()
+        ip2.set_f(f=f, f_evals=fe)
+        assert len(calls) == 0 and ip2.n_f_evals_new == 0
+        x = np.random.default_rng(0).uniform(-0.5, 0.5, (9, d))
+        assert rel(ip(x).cpu().numpy(), ip2(x).cpu().numpy()) == 0.0

@@ -1,0 +1,83 @@
+"""The native library: it loads, exports every declared symbol, and its host-side index machinery
+is bit-identical to the oracle (no GPU needed)."""
+
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    hdr = open(os.path.join(ROOT, "include", "smolyax_amd.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    return sorted(set(re.findall(r"\b(sbi_[a-z0-9_]+)\s*\(", hdr)))
+
+
+def test_library_exports_all_declared_symbols():
+    from smolyax_amd import _native as N
+
+    out = subprocess.run(["nm", "-D", "--defined-only", N.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r" T (sbi_\w+)", out))
+    missing = [s for s in declared_symbols() if s not in exported]
+    assert not missing, missing
+    assert set(declared_symbols()) <= set(N.SIGNATURES), "ctypes signatures incomplete"
+
+
+def test_error_reporting():
+    import ctypes
+
+    from smolyax_amd import _native as N
+
+    plan = ctypes.c_void_p()
+    rc = N.fn["sbi_bary_plan_create"](None, ctypes.byref(plan))
+    assert rc == -1
+    assert b"null" in N.fn["sbi_last_error"]()
+
+
+def k_rule(d):
+    return np.array([np.log((2 + j) / np.log(2)) for j in range(d)])
+
+
+@pytest.mark.parametrize("d,m,nested", [(10, 1000, True), (200, 5000, True), (100, 10000, True),
+                                         (20, 800, False), (8, 2000, False)])
+def test_native_indices_bit_exact(d, m, nested):
+    from oracle import indices as OI
+    from smolyax_amd import indices as SI
+
+    k = k_rule(d)
+    t_o = OI.find_approximate_threshold(k, m, nested)
+    t_n = SI.find_approximate_threshold(k, m, nested)
+    assert t_o == t_n  # bitwise equal double
+    assert SI.nodeset_cardinality(k, t_o, nested) == OI.nodeset_cardinality(k, t_o, nested)
+    assert SI.indexset_cardinality(k, t_o) == OI.indexset_cardinality(k, t_o)
+    assert SI.indexset(k, t_o) == OI.indexset(k, t_o)
+    a, b = SI.non_zero_indices_and_zetas(k, t_o), OI.non_zero_indices_and_zetas(k, t_o)
+    assert dict(a[0]) == dict(b[0]) and dict(a[1]) == dict(b[1])
+    assert list(a[0].keys()) == list(b[0].keys())
+
+
+def test_native_known_cardinalities():
+    from smolyax_amd import indices as SI
+
+    k = np.log([2 + i for i in range(10000)]) / np.log(2)
+    for t, m in [(5.3, 98), (7.78, 1000), (10.317, 10003)]:
+        assert SI.indexset_cardinality(k, t) == m
+        assert len(SI.indexset(k, t)) == m
+
+
+def test_native_smolyak_coefficient_random():
+    from oracle import indices as OI
+    from smolyax_amd import indices as SI
+
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        d = int(rng.integers(1, 12))
+        k = np.sort(rng.uniform(0.5, 3, d))
+        rem = rng.uniform(0, 6)
+        p = int(rng.integers(0, 2))
+        assert SI.smolyak_coefficient(k, d, rem, p) == OI.smolyak_coefficient(k, d, rem, p)
