@@ -102,9 +102,17 @@ typedef struct sbi_bary_desc {
   const double* F;               /* [k_raw * d_out] zeta-scaled values, row-major, block entries
                                     in C order over (prefix factors..., last factor) */
   const double* offset;          /* [d_out] zeta_0 * f(xi_0) */
-  const int32_t* wave_run;       /* [n_split + 1] run ranges per wave (cost-balanced) */
-  int32_t n_split;               /* waves per point tile */
-  int32_t _pad2;
+  const int32_t* blk_desc;       /* [4 * n_blocks] packed block record (eval kernel):
+                                    [0] F row offset; [1] npre | dl<<3 | d1<<8 | d2<<13 | d3<<18
+                                    (npre = 7: more than 3 prefix factors, read fac_*);
+                                    [2] row1 | row2<<16; [3] row3 | (streamed slot + 1)<<16 */
+  const int32_t* seg_off;        /* [n_split + 1] per wave: range of its segments */
+  const int32_t* seg_slot;       /* [n_seg] streamed slot of a segment (-1: none) */
+  const int32_t* seg_b0;         /* [n_seg] first block of a segment */
+  const int32_t* seg_b1;         /* [n_seg] one past the last block of a segment */
+  int32_t n_split;               /* waves per point tile (16); segments = runs cut at wave
+                                    boundaries, cost-balanced over the waves */
+  int32_t n_seg;
   const int32_t* dim_blk;        /* [d_in + 1] CSR: blocks containing each dim (jacobian) */
   const int32_t* dim_blk_idx;    /* block ids, and the factor position in dim_blk_pos */
   const int32_t* dim_blk_pos;    /*   (0..n-1, n-1 = last factor) */
@@ -132,6 +140,50 @@ int sbi_bary_jacobian_f64(const sbi_plan* plan, const double* x, int64_t P, int6
  * `partial` is caller-provided device workspace of sbi_bary_integral_workspace(plan) doubles. */
 int64_t sbi_bary_integral_workspace(const sbi_plan* plan);
 int sbi_bary_integral_f64(const sbi_plan* plan, double* Q, double* partial, void* stream);
+
+
+/* ----------------------------------------------------------------------------------------------
+ * Hierarchical (Newton-surplus) form, nested nodes only (Leja).  Same interpolant as the
+ * combination technique of interpolation.py:264-304, written as
+ *   y(x) = s_root + sum_{nu != 0} s_nu prod_{(j,k) in nu} h^{(j)}_k(x_j),
+ *   h^{(j)}_k(x) = c_k prod_{r<k} (x - xi_r).
+ * It has no counterpart function in the reference: it is the mixed-precision path of
+ * __call__ (FP32 surpluses / products / group sums, FP64 node differences and accumulation).
+ * -------------------------------------------------------------------------------------------- */
+typedef struct sbi_newton_desc {
+  int64_t d_in, d_out;        /* d_out must be 1 for this kernel */
+  int32_t n_nodes;            /* nodes nu != 0, in (dim, deg) group order */
+  int32_t n_internal;         /* nodes with children: rows 1..n_internal of the on-chip table */
+  int32_t n_groups;           /* (dim j, deg k) groups of nodes whose last factor is (j, k) */
+  int32_t n_split;            /* waves per point tile (16) */
+  int32_t n_levels;           /* depth levels of the internal nodes */
+  int32_t kmax1;              /* row stride of dim_xi / dim_c (max deg + 1) */
+  const double* s_root;       /* [d_out] surplus of nu = 0 */
+  const int32_t* int_parent;  /* [n_internal] table row of the parent (0 = root) */
+  const int32_t* int_dim;     /* [n_internal] dim of the node's last factor */
+  const int32_t* int_deg;     /* [n_internal] deg of the node's last factor */
+  const int32_t* level_off;   /* [n_levels + 1] internal nodes grouped by depth */
+  const double* dim_xi;       /* [d_in * kmax1] nodes of each dim's sequence */
+  const double* dim_c;        /* [d_in * kmax1] Newton constants c_k = 1/prod_{r<k}(xi_k-xi_r) */
+  const int32_t* grp;         /* [4 * n_groups] first node, end node, dim, deg */
+  const int32_t* grp_next_dim;/* [n_groups] next different dim in the same wave (-1: none) */
+  const int32_t* wave_grp;    /* [n_split + 1] group ranges per wave (cost-balanced) */
+  const float* node_s32;      /* [n_nodes] surpluses, FP32 (mixed path) */
+  const double* node_s64;     /* [n_nodes] surpluses, FP64 */
+  const int32_t* node_q;      /* [n_nodes] table row of the node's parent */
+} sbi_newton_desc;
+
+typedef struct sbi_newton_plan sbi_newton_plan;
+
+int sbi_newton_plan_create(const sbi_newton_desc* desc, sbi_newton_plan** out);
+int sbi_newton_plan_destroy(sbi_newton_plan* plan);
+/* y[P] (FP64) at x[P, ldx]: FP32 surpluses, products and per-group sums; FP64 node differences
+ * x - xi, Newton factors and accumulation across groups. */
+int sbi_newton_eval_mixed(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
+                          double* y, void* stream);
+/* Same in FP64 throughout. */
+int sbi_newton_eval_f64(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
+                        double* y, void* stream);
 
 #ifdef __cplusplus
 }

@@ -36,8 +36,21 @@ class BaryDesc(ctypes.Structure):
         ("slot_q", c_vp), ("prefix_slot", c_vp), ("prefix_off", c_vp), ("run_slot", c_vp),
         ("run_start", c_vp), ("blk_n", c_vp), ("blk_fac", c_vp), ("fac_row", c_vp),
         ("fac_deg", c_vp), ("fac_slot", c_vp), ("blk_last_slot", c_vp), ("blk_foff", c_vp),
-        ("F", c_vp), ("offset", c_vp), ("wave_run", c_vp), ("n_split", c_i32), ("_pad2", c_i32),
+        ("F", c_vp), ("offset", c_vp), ("blk_desc", c_vp), ("seg_off", c_vp), ("seg_slot", c_vp),
+        ("seg_b0", c_vp), ("seg_b1", c_vp), ("n_split", c_i32), ("n_seg", c_i32),
         ("dim_blk", c_vp), ("dim_blk_idx", c_vp), ("dim_blk_pos", c_vp),
+    ]
+
+
+class NewtonDesc(ctypes.Structure):
+    _fields_ = [
+        ("d_in", c_i64), ("d_out", c_i64),
+        ("n_nodes", c_i32), ("n_internal", c_i32), ("n_groups", c_i32), ("n_split", c_i32),
+        ("n_levels", c_i32), ("kmax1", c_i32),
+        ("s_root", c_vp), ("int_parent", c_vp), ("int_dim", c_vp), ("int_deg", c_vp),
+        ("level_off", c_vp), ("dim_xi", c_vp), ("dim_c", c_vp), ("grp", c_vp),
+        ("grp_next_dim", c_vp), ("wave_grp", c_vp), ("node_s32", c_vp), ("node_s64", c_vp),
+        ("node_q", c_vp),
     ]
 
 
@@ -65,6 +78,10 @@ SIGNATURES = {
     "sbi_bary_jacobian_f64": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp]),
     "sbi_bary_integral_workspace": (c_i64, [c_vp]),
     "sbi_bary_integral_f64": (c_i32, [c_vp, c_vp, c_vp, c_vp]),
+    "sbi_newton_plan_create": (c_i32, [ctypes.POINTER(NewtonDesc), ctypes.POINTER(c_vp)]),
+    "sbi_newton_plan_destroy": (c_i32, [c_vp]),
+    "sbi_newton_eval_mixed": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp]),
+    "sbi_newton_eval_f64": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp]),
 }
 
 fn = {name: _sig(name, r, a) for name, (r, a) in SIGNATURES.items()}

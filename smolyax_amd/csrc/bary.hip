@@ -49,9 +49,20 @@ void set_error(const std::string& msg) { g_err = msg; }
 namespace {
 
 constexpr int kTile = 64;       // points per tile (one per lane)
-constexpr int kWaves = 8;       // waves per workgroup sharing one tile
+constexpr int kWaves = 16;      // waves per workgroup sharing one tile
 constexpr int kLMax = 12;       // max (deg + 1) of a streamed factor held in registers
 constexpr int kMaxLds = 160 * 1024;
+
+// Read-only tables are read through the constant address space: a load with a wave-uniform
+// address then becomes a scalar (SMEM) load into SGPRs and the control flow that depends on it
+// stays uniform.  LDS is addressed through address space 3 explicitly.
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+template <typename T>
+__device__ __forceinline__ cptr<T> C(const T* p) {
+  return (cptr<T>)(p);
+}
+typedef __attribute__((address_space(3))) double lds_f64;
 
 // ------------------------------------------------------------------------------------------
 // 1D basis helpers
@@ -59,8 +70,7 @@ constexpr int kMaxLds = 160 * 1024;
 
 // Streamed factor: first-form basis of degree `deg` (< kLMax) into registers L[0..kLMax).
 // Entries beyond deg are zero.  Exact one-hot at nodes.
-__device__ __forceinline__ void basis_regs(double x, const double* __restrict__ xi,
-                                           const double* __restrict__ w, int deg,
+__device__ __forceinline__ void basis_regs(double x, cptr<double> xi, cptr<double> w, int deg,
                                            double (&L)[kLMax]) {
   double dlt[kLMax];
 #pragma unroll
@@ -84,9 +94,8 @@ __device__ __forceinline__ void basis_regs(double x, const double* __restrict__ 
 
 // Prefix slot: first-form basis of degree `deg` (any) written to LDS rows row0..row0+deg for this
 // lane.  Two passes: suffix products stored, then the forward pass finishes each value.
-__device__ __forceinline__ void basis_lds(double x, const double* __restrict__ xi,
-                                          const double* __restrict__ w, int deg,
-                                          double* __restrict__ tab, int row0, int lane) {
+__device__ __forceinline__ void basis_lds(double x, cptr<double> xi, cptr<double> w, int deg,
+                                          lds_f64* tab, int row0, int lane) {
   double s = 1.0;
   for (int m = deg; m >= 0; --m) {
     tab[(row0 + m) * kTile + lane] = s;
@@ -109,21 +118,123 @@ __device__ __forceinline__ void basis_lds(double x, const double* __restrict__ x
 // Evaluation kernel
 // ------------------------------------------------------------------------------------------
 
-// Inner contraction over the streamed factor for OC outputs of one F row group:
-//   out[o] = sum_{m <= dl} F[(row + m) * d_out + o] * Ll[m]
-template <int OC>
-__device__ __forceinline__ void row_dot(const double* __restrict__ Frow, int64_t d_out, int oc_n,
-                                        int dl, const double (&Ll)[kLMax], double (&out)[OC]) {
+// Inner product over the streamed factor for OC outputs, DL = deg + 1 known at compile time:
+//   out[o] = sum_{m < DL} F[m * d_out + o] * Ll[m]
+template <int DL, int OC>
+__device__ __forceinline__ void row_dot(cptr<double> Frow, int64_t d_out, int oc_n,
+                                        const double (&Ll)[kLMax], double (&out)[OC]) {
+  if constexpr (OC == 1) {
+    // d_out == 1: the streamed entries are contiguous (wide scalar loads)
+    double s = 0.0;
 #pragma unroll
-  for (int o = 0; o < OC; ++o) out[o] = 0.0;
+    for (int m = 0; m < DL; ++m) s = fma(Frow[m], Ll[m], s);
+    out[0] = s;
+  } else {
 #pragma unroll
-  for (int m = 0; m < kLMax; ++m) {
-    if (m <= dl) {
+    for (int o = 0; o < OC; ++o) {
+      double s = 0.0;
+      if (o < oc_n) {
 #pragma unroll
-      for (int o = 0; o < OC; ++o) {
-        if (o < oc_n) out[o] = fma(Frow[m * d_out + o], Ll[m], out[o]);
+        for (int m = 0; m < DL; ++m) s = fma(Frow[m * d_out + o], Ll[m], s);
+      }
+      out[o] = s;
+    }
+  }
+}
+
+// Contraction of one block for a streamed degree known at compile time (DL = deg + 1).
+// Prefix factors (up to three fast paths, any number generically) come from the LDS table.
+template <int DL, int OC>
+__device__ __forceinline__ void contract_block(const sbi_bary_desc& d, int b, cptr<double> Fb,
+                                               int oc_n, uint32_t w1, uint32_t w2, uint32_t w3,
+                                               const double (&Ll)[kLMax], const lds_f64* tab,
+                                               int lane, double (&acc)[OC]) {
+  const int64_t dout = (OC == 1) ? 1 : d.d_out;
+  const int64_t rs = (int64_t)DL * dout;
+  const int npre = w1 & 7;
+  double inner[OC];
+  if (npre == 0) {
+    row_dot<DL, OC>(Fb, dout, oc_n, Ll, inner);
+#pragma unroll
+    for (int o = 0; o < OC; ++o) acc[o] += inner[o];
+  } else if (npre == 1) {
+    const int d1 = (w1 >> 8) & 31, r1 = w2 & 0xffff;
+    for (int m1 = 0; m1 <= d1; ++m1) {
+      const double p1 = tab[(r1 + m1) * kTile + lane];
+      row_dot<DL, OC>(Fb + m1 * rs, dout, oc_n, Ll, inner);
+#pragma unroll
+      for (int o = 0; o < OC; ++o) acc[o] = fma(p1, inner[o], acc[o]);
+    }
+  } else if (npre == 2) {
+    const int d1 = (w1 >> 8) & 31, d2 = (w1 >> 13) & 31;
+    const int r1 = w2 & 0xffff, r2 = w2 >> 16;
+    cptr<double> Fe = Fb;
+    for (int m1 = 0; m1 <= d1; ++m1) {
+      const double p1 = tab[(r1 + m1) * kTile + lane];
+      for (int m2 = 0; m2 <= d2; ++m2, Fe += rs) {
+        const double p2 = p1 * tab[(r2 + m2) * kTile + lane];
+        row_dot<DL, OC>(Fe, dout, oc_n, Ll, inner);
+#pragma unroll
+        for (int o = 0; o < OC; ++o) acc[o] = fma(p2, inner[o], acc[o]);
       }
     }
+  } else if (npre == 3) {
+    const int d1 = (w1 >> 8) & 31, d2 = (w1 >> 13) & 31, d3 = (w1 >> 18) & 31;
+    const int r1 = w2 & 0xffff, r2 = w2 >> 16, r3 = w3 & 0xffff;
+    cptr<double> Fe = Fb;
+    for (int m1 = 0; m1 <= d1; ++m1) {
+      const double p1 = tab[(r1 + m1) * kTile + lane];
+      for (int m2 = 0; m2 <= d2; ++m2) {
+        const double p2 = p1 * tab[(r2 + m2) * kTile + lane];
+        for (int m3 = 0; m3 <= d3; ++m3, Fe += rs) {
+          const double p3 = p2 * tab[(r3 + m3) * kTile + lane];
+          row_dot<DL, OC>(Fe, dout, oc_n, Ll, inner);
+#pragma unroll
+          for (int o = 0; o < OC; ++o) acc[o] = fma(p3, inner[o], acc[o]);
+        }
+      }
+    }
+  } else {
+    // generic: any number of prefix factors, product recomputed per prefix entry
+    cptr<int32_t> blk_fac = C(d.blk_fac), fac_deg = C(d.fac_deg), fac_row = C(d.fac_row);
+    const int f0 = blk_fac[b];
+    const int np = blk_fac[b + 1] - f0;
+    int64_t nent = 1;
+    for (int i = 0; i < np; ++i) nent *= (fac_deg[f0 + i] + 1);
+    for (int64_t e = 0; e < nent; ++e) {
+      int64_t rem = e;
+      double pw = 1.0;
+      for (int i = np - 1; i >= 0; --i) {
+        const int di1 = fac_deg[f0 + i] + 1;
+        const int mi = (int)(rem % di1);
+        rem /= di1;
+        pw *= tab[(fac_row[f0 + i] + mi) * kTile + lane];
+      }
+      row_dot<DL, OC>(Fb + e * rs, dout, oc_n, Ll, inner);
+#pragma unroll
+      for (int o = 0; o < OC; ++o) acc[o] = fma(pw, inner[o], acc[o]);
+    }
+  }
+}
+
+template <int OC>
+__device__ __forceinline__ void contract_dispatch(const sbi_bary_desc& d, int b, cptr<double> Fb,
+                                                  int oc_n, uint32_t w1, uint32_t w2, uint32_t w3,
+                                                  const double (&Ll)[kLMax], const lds_f64* tab,
+                                                  int lane, double (&acc)[OC]) {
+  switch ((w1 >> 3) & 31) {
+    case 0: contract_block<1, OC>(d, b, Fb, oc_n, w1, w2, w3, Ll, tab, lane, acc); break;
+    case 1: contract_block<2, OC>(d, b, Fb, oc_n, w1, w2, w3, Ll, tab, lane, acc); break;
+    case 2: contract_block<3, OC>(d, b, Fb, oc_n, w1, w2, w3, Ll, tab, lane, acc); break;
+    case 3: contract_block<4, OC>(d, b, Fb, oc_n, w1, w2, w3, Ll, tab, lane, acc); break;
+    case 4: contract_block<5, OC>(d, b, Fb, oc_n, w1, w2, w3, Ll, tab, lane, acc); break;
+    case 5: contract_block<6, OC>(d, b, Fb, oc_n, w1, w2, w3, Ll, tab, lane, acc); break;
+    case 6: contract_block<7, OC>(d, b, Fb, oc_n, w1, w2, w3, Ll, tab, lane, acc); break;
+    case 7: contract_block<8, OC>(d, b, Fb, oc_n, w1, w2, w3, Ll, tab, lane, acc); break;
+    case 8: contract_block<9, OC>(d, b, Fb, oc_n, w1, w2, w3, Ll, tab, lane, acc); break;
+    case 9: contract_block<10, OC>(d, b, Fb, oc_n, w1, w2, w3, Ll, tab, lane, acc); break;
+    case 10: contract_block<11, OC>(d, b, Fb, oc_n, w1, w2, w3, Ll, tab, lane, acc); break;
+    default: contract_block<12, OC>(d, b, Fb, oc_n, w1, w2, w3, Ll, tab, lane, acc); break;
   }
 }
 
@@ -131,24 +242,36 @@ template <int OC>
 __global__ __launch_bounds__(kTile* kWaves) void bary_eval_kernel(
     sbi_bary_desc d, const double* __restrict__ x, int64_t P, int64_t ldx,
     double* __restrict__ y) {
-  extern __shared__ double lds[];
+  extern __shared__ double lds_raw[];
+  lds_f64* lds = (lds_f64*)lds_raw;
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t p_raw = (int64_t)blockIdx.x * kTile + lane;
   const bool valid = p_raw < P;
   const int64_t p = valid ? p_raw : (P - 1);
   const int oc0 = blockIdx.y * OC;
   const int oc_n = (int)min<int64_t>(OC, d.d_out - oc0);
   const double* __restrict__ xp = x + p * ldx;
+  cptr<int32_t> seg_off = C(d.seg_off), seg_slot = C(d.seg_slot), seg_b0 = C(d.seg_b0),
+                seg_b1 = C(d.seg_b1), slot_dim = C(d.slot_dim), slot_deg = C(d.slot_deg),
+                prefix_slot = C(d.prefix_slot), prefix_off = C(d.prefix_off);
+  cptr<double> slot_nodes = C(d.slot_nodes), slot_w = C(d.slot_w), F = C(d.F);
+  cptr<int32_t> desc = C(d.blk_desc);
 
-  // 1. prefix table
+  // segments of this wave; the first streamed coordinate is requested before the table work
+  const int g_begin = seg_off[wave];
+  const int g_end = seg_off[wave + 1];
+  double x_next = 0.0;
+  if (g_begin < g_end) {
+    const int s0 = seg_slot[g_begin];
+    if (s0 >= 0) x_next = xp[slot_dim[s0]];
+  }
+
+  // 1. prefix table (all waves cooperate)
   for (int s = wave; s < d.n_prefix_slots; s += kWaves) {
-    const int slot = d.prefix_slot[s];
-    const int row = d.prefix_off[s];
-    const int deg = d.slot_deg[slot];
-    const double xv = xp[d.slot_dim[slot]];
-    basis_lds(xv, d.slot_nodes + (int64_t)slot * d.dmax1, d.slot_w + (int64_t)slot * d.dmax1, deg,
-              lds, row, lane);
+    const int slot = prefix_slot[s];
+    basis_lds(xp[slot_dim[slot]], slot_nodes + (int64_t)slot * d.dmax1,
+              slot_w + (int64_t)slot * d.dmax1, slot_deg[slot], lds, prefix_off[s], lane);
   }
   __syncthreads();
 
@@ -156,89 +279,28 @@ __global__ __launch_bounds__(kTile* kWaves) void bary_eval_kernel(
 #pragma unroll
   for (int o = 0; o < OC; ++o) acc[o] = 0.0;
 
-  const int r_begin = d.wave_run[wave];
-  const int r_end = d.wave_run[wave + 1];
-  for (int r = r_begin; r < r_end; ++r) {
-    const int ls = d.run_slot[r];
+  for (int g = g_begin; g < g_end; ++g) {
+    const int ls = seg_slot[g];
+    const double x_cur = x_next;
+    if (g + 1 < g_end) {  // prefetch the next segment's streamed coordinate
+      const int sn = seg_slot[g + 1];
+      if (sn >= 0) x_next = xp[slot_dim[sn]];
+    }
     double Ll[kLMax];
-    int dl;
     if (ls >= 0) {
-      dl = d.slot_deg[ls];
-      const double xv = xp[d.slot_dim[ls]];
-      basis_regs(xv, d.slot_nodes + (int64_t)ls * d.dmax1, d.slot_w + (int64_t)ls * d.dmax1, dl,
-                 Ll);
+      basis_regs(x_cur, slot_nodes + (int64_t)ls * d.dmax1, slot_w + (int64_t)ls * d.dmax1,
+                 slot_deg[ls], Ll);
     } else {
-      dl = 0;
 #pragma unroll
       for (int m = 0; m < kLMax; ++m) Ll[m] = (m == 0) ? 1.0 : 0.0;
     }
-    const int b_end = d.run_start[r + 1];
-    for (int b = d.run_start[r]; b < b_end; ++b) {
-      const int f0 = d.blk_fac[b];
-      const int npre = d.blk_fac[b + 1] - f0;
-      const double* __restrict__ Fb = d.F + d.blk_foff[b] * d.d_out + oc0;
-      const int64_t rs = (int64_t)(dl + 1) * d.d_out;  // stride between prefix entries
-      double inner[OC];
-      if (npre == 0) {
-        row_dot<OC>(Fb, d.d_out, oc_n, dl, Ll, inner);
-#pragma unroll
-        for (int o = 0; o < OC; ++o) acc[o] += inner[o];
-      } else if (npre == 1) {
-        const int r1 = d.fac_row[f0], d1 = d.fac_deg[f0];
-        for (int m1 = 0; m1 <= d1; ++m1) {
-          const double p1 = lds[(r1 + m1) * kTile + lane];
-          row_dot<OC>(Fb + m1 * rs, d.d_out, oc_n, dl, Ll, inner);
-#pragma unroll
-          for (int o = 0; o < OC; ++o) acc[o] = fma(p1, inner[o], acc[o]);
-        }
-      } else if (npre == 2) {
-        const int r1 = d.fac_row[f0], d1 = d.fac_deg[f0];
-        const int r2 = d.fac_row[f0 + 1], d2 = d.fac_deg[f0 + 1];
-        int e = 0;
-        for (int m1 = 0; m1 <= d1; ++m1) {
-          const double p1 = lds[(r1 + m1) * kTile + lane];
-          for (int m2 = 0; m2 <= d2; ++m2, ++e) {
-            const double p2 = p1 * lds[(r2 + m2) * kTile + lane];
-            row_dot<OC>(Fb + e * rs, d.d_out, oc_n, dl, Ll, inner);
-#pragma unroll
-            for (int o = 0; o < OC; ++o) acc[o] = fma(p2, inner[o], acc[o]);
-          }
-        }
-      } else if (npre == 3) {
-        const int r1 = d.fac_row[f0], d1 = d.fac_deg[f0];
-        const int r2 = d.fac_row[f0 + 1], d2 = d.fac_deg[f0 + 1];
-        const int r3 = d.fac_row[f0 + 2], d3 = d.fac_deg[f0 + 2];
-        int e = 0;
-        for (int m1 = 0; m1 <= d1; ++m1) {
-          const double p1 = lds[(r1 + m1) * kTile + lane];
-          for (int m2 = 0; m2 <= d2; ++m2) {
-            const double p2 = p1 * lds[(r2 + m2) * kTile + lane];
-            for (int m3 = 0; m3 <= d3; ++m3, ++e) {
-              const double p3 = p2 * lds[(r3 + m3) * kTile + lane];
-              row_dot<OC>(Fb + e * rs, d.d_out, oc_n, dl, Ll, inner);
-#pragma unroll
-              for (int o = 0; o < OC; ++o) acc[o] = fma(p3, inner[o], acc[o]);
-            }
-          }
-        }
-      } else {
-        // generic odometer: product recomputed per prefix entry
-        int64_t nent = 1;
-        for (int i = 0; i < npre; ++i) nent *= (d.fac_deg[f0 + i] + 1);
-        for (int64_t e = 0; e < nent; ++e) {
-          int64_t rem = e;
-          double pw = 1.0;
-          for (int i = npre - 1; i >= 0; --i) {
-            const int di1 = d.fac_deg[f0 + i] + 1;
-            const int mi = (int)(rem % di1);
-            rem /= di1;
-            pw *= lds[(d.fac_row[f0 + i] + mi) * kTile + lane];
-          }
-          row_dot<OC>(Fb + e * rs, d.d_out, oc_n, dl, Ll, inner);
-#pragma unroll
-          for (int o = 0; o < OC; ++o) acc[o] = fma(pw, inner[o], acc[o]);
-        }
-      }
+    const int b_end = seg_b1[g];
+    for (int b = seg_b0[g]; b < b_end; ++b) {
+      const int rx = desc[4 * b];
+      const uint32_t ry = (uint32_t)desc[4 * b + 1], rz = (uint32_t)desc[4 * b + 2],
+                     rw = (uint32_t)desc[4 * b + 3];
+      cptr<double> Fb = F + (OC == 1 ? (int64_t)rx : (int64_t)rx * d.d_out + oc0);
+      contract_dispatch<OC>(d, b, Fb, oc_n, ry, rz, rw, Ll, lds, lane, acc);
     }
   }
 
@@ -419,13 +481,13 @@ int sbi_bary_plan_create(const sbi_bary_desc* desc, sbi_plan** out) {
   }
   const sbi_bary_desc& d = *desc;
   if (d.d_in <= 0 || d.d_out <= 0 || d.n_slots < 0 || d.n_blocks < 0 || d.n_runs < 0 ||
-      d.dmax1 <= 0 || !d.F || !d.offset || !d.wave_run || !d.run_start || !d.blk_fac ||
+      d.dmax1 <= 0 || !d.F || !d.offset || !d.blk_desc || !d.seg_off || !d.run_start || !d.blk_fac ||
       !d.blk_foff) {
     sbi::set_error("sbi_bary_plan_create: invalid descriptor");
     return SBI_ERR_INVALID;
   }
   if (d.n_split != kWaves) {
-    sbi::set_error("sbi_bary_plan_create: n_split must equal the waves per workgroup (8)");
+    sbi::set_error("sbi_bary_plan_create: n_split must equal the waves per workgroup (16)");
     return SBI_ERR_INVALID;
   }
   if (d.dmax1 < kLMax) {
@@ -433,7 +495,7 @@ int sbi_bary_plan_create(const sbi_bary_desc* desc, sbi_plan** out) {
     return SBI_ERR_INVALID;
   }
   const size_t table = (size_t)d.n_prefix_vals * kTile * sizeof(double);
-  const size_t red = (size_t)kWaves * 8 * kTile * sizeof(double);
+  const size_t red = (size_t)kWaves * 4 * kTile * sizeof(double);
   const size_t lds = table > red ? table : red;
   if (lds > (size_t)kMaxLds) {
     sbi::set_error("sbi_bary_plan_create: prefix basis table exceeds LDS (" +
@@ -480,10 +542,9 @@ extern "C" int sbi_bary_eval_f64(const sbi_plan* plan, const double* x, int64_t 
   if (P == 0) return SBI_OK;
   hipStream_t st = (hipStream_t)stream;
   const int64_t dout = plan->bary.d_out;
-  if (dout == 1) return launch_eval<1>(plan, x, P, ldx, y, st);
+  if (dout == 1) return launch_eval<1>(plan, x, P, ldx, y, st);  // unit-stride F rows
   if (dout == 2) return launch_eval<2>(plan, x, P, ldx, y, st);
-  if (dout <= 4) return launch_eval<4>(plan, x, P, ldx, y, st);
-  return launch_eval<8>(plan, x, P, ldx, y, st);
+  return launch_eval<4>(plan, x, P, ldx, y, st);
 }
 
 extern "C" int sbi_bary_jacobian_f64(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx,

@@ -13,8 +13,10 @@ import numpy as np
 import torch
 
 from . import _native as N
-from . import indices, packing
+from . import indices, newton, packing
 from .nodes import Generator
+
+PRECISIONS = ("fp64", "mixed", "newton-fp64")
 
 
 def _dev_tensor(a, dtype, device):
@@ -42,7 +44,17 @@ class SmolyakBarycentricInterpolator:
 
     def __init__(self, *, d_out: int, node_gen: Generator, k: Sequence[float], t: float,
                  f: Callable = None, n_inputs: int = None, memory_limit: float = 4.0,
-                 device=None) -> None:
+                 device=None, precision: str = "fp64") -> None:
+        """Keyword-only, as interpolation.py:51-113.  Extensions: `device` (torch device) and
+        `precision`, the default evaluation path of __call__:
+          "fp64"        barycentric combination technique in FP64 (the reference's algorithm);
+          "mixed"       hierarchical Newton form, FP32 surpluses / products / group sums with FP64
+                        node differences and accumulation (nested nodes, d_out == 1);
+          "newton-fp64" hierarchical Newton form in FP64 (nested nodes, d_out == 1).
+        """
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {PRECISIONS}")
+        self._precision = precision
         self._d_in = len(k)
         self._d_out = int(d_out)
         self._node_gen = node_gen
@@ -55,6 +67,7 @@ class SmolyakBarycentricInterpolator:
         self._n_f_evals_new = 0
         self._device = torch.device(device) if device is not None else None
         self._plan = None
+        self._nplan = None
         self._layout = None
         if f is not None:
             self.set_f(f=f)
@@ -76,6 +89,15 @@ class SmolyakBarycentricInterpolator:
         self._layout = L
         self._F_host = F
         self._offset_host = offset
+        self._nlayout = None
+        if self._is_nested and self._d_out == 1:
+            nf = newton.NewtonForm(self._k, self._t, self._node_gen, self._d_out)
+            vals = np.array([np.broadcast_to(np.asarray(f_evals[nu], dtype=np.float64).reshape(-1),
+                                             (self._d_out,)) for nu in nf.nus])
+            nf.set_values(vals)
+            self._nlayout = newton.build_newton_layout(nf)
+        elif self._precision != "fp64":
+            raise ValueError(f"precision {self._precision!r} needs nested nodes and d_out == 1")
         self._upload()
         if self._n_inputs is not None:
             x0 = torch.rand((self._n_inputs, self._d_in), dtype=torch.float64, device=self._device)
@@ -108,7 +130,11 @@ class SmolyakBarycentricInterpolator:
         t["blk_foff"] = _dev_tensor(L.blk_foff, i64, dev)
         t["F"] = _dev_tensor(self._F_host if self._F_host.size else np.zeros((1, self._d_out)), f64, dev)
         t["offset"] = _dev_tensor(self._offset_host, f64, dev)
-        t["wave_run"] = _dev_tensor(L.wave_run, i32, dev)
+        t["blk_desc"] = _dev_tensor(L.blk_desc, i32, dev)
+        t["seg_off"] = _dev_tensor(L.seg_off, i32, dev)
+        t["seg_slot"] = _dev_tensor(L.seg_slot, i32, dev)
+        t["seg_b0"] = _dev_tensor(L.seg_b0, i32, dev)
+        t["seg_b1"] = _dev_tensor(L.seg_b1, i32, dev)
         t["dim_blk"] = _dev_tensor(L.dim_blk, i32, dev)
         t["dim_blk_idx"] = _dev_tensor(L.dim_blk_idx, i32, dev)
         t["dim_blk_pos"] = _dev_tensor(L.dim_blk_pos, i32, dev)
@@ -126,6 +152,7 @@ class SmolyakBarycentricInterpolator:
         for name, ten in t.items():
             setattr(d, name, ten.data_ptr())
         d.n_split = packing.N_WAVES
+        d.n_seg = int(L.seg_off[-1])
         self._desc = d
         if self._plan is not None:
             N.fn["sbi_plan_destroy"](self._plan)
@@ -133,14 +160,50 @@ class SmolyakBarycentricInterpolator:
         plan = ctypes.c_void_p()
         N.check(N.fn["sbi_bary_plan_create"](ctypes.byref(d), ctypes.byref(plan)), "plan_create")
         self._plan = plan
+        if self._nlayout is not None:
+            self._upload_newton()
+
+    def _upload_newton(self):
+        NL, dev = self._nlayout, self._device
+        i32, f32, f64 = torch.int32, torch.float32, torch.float64
+        t = {
+            "s_root": _dev_tensor(NL.s_root, f64, dev),
+            "int_parent": _dev_tensor(NL.int_parent, i32, dev),
+            "int_dim": _dev_tensor(NL.int_dim, i32, dev),
+            "int_deg": _dev_tensor(NL.int_deg, i32, dev),
+            "level_off": _dev_tensor(NL.level_off, i32, dev),
+            "dim_xi": _dev_tensor(NL.dim_xi, f64, dev),
+            "dim_c": _dev_tensor(NL.dim_c, f64, dev),
+            "grp": _dev_tensor(NL.grp, i32, dev),
+            "grp_next_dim": _dev_tensor(NL.grp_next_dim, i32, dev),
+            "wave_grp": _dev_tensor(NL.wave_grp, i32, dev),
+            "node_s32": _dev_tensor(NL.node_s32, f32, dev),
+            "node_s64": _dev_tensor(NL.node_s64, f64, dev),
+            "node_q": _dev_tensor(NL.node_q, i32, dev),
+        }
+        self._ndev = t
+        d = N.NewtonDesc()
+        d.d_in, d.d_out = self._d_in, self._d_out
+        d.n_nodes, d.n_internal, d.n_groups = NL.n_nodes, NL.n_internal, NL.n_groups
+        d.n_split, d.n_levels, d.kmax1 = newton.N_WAVES, NL.n_levels, NL.kmax1
+        for name, ten in t.items():
+            setattr(d, name, ten.data_ptr())
+        self._ndesc = d
+        if self._nplan is not None:
+            N.fn["sbi_newton_plan_destroy"](self._nplan)
+            self._nplan = None
+        plan = ctypes.c_void_p()
+        N.check(N.fn["sbi_newton_plan_create"](ctypes.byref(d), ctypes.byref(plan)), "newton_plan")
+        self._nplan = plan
 
     def __del__(self):
-        plan = getattr(self, "_plan", None)
-        if plan is not None:
-            try:
-                N.fn["sbi_plan_destroy"](plan)
-            except Exception:
-                pass
+        for attr, fn in (("_plan", "sbi_plan_destroy"), ("_nplan", "sbi_newton_plan_destroy")):
+            plan = getattr(self, attr, None)
+            if plan is not None:
+                try:
+                    N.fn[fn](plan)
+                except Exception:
+                    pass
 
     # ------------------------------------------------------------------------------ inputs
     def _validate_input(self, x) -> torch.Tensor:
@@ -165,12 +228,26 @@ class SmolyakBarycentricInterpolator:
 
     # ------------------------------------------------------------------------------ evaluation
     def __call__(self, x) -> torch.Tensor:
-        """interpolation.py:264-304: y[P, d_out] in float64 (barycentric combination technique)."""
+        """interpolation.py:264-304: y[P, d_out] (float64 tensor) on the default precision path."""
+        return self.evaluate(x, precision=self._precision)
+
+    def evaluate(self, x, precision: str = None) -> torch.Tensor:
+        """Evaluate on a chosen path (see the constructor's `precision`)."""
+        precision = precision or self._precision
         x = self._validate_input(x)
         P = x.shape[0]
         y = torch.empty((P, self._d_out), dtype=torch.float64, device=self._device)
-        N.check(N.fn["sbi_bary_eval_f64"](self._plan, x.data_ptr(), P, x.stride(0), y.data_ptr(),
-                                          self._stream()), "eval")
+        if precision == "fp64":
+            N.check(N.fn["sbi_bary_eval_f64"](self._plan, x.data_ptr(), P, x.stride(0), y.data_ptr(),
+                                              self._stream()), "eval")
+        elif precision in ("mixed", "newton-fp64"):
+            if self._nplan is None:
+                raise ValueError(f"precision {precision!r} needs nested nodes and d_out == 1")
+            fn = "sbi_newton_eval_mixed" if precision == "mixed" else "sbi_newton_eval_f64"
+            N.check(N.fn[fn](self._nplan, x.data_ptr(), P, x.stride(0), y.data_ptr(),
+                             self._stream()), precision)
+        else:
+            raise ValueError(f"precision must be one of {PRECISIONS}")
         return y
 
     def gradient(self, x) -> torch.Tensor:

@@ -1,0 +1,238 @@
+r"""Hierarchical (Newton-surplus) form of a Smolyak interpolant on nested nodes (host, FP64).
+
+For nested node sequences and a downward-closed set Lambda (Lambda_{k,t} is downward closed),
+the combination-technique interpolant of the reference
+    I^Lambda[f] = sum_nu zeta_nu I^nu[f]            (interpolation.py:264-304)
+equals the sum of hierarchical increments
+    I^Lambda[f](x) = sum_{nu in Lambda} s_nu prod_{j: nu_j > 0} h^{(j)}_{nu_j}(x_j),
+    h^{(j)}_k(x) = prod_{r < k} (x - xi_r) / (xi_k - xi_r)   (Newton basis, h_k(xi_k) = 1),
+with the surpluses s_nu obtained from the grid values f(xi_nu) by 1D hierarchisation along
+every dimension.  Same polynomial, |Lambda| terms instead of K_raw, and no cancelling zeta
+coefficients, which is what lets the mixed-precision kernel meet a 1e-6 relative error.
+
+The node sequence of every dimension must be nested: the 1D nodes of degree k are the first
+k + 1 nodes of the sequence (Leja).  Gauss-Hermite (non-nested) is rejected.
+"""
+
+import numpy as np
+
+from . import indices
+
+
+def newton_basis_consts(xi):
+    """c_k = 1 / prod_{r<k} (xi_k - xi_r) so that h_k(x) = c_k prod_{r<k} (x - xi_r)."""
+    xi = np.asarray(xi, dtype=np.float64)
+    c = np.ones(len(xi))
+    for k in range(1, len(xi)):
+        p = 1.0
+        for r in range(k):
+            p *= xi[k] - xi[r]
+        c[k] = 1.0 / p
+    return c
+
+
+def newton_basis(x, xi, c, k):
+    """h_k(x) for an array x."""
+    out = np.full(np.shape(x), c[k], dtype=np.float64)
+    for r in range(k):
+        out = out * (x - xi[r])
+    return out
+
+
+def hierarchize(nus, values, node_gen):
+    """Surpluses for the multi-indices `nus` (sparse tuples, downward closed) from grid values.
+
+    values: (|Lambda|, d_out) f at the grid nodes, row order of `nus`.  Works in place on a copy.
+    """
+    index = {nu: i for i, nu in enumerate(nus)}
+    s = np.array(values, dtype=np.float64, copy=True)
+    dims = sorted({j for nu in nus for j, _ in nu})
+    maxdeg = {}
+    for nu in nus:
+        for j, v in nu:
+            maxdeg[j] = max(maxdeg.get(j, 0), v)
+    for j in dims:
+        g = node_gen[j]
+        xi = np.asarray(g(maxdeg[j]), dtype=np.float64)
+        c = newton_basis_consts(xi)
+        # H[i, l] = h_l(xi_i) for l < i
+        H = np.zeros((len(xi), len(xi)))
+        for i in range(len(xi)):
+            for l in range(i):
+                H[i, l] = newton_basis(xi[i], xi, c, l)
+        # process nu with nu_j = 1, 2, ... in increasing order (lower levels already final)
+        by_level = {}
+        for nu in nus:
+            d = dict(nu)
+            lv = d.get(j, 0)
+            if lv > 0:
+                by_level.setdefault(lv, []).append(nu)
+        for lv in sorted(by_level):
+            for nu in by_level[lv]:
+                d = dict(nu)
+                tgt = index[nu]
+                for l in range(lv):
+                    if l == 0:
+                        d.pop(j, None)
+                    else:
+                        d[j] = l
+                    key = tuple(sorted(d.items()))
+                    s[tgt] -= H[lv, l] * s[index[key]]
+    return s
+
+
+class NewtonForm:
+    """Host description of the Newton form: multi-indices, surpluses and per-dim node tables."""
+
+    def __init__(self, k, t, node_gen, d_out):
+        if not node_gen.is_nested:
+            raise ValueError("the Newton (hierarchical) form needs nested nodes (Leja)")
+        self.nus = indices.indexset(k, t)
+        self.d_in = len(k)
+        self.d_out = d_out
+        self.node_gen = node_gen
+        maxdeg = np.zeros(self.d_in, dtype=np.int64)
+        for nu in self.nus:
+            for j, v in nu:
+                maxdeg[j] = max(maxdeg[j], v)
+        self.maxdeg = maxdeg
+
+    def grid_point(self, nu, zero):
+        x = zero.copy()
+        for j, v in nu:
+            x[j] = self.node_gen[j](v)[v]
+        return x
+
+    def set_values(self, values):
+        """values[i] = f at the grid node of nus[i] (rows of d_out)."""
+        self.s = hierarchize(self.nus, values, self.node_gen)
+
+    def evaluate(self, x):
+        """FP64 numpy evaluation (reference for the kernels' layout tests)."""
+        x = np.atleast_2d(np.asarray(x, dtype=np.float64))
+        P = x.shape[0]
+        cache = {}
+        y = np.zeros((P, self.d_out))
+        for i, nu in enumerate(self.nus):
+            w = np.ones(P)
+            for j, v in nu:
+                key = (j, v)
+                h = cache.get(key)
+                if h is None:
+                    xi = np.asarray(self.node_gen[j](self.maxdeg[j]), dtype=np.float64)
+                    h = newton_basis(x[:, j], xi, newton_basis_consts(xi), v)
+                    cache[key] = h
+                w = w * h
+            y += w[:, None] * self.s[i][None, :]
+        return y
+
+
+N_WAVES = 16  # waves per point tile (kWaves in newton.hip)
+
+
+class NewtonLayout:
+    """Device layout of the Newton form (see include/smolyax_amd.h, sbi_newton_desc)."""
+
+
+def build_newton_layout(nf: NewtonForm, n_waves: int = N_WAVES) -> NewtonLayout:
+    L = NewtonLayout()
+    nus = nf.nus
+    root = nus.index(())
+    L.s_root = np.asarray(nf.s[root], dtype=np.float64).reshape(-1)
+    parent_of = {nu: nu[:-1] for nu in nus if nu}
+    internal = sorted({p for p in parent_of.values() if p}, key=lambda v: (len(v), v))
+    row = {(): 0}
+    for i, v in enumerate(internal):
+        row[v] = i + 1
+    L.n_internal = len(internal)
+    L.int_parent = np.array([row[v[:-1]] for v in internal] or [0], dtype=np.int32)
+    L.int_dim = np.array([v[-1][0] for v in internal] or [0], dtype=np.int32)
+    L.int_deg = np.array([v[-1][1] for v in internal] or [0], dtype=np.int32)
+    depth = [len(v) for v in internal]
+    levels = sorted(set(depth))
+    L.level_off = np.array([0] + [int(np.sum(np.array(depth) <= lv)) for lv in levels], dtype=np.int32)
+    L.n_levels = len(levels)
+
+    idx = {nu: i for i, nu in enumerate(nus)}
+    nodes = sorted((nu for nu in nus if nu), key=lambda v: (v[-1][0], v[-1][1], row[v[:-1]]))
+    L.n_nodes = len(nodes)
+    L.node_q = np.array([row[v[:-1]] for v in nodes] or [0], dtype=np.int32)
+    s = np.array([nf.s[idx[v]] for v in nodes]).reshape(len(nodes), -1) if nodes else np.zeros((1, 1))
+    L.node_s64 = np.ascontiguousarray(s[:, 0], dtype=np.float64)
+    L.node_s32 = L.node_s64.astype(np.float32)
+    # groups by last factor (j, k), in node order
+    grp = []
+    i = 0
+    while i < len(nodes):
+        key = nodes[i][-1]
+        j0 = i
+        while i < len(nodes) and nodes[i][-1] == key:
+            i += 1
+        grp.append((j0, i, key[0], key[1]))
+    L.grp = np.array(grp or [(0, 0, 0, 1)], dtype=np.int32).reshape(-1)
+    L.n_groups = len(grp)
+    # Newton constants per dim
+    kmax1 = int(nf.maxdeg.max()) + 1 if len(nf.maxdeg) else 1
+    L.kmax1 = kmax1
+    L.dim_xi = np.zeros((nf.d_in, kmax1))
+    L.dim_c = np.zeros((nf.d_in, kmax1))
+    cache = {}
+    for j in range(nf.d_in):
+        g = nf.node_gen[j]
+        key = (g._table_key(), kmax1)
+        v = cache.get(key)
+        if v is None:
+            xi = np.asarray(g(kmax1 - 1), dtype=np.float64)
+            v = (xi, newton_basis_consts(xi))
+            cache[key] = v
+        L.dim_xi[j], L.dim_c[j] = v
+    # cost-balanced split of the groups over the waves
+    cost = np.array([(b - a) + 6.0 for a, b, _, _ in grp]) if grp else np.zeros(0)
+    cum = np.concatenate([[0.0], np.cumsum(cost)])
+    cuts = [0] + [int(np.searchsorted(cum, cum[-1] * w / n_waves)) for w in range(1, n_waves)] + [len(grp)]
+    cuts = np.maximum.accumulate(np.clip(np.array(cuts), 0, len(grp)))
+    L.wave_grp = cuts.astype(np.int32)
+    nxt = np.full(max(len(grp), 1), -1, dtype=np.int32)
+    for w in range(n_waves):
+        a, b = int(cuts[w]), int(cuts[w + 1])
+        nd = -1
+        for g in range(b - 1, a - 1, -1):
+            nxt[g] = nd
+            if g == a or grp[g - 1][2] != grp[g][2]:
+                nd = grp[g][2]
+    L.grp_next_dim = nxt
+    return L
+
+
+def eval_newton_layout(L: NewtonLayout, x, fp32=False):
+    """Numpy model of the kernel's arithmetic (test helper for the host layout and precision)."""
+    x = np.atleast_2d(np.asarray(x, dtype=np.float64))
+    P = x.shape[0]
+    T = np.float32 if fp32 else np.float64
+    W = np.zeros((L.n_internal + 1, P), dtype=T)
+    W[0] = 1
+    for q in range(L.n_internal):
+        j, k = L.int_dim[q], L.int_deg[q]
+        h = np.full(P, L.dim_c[j, k])
+        for r in range(k):
+            h = h * (x[:, j] - L.dim_xi[j, r])
+        W[q + 1] = (W[L.int_parent[q]].astype(np.float64) * h).astype(T)
+    s = (L.node_s32 if fp32 else L.node_s64)
+    acc = np.zeros(P)
+    g4 = L.grp.reshape(-1, 4)
+    starts = set(int(v) for v in L.wave_grp[:-1])
+    jcur, kdone, pr = -1, 0, None
+    for g in range(L.n_groups):
+        n0, n1, j, k = g4[g]
+        # incremental Newton product as in the kernel (restarted at every wave's first group)
+        if j != jcur or g in starts:
+            jcur, kdone, pr = j, 0, np.ones(P)
+        while kdone < k:
+            pr = pr * (x[:, j] - L.dim_xi[j, kdone])
+            kdone += 1
+        h = pr * L.dim_c[j, k]
+        t = np.zeros(P, dtype=T)
+        for i in range(n0, n1):
+            t = (t + s[i] * W[L.node_q[i]]).astype(T)
+        acc += h * t.astype(np.float64)
+    return acc + L.s_root[0]

@@ -19,7 +19,7 @@ from . import barycentric, indices
 
 STREAM_LMAX = 12     # max (deg + 1) of a streamed factor (kLMax in bary.hip)
 SLOT_ROW_MIN = 12    # slot rows are padded to at least this many entries
-N_WAVES = 8          # waves per point tile (kWaves in bary.hip)
+N_WAVES = 16         # waves per point tile (kWaves in bary.hip)
 
 
 @dataclass
@@ -81,7 +81,11 @@ class Layout:
     blk_degs: list = None
     run_slot: np.ndarray = None
     run_start: np.ndarray = None
-    wave_run: np.ndarray = None
+    blk_desc: np.ndarray = None
+    seg_off: np.ndarray = None
+    seg_slot: np.ndarray = None
+    seg_b0: np.ndarray = None
+    seg_b1: np.ndarray = None
     dim_blk: np.ndarray = None
     dim_blk_idx: np.ndarray = None
     dim_blk_pos: np.ndarray = None
@@ -187,19 +191,50 @@ def build_layout(blocks: Blocks, node_gen, d_in: int, d_out: int) -> Layout:
     L.k_raw = int(foff[-1])
     L.max_n = int(max(blk_n)) if blk_n else 0
 
-    # cost-balanced split of the runs over the waves of a tile
-    costs = np.zeros(len(run_slot))
-    for r in range(len(run_slot)):
-        b0, b1 = run_start[r], run_start[r + 1]
-        costs[r] = 16 + np.sum(np.diff(L.blk_foff[b0:b1 + 1]) + 6)
-    cum = np.concatenate([[0.0], np.cumsum(costs)])
-    total = cum[-1]
-    wr = [0]
+    # packed per-block records for the eval kernel
+    desc = np.zeros((max(nb, 1), 4), dtype=np.int64)
+    for pos in range(nb):
+        f0, f1 = L.blk_fac[pos], L.blk_fac[pos + 1]
+        npre = f1 - f0
+        ls = L.blk_last_slot[pos]
+        dl = int(L.slot_deg[ls]) if ls >= 0 else 0
+        if npre > 3:
+            packed = 7 | (dl << 3)
+            r = [0, 0, 0]
+        else:
+            dg = [int(L.fac_deg[f0 + i]) for i in range(npre)] + [0] * (3 - npre)
+            r = [int(L.fac_row[f0 + i]) for i in range(npre)] + [0] * (3 - npre)
+            assert max(dg) < 32 and dl < 32 and max(r) < 65536
+            packed = npre | (dl << 3) | (dg[0] << 8) | (dg[1] << 13) | (dg[2] << 18)
+        assert L.blk_foff[pos] < 2**31 and ls + 1 < 65536
+        desc[pos] = [L.blk_foff[pos], packed, r[0] | (r[1] << 16), r[2] | ((ls + 1) << 16)]
+    L.blk_desc = desc.astype(np.uint32).view(np.int32).reshape(-1)
+
+    # cost-balanced split of the blocks over the waves of a tile; segments = runs cut at the
+    # wave boundaries (a wave recomputes the streamed basis at each segment start)
+    bcost = np.diff(L.blk_foff).astype(np.float64) + 8.0
+    cum = np.concatenate([[0.0], np.cumsum(bcost)]) if nb else np.zeros(1)
+    cuts = [0]
     for w in range(1, N_WAVES):
-        wr.append(int(np.searchsorted(cum, total * w / N_WAVES, side="left")))
-    wr.append(len(run_slot))
-    wr = np.maximum.accumulate(np.clip(np.array(wr), 0, len(run_slot)))
-    L.wave_run = wr.astype(np.int32)
+        cuts.append(int(np.searchsorted(cum, cum[-1] * w / N_WAVES, side="left")))
+    cuts.append(nb)
+    cuts = np.maximum.accumulate(np.clip(np.array(cuts), 0, nb))
+    seg_slot, seg_b0, seg_b1, seg_off = [], [], [], [0]
+    for w in range(N_WAVES):
+        b0, b1 = int(cuts[w]), int(cuts[w + 1])
+        b = b0
+        while b < b1:
+            r = int(np.searchsorted(L.run_start, b, side="right")) - 1
+            e = min(int(L.run_start[r + 1]), b1)
+            seg_slot.append(int(L.run_slot[r]))
+            seg_b0.append(b)
+            seg_b1.append(e)
+            b = e
+        seg_off.append(len(seg_slot))
+    L.seg_off = np.array(seg_off, dtype=np.int32)
+    L.seg_slot = np.array(seg_slot or [0], dtype=np.int32)
+    L.seg_b0 = np.array(seg_b0 or [0], dtype=np.int32)
+    L.seg_b1 = np.array(seg_b1 or [0], dtype=np.int32)
 
     # dim -> (block, factor position) CSR for the jacobian
     lists = [[] for _ in range(d_in)]

@@ -1,0 +1,56 @@
+"""GPU: the hierarchical (Newton) kernels against the FP64 oracle.
+
+newton-fp64: same tolerance as the barycentric FP64 path.  mixed: <= 1e-6 relative (max |diff| /
+max |y|), the precision target of the mixed path.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import indices as OI
+from oracle import nodes as ON
+from oracle import smolyak as OS
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize("d,m,dom", [(1, 9, False), (4, 60, True), (10, 1000, False),
+                                     (40, 2000, False), (200, 5000, False)])
+def test_newton_paths(gpu, d, m, dom):
+    from smolyax_amd import nodes as SN
+    from smolyax_amd.interpolation import SmolyakBarycentricInterpolator
+
+    rng = np.random.default_rng(d)
+    k = np.log((2 + np.arange(d)) / np.log(2))
+    if dom:
+        dm = np.sort(rng.random((d, 2)), axis=1) + [0, 0.3]
+        ong, sng = ON.leja(domains=dm), SN.Leja(domains=dm)
+    else:
+        ong, sng = ON.leja(dim=d), SN.Leja(dim=d)
+    t = OI.find_approximate_threshold(k, m, True)
+    a = rng.standard_normal(d) / (np.arange(d) + 1.0) ** 2
+    f = lambda x: 1.0 / (1.5 + np.tanh(np.asarray(x) @ a))
+    ref = OS.Interpolant(d_out=1, node_gen=ong, k=k, t=t, f=f)
+    ip = SmolyakBarycentricInterpolator(d_out=1, node_gen=sng, k=k, t=t, f=f, precision="mixed")
+    P = 300
+    x = np.stack([g.scale(c) for g, c in zip(ong, rng.uniform(-1, 1, (P, d)).T)], axis=1)
+    yr = ref(x)
+    e_mixed = rel(ip(x).cpu().numpy(), yr)
+    e_n64 = rel(ip.evaluate(x, precision="newton-fp64").cpu().numpy(), yr)
+    e_b64 = rel(ip.evaluate(x, precision="fp64").cpu().numpy(), yr)
+    # FP64 paths agree with the oracle up to the rounding the Smolyak coefficients amplify
+    assert e_n64 < 1e-11 and e_b64 < 1e-11, (e_n64, e_b64)
+    assert e_mixed < 1e-6, e_mixed
+
+
+def test_newton_rejects_non_nested(gpu):
+    from smolyax_amd import nodes as SN
+    from smolyax_amd.interpolation import SmolyakBarycentricInterpolator
+
+    with pytest.raises(ValueError):
+        SmolyakBarycentricInterpolator(d_out=1, node_gen=SN.GaussHermite(dim=2), k=[1.0, 1.5], t=3.0,
+                                       f=lambda x: x[0], precision="mixed")

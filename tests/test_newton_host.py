@@ -1,0 +1,60 @@
+"""Hierarchical (Newton) form on the host, no GPU: surpluses and the device layout reproduce the
+oracle; the kernel's mixed-precision arithmetic (modelled in numpy) stays within 1e-6."""
+
+import numpy as np
+import pytest
+
+from oracle import indices as OI
+from oracle import nodes as ON
+from oracle import smolyak as OS
+
+
+@pytest.mark.parametrize("d,m,dom", [(1, 9, False), (3, 40, True), (6, 200, False), (10, 1000, False)])
+def test_newton_layout_matches_oracle(d, m, dom):
+    from smolyax_amd import newton as NW
+    from smolyax_amd import nodes as SN
+
+    rng = np.random.default_rng(d)
+    k = np.log((2 + np.arange(d)) / np.log(2))
+    if dom:
+        dm = np.sort(rng.random((d, 2)), axis=1) + [0, 0.3]
+        ong, sng = ON.leja(domains=dm), SN.Leja(domains=dm)
+    else:
+        ong, sng = ON.leja(dim=d), SN.Leja(dim=d)
+    t = OI.find_approximate_threshold(k, m, True)
+    a = rng.standard_normal(d) / (np.arange(d) + 1.0)
+    f = lambda x: np.exp(-(np.asarray(x) @ a))
+    ref = OS.Interpolant(d_out=1, node_gen=ong, k=k, t=t, f=f)
+    nf = NW.NewtonForm(k, t, sng, 1)
+    zero = np.array([g(0)[0] for g in sng])
+    X = np.stack([nf.grid_point(nu, zero) for nu in nf.nus])
+    nf.set_values(f(X).reshape(-1, 1))
+    L = NW.build_newton_layout(nf)
+    x = np.stack([g.scale(c) for g, c in zip(ong, rng.uniform(-1, 1, (64, d)).T)], axis=1)
+    yr = ref(x)[:, 0]
+    y64 = NW.eval_newton_layout(L, x, fp32=False)
+    y32 = NW.eval_newton_layout(L, x, fp32=True)
+    scale = np.max(np.abs(yr))
+    assert np.max(np.abs(y64 - yr)) / scale < 1e-12
+    assert np.max(np.abs(y32 - yr)) / scale < 1e-6
+    # grid nodes are interpolated
+    assert np.max(np.abs(NW.eval_newton_layout(L, X[:50]) - f(X[:50]))) < 1e-12 * max(1, np.max(np.abs(f(X))))
+    # every node is in exactly one group and its parent row precedes it in depth
+    g4 = L.grp.reshape(-1, 4)
+    assert g4[0, 0] == 0 and g4[-1, 1] == L.n_nodes and np.all(g4[1:, 0] == g4[:-1, 1])
+
+
+def test_hierarchize_polynomial_reproduction():
+    """Surpluses of a polynomial in P_Lambda vanish outside its support."""
+    from smolyax_amd import newton as NW
+    from smolyax_amd import nodes as SN
+
+    d = 3
+    k = np.log((2 + np.arange(d)) / np.log(2))
+    t = OI.find_approximate_threshold(k, 60, True)
+    nf = NW.NewtonForm(k, t, SN.Leja(dim=d), 1)
+    X = np.stack([nf.grid_point(nu, np.zeros(d)) for nu in nf.nus])
+    f = lambda x: 1.0 + 2.0 * x[:, 0] - 0.5 * x[:, 1] * x[:, 0]
+    nf.set_values(f(X).reshape(-1, 1))
+    nz = {nu for nu, s in zip(nf.nus, nf.s[:, 0]) if abs(s) > 1e-13}
+    assert nz <= {(), ((0, 1),), ((0, 1), (1, 1))}

@@ -40,9 +40,13 @@ def test_layout_matches_oracle(kind, d, d_out, m):
     y = layout_eval.eval_layout(L, F, off, x)
     yr = ref(x)
     assert np.max(np.abs(y - yr)) / np.max(np.abs(yr)) < 1e-12
-    # runs are contiguous in the streamed slot and the wave split covers every run once
-    assert L.wave_run[0] == 0 and L.wave_run[-1] == len(L.run_slot)
-    assert np.all(np.diff(L.wave_run) >= 0)
+    # the wave segments cover every block exactly once, each inside one run of its streamed slot
+    covered = np.zeros(len(L.blk_n), dtype=int)
+    for g in range(L.seg_off[-1]):
+        covered[L.seg_b0[g]:L.seg_b1[g]] += 1
+        assert np.all(L.blk_last_slot[L.seg_b0[g]:L.seg_b1[g]] == L.seg_slot[g])
+    assert np.all(covered == 1)
+    assert len(L.seg_off) == packing.N_WAVES + 1
 
 
 def test_vectorized_f_matches_pointwise():
