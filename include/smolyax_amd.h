@@ -171,6 +171,20 @@ typedef struct sbi_newton_desc {
   const float* node_s32;      /* [n_nodes] surpluses, FP32 (mixed path) */
   const double* node_s64;     /* [n_nodes] surpluses, FP64 */
   const int32_t* node_q;      /* [n_nodes] table row of the node's parent */
+  /* staged mixed kernel (all optional: NULL selects the unstaged kernel) */
+  int32_t n_nodes_pad;        /* nodes with every group padded to a multiple of 4 */
+  int32_t n_hs_dims;          /* dims of the internal nodes' factors */
+  int32_t n_hs_rows;          /* rows of the per-tile h table */
+  int32_t _pad;
+  const float* node_s32p;     /* [n_nodes_pad] surpluses FP32, padded groups (pads are 0) */
+  const uint16_t* node_q16p;  /* [n_nodes_pad] parent table rows, padded groups */
+  const void* grp_rec;        /* [n_groups] 32-byte records {int32 n0, n1, dim, deg;
+                                 double xi_{deg-1}, c_deg} over the padded node ranges */
+  const int32_t* grp_j;       /* [n_groups] dim of each group */
+  const int32_t* hs_dim;      /* [n_hs_dims] */
+  const int32_t* hs_kmax;     /* [n_hs_dims] max deg of the dim among internal factors */
+  const int32_t* hs_row;      /* [n_hs_dims] first h-table row of the dim */
+  const int32_t* int_rec;     /* [n_internal] parent row | (h-table row << 16) */
 } sbi_newton_desc;
 
 typedef struct sbi_newton_plan sbi_newton_plan;

@@ -67,6 +67,19 @@ __device__ __forceinline__ double bcast(double v, int l) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// A window of 8 consecutive coordinates of this lane's point, fetched with independent loads
+// and addressed only with compile-time indices (so it stays in registers).
+constexpr int kXW = 8;
+struct XWin {
+  double v[kXW];
+};
+__device__ __forceinline__ XWin load_window(const double* __restrict__ xp, int j0, int d_in) {
+  XWin w;
+#pragma unroll
+  for (int i = 0; i < kXW; ++i) w.v[i] = (j0 + i < d_in) ? xp[j0 + i] : 0.0;
+  return w;
+}
+
 // h_k(x) = c_k prod_{r<k} (x - xi_r) in FP64
 __device__ __forceinline__ double newton_h(double x, cptr<double> xi, cptr<double> c, int k) {
   double p = c[k];
@@ -90,18 +103,20 @@ __global__ __launch_bounds__(kTile* kWaves) void newton_eval_kernel(sbi_newton_d
   const double* __restrict__ xp = x + p * ldx;
 
   cptr<int32_t> int_parent = C(d.int_parent), int_dim = C(d.int_dim), int_deg = C(d.int_deg),
-                level_off = C(d.level_off), grp = C(d.grp), grp_next_dim = C(d.grp_next_dim),
-                wave_grp = C(d.wave_grp);
+                level_off = C(d.level_off), grp = C(d.grp), wave_grp = C(d.wave_grp);
   cptr<double> dim_xi = C(d.dim_xi), dim_c = C(d.dim_c);
   const T* __restrict__ node_s =
       (const T*)(sizeof(T) == 4 ? (const void*)d.node_s32 : (const void*)d.node_s64);
   const int32_t* __restrict__ node_q = d.node_q;
 
-  // first streamed coordinate of this wave, requested before the table work
+  // the wave streams its dims in increasing order through two windows of coordinates (current,
+  // next); both are requested before the table work so their latency overlaps it
   const int g_begin = wave_grp[wave];
   const int g_end = wave_grp[wave + 1];
-  double x_next = 0.0;
-  if (g_begin < g_end) x_next = xp[grp[4 * g_begin + 2]];
+  const int d_in = (int)d.d_in;
+  int wj0 = (g_begin < g_end) ? (grp[4 * g_begin + 2] & ~(kXW - 1)) : 0;
+  XWin xa = load_window(xp, wj0, d_in);
+  XWin xb = load_window(xp, wj0 + kXW, d_in);
 
   // 1. table of internal-node products, level by level (row 0 = root = 1)
   if (wave == 0) W[lane] = (T)1;
@@ -118,73 +133,84 @@ __global__ __launch_bounds__(kTile* kWaves) void newton_eval_kernel(sbi_newton_d
     __syncthreads();
   }
 
-  // 2. streamed groups
+  // 2. streamed groups, in increasing dim order, consumed window by window
   double acc = 0.0;
   if (g_begin < g_end) {
-    const int n_first = grp[4 * g_begin];
-    int chunk = n_first >> 6;
-    // current and next metadata chunks (one node per lane)
     const int nn = d.n_nodes;
-    int i0 = chunk * 64 + lane;
+    int chunk = grp[4 * g_begin] >> 6;
+    // current and next metadata chunks (one node per lane)
+    const int i0 = chunk * 64 + lane;
     T s_cur = (i0 < nn) ? node_s[i0] : (T)0;
     int q_cur = (i0 < nn) ? node_q[i0] : 0;
-    int i1 = i0 + 64;
-    T s_nxt = (i1 < nn) ? node_s[i1] : (T)0;
-    int q_nxt = (i1 < nn) ? node_q[i1] : 0;
+    T s_nxt = (i0 + 64 < nn) ? node_s[i0 + 64] : (T)0;
+    int q_nxt = (i0 + 64 < nn) ? node_q[i0 + 64] : 0;
 
-    double xj = 0.0, pr = 1.0;
-    int jcur = -1, kdone = 0;  // pr = prod_{r < kdone} (x_j - xi_r)
-    for (int g = g_begin; g < g_end; ++g) {
-      const int n0 = grp[4 * g], n1 = grp[4 * g + 1], j = grp[4 * g + 2], k = grp[4 * g + 3];
-      cptr<double> xi = dim_xi + (int64_t)j * d.kmax1;
-      if (j != jcur) {
-        jcur = j;
-        xj = x_next;
-        const int jn = grp_next_dim[g];
-        if (jn >= 0) x_next = xp[jn];
-        pr = 1.0;
-        kdone = 0;
+    int g = g_begin;
+    while (g < g_end) {
+#pragma unroll
+      for (int u = 0; u < kXW; ++u) {
+        const int j = wj0 + u;
+        const double xj = xa.v[u];
+        double pr = 1.0;
+        int kdone = 0;  // pr = prod_{r < kdone} (x_j - xi_r)
+        cptr<double> xi = dim_xi + (int64_t)j * d.kmax1;
+        cptr<double> cj = dim_c + (int64_t)j * d.kmax1;
+        // the groups of dim j come in increasing deg k
+        while (g < g_end && grp[4 * g + 2] == j) {
+          const int n0 = grp[4 * g], n1 = grp[4 * g + 1], k = grp[4 * g + 3];
+          for (; kdone < k; ++kdone) pr *= (xj - xi[kdone]);
+          const double h = pr * cj[k];
+          T t = (T)0;
+          int i = n0;
+          while (i < n1) {
+            if ((i >> 6) != chunk) {
+              // groups are visited in node order, so this is always the next chunk
+              chunk += 1;
+              s_cur = s_nxt;
+              q_cur = q_nxt;
+              const int in = (chunk + 1) * 64 + lane;
+              s_nxt = (in < nn) ? node_s[in] : (T)0;
+              q_nxt = (in < nn) ? node_q[in] : 0;
+            }
+            const int lim = min(n1, (chunk + 1) * 64);
+            // batches of 4 nodes: broadcast metadata, gather parents' table entries, accumulate
+            for (; i + 4 <= lim; i += 4) {
+              const int l = i & 63;
+              const T s0 = bcast(s_cur, l), s1 = bcast(s_cur, l + 1), s2 = bcast(s_cur, l + 2),
+                      s3 = bcast(s_cur, l + 3);
+              const int q0 = __builtin_amdgcn_readlane(q_cur, l);
+              const int q1 = __builtin_amdgcn_readlane(q_cur, l + 1);
+              const int q2 = __builtin_amdgcn_readlane(q_cur, l + 2);
+              const int q3 = __builtin_amdgcn_readlane(q_cur, l + 3);
+              const T w0 = W[q0 * kTile + lane], w1 = W[q1 * kTile + lane];
+              const T w2 = W[q2 * kTile + lane], w3 = W[q3 * kTile + lane];
+              t = fma(s0, w0, t);
+              t = fma(s1, w1, t);
+              t = fma(s2, w2, t);
+              t = fma(s3, w3, t);
+            }
+            for (; i < lim; ++i) {
+              const int l = i & 63;
+              t = fma(bcast(s_cur, l), W[__builtin_amdgcn_readlane(q_cur, l) * kTile + lane], t);
+            }
+          }
+          acc = fma(h, (double)t, acc);
+          ++g;
+        }
       }
-      // groups of one dim come in increasing k; a wave may start inside a dim's run of groups
-      for (; kdone < k; ++kdone) pr *= (xj - xi[kdone]);
-      const double h = pr * dim_c[(int64_t)j * d.kmax1 + k];
-      T t = (T)0;
-      int i = n0;
-      while (i < n1) {
-        if ((i >> 6) != chunk) {
-          // advance one chunk (groups are visited in node order, so this is the next one)
-          chunk += 1;
-          s_cur = s_nxt;
-          q_cur = q_nxt;
-          const int in = (chunk + 1) * 64 + lane;
-          s_nxt = (in < nn) ? node_s[in] : (T)0;
-          q_nxt = (in < nn) ? node_q[in] : 0;
-        }
-        const int lim = min(n1, (chunk + 1) * 64);
-        // batches of 4 nodes: broadcast metadata, gather the parents' table entries, accumulate
-        for (; i + 4 <= lim; i += 4) {
-          const int l = i & 63;
-          const T s0 = bcast(s_cur, l), s1 = bcast(s_cur, l + 1), s2 = bcast(s_cur, l + 2),
-                  s3 = bcast(s_cur, l + 3);
-          const int q0 = __builtin_amdgcn_readlane(q_cur, l);
-          const int q1 = __builtin_amdgcn_readlane(q_cur, l + 1);
-          const int q2 = __builtin_amdgcn_readlane(q_cur, l + 2);
-          const int q3 = __builtin_amdgcn_readlane(q_cur, l + 3);
-          const T w0 = W[q0 * kTile + lane], w1 = W[q1 * kTile + lane];
-          const T w2 = W[q2 * kTile + lane], w3 = W[q3 * kTile + lane];
-          t = fma(s0, w0, t);
-          t = fma(s1, w1, t);
-          t = fma(s2, w2, t);
-          t = fma(s3, w3, t);
-        }
-        for (; i < lim; ++i) {
-          const int l = i & 63;
-          const T s0 = bcast(s_cur, l);
-          const int q0 = __builtin_amdgcn_readlane(q_cur, l);
-          t = fma(s0, W[q0 * kTile + lane], t);
+      // next window: slide (requesting the one after) or jump over a gap in the dims
+      if (g < g_end) {
+        const int jn = grp[4 * g + 2];
+        if (jn >= wj0 + 2 * kXW) {
+          wj0 = jn & ~(kXW - 1);
+          xa = load_window(xp, wj0, d_in);
+          xb = load_window(xp, wj0 + kXW, d_in);
+        } else {
+          wj0 += kXW;
+          xa = xb;
+          xb = load_window(xp, wj0 + kXW, d_in);
         }
       }
-      acc = fma(h, (double)t, acc);
     }
   }
 
@@ -201,11 +227,171 @@ __global__ __launch_bounds__(kTile* kWaves) void newton_eval_kernel(sbi_newton_d
   }
 }
 
+
+// ------------------------------------------------------------------------------------------
+// Staged mixed-precision kernel.  Everything the streaming loop reads is placed in LDS once per
+// tile by asynchronous global->LDS copies (global_load_lds_dwordx4), so the loop's only memory
+// operations are LDS reads (in order, batched by the compiler) and the per-lane coordinate
+// windows, which are requested one window ahead.  LDS layout:
+//   [W table: (n_internal+1) x 64 floats][region R]
+//   region R, phase 1: h table of the internal nodes' factors (n_hs_rows x 64 floats)
+//   region R, phase 2: surpluses (n_pad floats) | parent rows (n_pad u16) | group records
+// Group records (32 bytes): first node, end node, dim, deg, a = xi_{deg-1}, c = c_deg; node
+// ranges are padded to multiples of 4 so each batch of 4 is one aligned broadcast read.
+// ------------------------------------------------------------------------------------------
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) float lds_f32;
+typedef __attribute__((address_space(3))) double lds_f64;
+typedef __attribute__((address_space(3))) void lds_void;
+
+__device__ __forceinline__ void stage_to_lds(const void* src, lds_void* dst, int nbytes, int wave,
+                                             int lane) {
+  // nbytes is a multiple of 16; each wave copies 1 KiB chunks, one 16-byte piece per lane
+  const int nchunks = (nbytes + 1023) >> 10;
+  for (int c = wave; c < nchunks; c += kWaves) {
+    const int off = c * 1024 + lane * 16;
+    if (off < nbytes)
+      __builtin_amdgcn_global_load_lds((const char*)src + off,
+                                       (lds_void*)((__attribute__((address_space(3))) char*)dst +
+                                                   c * 1024),
+                                       16, 0, 0);
+  }
+}
+
+__global__ __launch_bounds__(kTile* kWaves) void newton_mixed_staged_kernel(
+    sbi_newton_desc d, const double* __restrict__ x, int64_t P, int64_t ldx, double* __restrict__ y) {
+  extern __shared__ double lds_raw[];
+  lds_f32* W = (lds_f32*)lds_raw;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t p_raw = (int64_t)blockIdx.x * kTile + lane;
+  const bool valid = p_raw < P;
+  const int64_t p = valid ? p_raw : (P - 1);
+  const double* __restrict__ xp = x + p * ldx;
+  const int d_in = (int)d.d_in;
+  const int n_w = d.n_internal + 1;
+  lds_f32* R = W + n_w * kTile;  // region R (16-byte aligned: n_w * 256 bytes)
+
+  cptr<int32_t> hs_dim = C(d.hs_dim), hs_kmax = C(d.hs_kmax), hs_row = C(d.hs_row),
+                int_rec = C(d.int_rec), level_off = C(d.level_off), wave_grp = C(d.wave_grp),
+                grp_j = C(d.grp_j);
+  cptr<double> dim_xi = C(d.dim_xi), dim_c = C(d.dim_c);
+
+  // coordinate windows of the wave's first dims, requested before the table work
+  const int g_begin = wave_grp[wave];
+  const int g_end = wave_grp[wave + 1];
+  int wj0 = (g_begin < g_end) ? (grp_j[g_begin] & ~(kXW - 1)) : 0;
+  XWin xa = load_window(xp, wj0, d_in);
+  XWin xb = load_window(xp, wj0 + kXW, d_in);
+
+  // 1a. h values of the internal nodes' factors: h_k(x_j) for k = 1..kmax(j), FP64 -> FP32
+  for (int i = wave; i < d.n_hs_dims; i += kWaves) {
+    const int j = hs_dim[i], km = hs_kmax[i], row = hs_row[i];
+    const double xj = xp[j];
+    cptr<double> xi = dim_xi + (int64_t)j * d.kmax1;
+    cptr<double> cj = dim_c + (int64_t)j * d.kmax1;
+    double pr = 1.0;
+    for (int k = 1; k <= km; ++k) {
+      pr *= (xj - xi[k - 1]);
+      R[(row + k - 1) * kTile + lane] = (float)(pr * cj[k]);
+    }
+  }
+  if (wave == 0) W[lane] = 1.0f;
+  __syncthreads();
+  // 1b. table of internal-node products, level by level: W[q] = W[parent] * h
+  for (int lv = 0; lv < d.n_levels; ++lv) {
+    const int a = level_off[lv], b = level_off[lv + 1];
+    for (int q = a + wave; q < b; q += kWaves) {
+      const int rec = int_rec[q];
+      const int par = rec & 0xffff, hr = (rec >> 16) & 0xffff;
+      W[(q + 1) * kTile + lane] = W[par * kTile + lane] * R[hr * kTile + lane];
+    }
+    __syncthreads();
+  }
+
+  // 2. stage surpluses, parent rows and group records into region R
+  const int n_pad = d.n_nodes_pad;
+  lds_f32* S = R;
+  const __attribute__((address_space(3))) uint16_t* Q =
+      (const __attribute__((address_space(3))) uint16_t*)(S + n_pad);
+  const int q_bytes = ((n_pad * 2) + 15) & ~15;
+  lds_void* G = (lds_void*)((__attribute__((address_space(3))) char*)(S + n_pad) + q_bytes);
+  stage_to_lds(d.node_s32p, (lds_void*)S, n_pad * 4, wave, lane);
+  stage_to_lds(d.node_q16p, (lds_void*)Q, q_bytes, wave, lane);
+  stage_to_lds(d.grp_rec, G, d.n_groups * 32, wave, lane);
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed
+  __syncthreads();
+  const __attribute__((address_space(3))) v4i* GI = (const __attribute__((address_space(3))) v4i*)G;
+  const __attribute__((address_space(3))) v2d* GD = (const __attribute__((address_space(3))) v2d*)G;
+  const __attribute__((address_space(3))) v4f* S4 = (const __attribute__((address_space(3))) v4f*)S;
+  const __attribute__((address_space(3))) v2u* Q4 = (const __attribute__((address_space(3))) v2u*)Q;
+
+  // 3. streamed groups in increasing dim order, window by window (static window slots)
+  double acc = 0.0;
+  int g = g_begin;
+  while (g < g_end) {
+#pragma unroll
+    for (int u = 0; u < kXW; ++u) {
+      const int j = wj0 + u;
+      const double xj = xa.v[u];
+      double pr = 1.0;
+      while (g < g_end) {
+        const v4i ri = GI[2 * g];
+        if (ri.z != j) break;
+        const v2d rd = GD[2 * g + 1];
+        pr *= (xj - rd.x);
+        const double h = pr * rd.y;
+        float t = 0.0f;
+        for (int i4 = ri.x >> 2; i4 < (ri.y >> 2); ++i4) {
+          const v4f s4 = S4[i4];
+          const v2u q4 = Q4[i4];
+          const float w0 = W[(q4.x & 0xffff) * kTile + lane];
+          const float w1 = W[(q4.x >> 16) * kTile + lane];
+          const float w2 = W[(q4.y & 0xffff) * kTile + lane];
+          const float w3 = W[(q4.y >> 16) * kTile + lane];
+          t = fmaf(s4.x, w0, t);
+          t = fmaf(s4.y, w1, t);
+          t = fmaf(s4.z, w2, t);
+          t = fmaf(s4.w, w3, t);
+        }
+        acc = fma(h, (double)t, acc);
+        ++g;
+      }
+    }
+    if (g < g_end) {
+      const int jn = grp_j[g];
+      if (jn >= wj0 + 2 * kXW) {
+        wj0 = jn & ~(kXW - 1);
+        xa = load_window(xp, wj0, d_in);
+        xb = load_window(xp, wj0 + kXW, d_in);
+      } else {
+        wj0 += kXW;
+        xa = xb;
+        xb = load_window(xp, wj0 + kXW, d_in);
+      }
+    }
+  }
+
+  // 4. deterministic cross-wave reduction in FP64 (W area reused)
+  __syncthreads();
+  lds_f64* Rd = (lds_f64*)lds_raw;
+  Rd[wave * kTile + lane] = acc;
+  __syncthreads();
+  if (wave == 0 && valid) {
+    double sacc = d.s_root[0];
+    for (int w = 0; w < kWaves; ++w) sacc += Rd[w * kTile + lane];
+    y[p] = sacc;
+  }
+}
+
 }  // namespace
 
 struct sbi_newton_plan {
   sbi_newton_desc d;
-  size_t lds_f32, lds_f64;
+  size_t lds_f32, lds_f64, lds_staged;
 };
 
 extern "C" int sbi_newton_plan_create(const sbi_newton_desc* desc, sbi_newton_plan** out) {
@@ -231,6 +417,18 @@ extern "C" int sbi_newton_plan_create(const sbi_newton_desc* desc, sbi_newton_pl
   p->d = d;
   p->lds_f32 = t32 > red ? t32 : red;
   p->lds_f64 = t64 > red ? t64 : red;
+  // staged kernel: W table + max(h table, surpluses + parent rows + group records)
+  {
+    const size_t w = (size_t)(d.n_internal + 1) * kTile * 4;
+    const size_t hs = (size_t)d.n_hs_rows * kTile * 4;
+    const size_t st = (size_t)d.n_nodes_pad * 4 + (((size_t)d.n_nodes_pad * 2 + 15) & ~(size_t)15) +
+                      (size_t)d.n_groups * 32;
+    size_t tot = w + (hs > st ? hs : st);
+    if (tot < red) tot = red;
+    p->lds_staged = (d.node_s32p && d.node_q16p && d.grp_rec && d.hs_dim && d.n_nodes_pad % 4 == 0)
+                        ? tot
+                        : (size_t)-1;
+  }
   if (p->lds_f32 > (size_t)kMaxLds) {
     delete p;
     sbi::set_error("sbi_newton_plan_create: too many internal nodes for the on-chip table");
@@ -276,7 +474,23 @@ static int newton_launch(const sbi_newton_plan* plan, const double* x, int64_t P
 
 extern "C" int sbi_newton_eval_mixed(const sbi_newton_plan* plan, const double* x, int64_t P,
                                      int64_t ldx, double* y, void* stream) {
-  return newton_launch<float>(plan, x, P, ldx, y, stream);
+  if (!plan || !x || !y || P < 0 || ldx < plan->d.d_in) {
+    sbi::set_error("sbi_newton_eval_mixed: invalid argument");
+    return SBI_ERR_INVALID;
+  }
+  if (plan->lds_staged > (size_t)kMaxLds) return newton_launch<float>(plan, x, P, ldx, y, stream);
+  if (P == 0) return SBI_OK;
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)newton_mixed_staged_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
+    attr = true;
+  }
+  dim3 grid((unsigned)((P + kTile - 1) / kTile));
+  hipLaunchKernelGGL(newton_mixed_staged_kernel, grid, dim3(kTile * kWaves), plan->lds_staged,
+                     (hipStream_t)stream, plan->d, x, P, ldx, y);
+  SBI_HIP_CHECK(hipGetLastError());
+  return SBI_OK;
 }
 
 extern "C" int sbi_newton_eval_f64(const sbi_newton_plan* plan, const double* x, int64_t P,

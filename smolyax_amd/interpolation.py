@@ -180,12 +180,21 @@ class SmolyakBarycentricInterpolator:
             "node_s32": _dev_tensor(NL.node_s32, f32, dev),
             "node_s64": _dev_tensor(NL.node_s64, f64, dev),
             "node_q": _dev_tensor(NL.node_q, i32, dev),
+            "node_s32p": _dev_tensor(NL.node_s32p, f32, dev),
+            "node_q16p": torch.as_tensor(NL.node_q16p.view(np.int16).copy()).to(dev),
+            "grp_rec": torch.as_tensor(NL.grp_rec.view(np.uint8).copy()).to(dev),
+            "grp_j": _dev_tensor(NL.grp_j, i32, dev),
+            "hs_dim": _dev_tensor(NL.hs_dim, i32, dev),
+            "hs_kmax": _dev_tensor(NL.hs_kmax, i32, dev),
+            "hs_row": _dev_tensor(NL.hs_row, i32, dev),
+            "int_rec": _dev_tensor(NL.int_rec, i32, dev),
         }
         self._ndev = t
         d = N.NewtonDesc()
         d.d_in, d.d_out = self._d_in, self._d_out
         d.n_nodes, d.n_internal, d.n_groups = NL.n_nodes, NL.n_internal, NL.n_groups
         d.n_split, d.n_levels, d.kmax1 = newton.N_WAVES, NL.n_levels, NL.kmax1
+        d.n_nodes_pad, d.n_hs_dims, d.n_hs_rows = NL.n_nodes_pad, NL.n_hs_dims, NL.n_hs_rows
         for name, ten in t.items():
             setattr(d, name, ten.data_ptr())
         self._ndesc = d

@@ -186,10 +186,18 @@ def build_newton_layout(nf: NewtonForm, n_waves: int = N_WAVES) -> NewtonLayout:
             v = (xi, newton_basis_consts(xi))
             cache[key] = v
         L.dim_xi[j], L.dim_c[j] = v
-    # cost-balanced split of the groups over the waves
+    # cost-balanced split of the groups over the waves, cut only at dim boundaries (a wave owns
+    # all groups of its dims, so the incremental Newton product starts at k = 1)
     cost = np.array([(b - a) + 6.0 for a, b, _, _ in grp]) if grp else np.zeros(0)
+    dim_start = [g for g in range(len(grp)) if g == 0 or grp[g - 1][2] != grp[g][2]] + [len(grp)]
     cum = np.concatenate([[0.0], np.cumsum(cost)])
-    cuts = [0] + [int(np.searchsorted(cum, cum[-1] * w / n_waves)) for w in range(1, n_waves)] + [len(grp)]
+    cuts = [0]
+    for w in range(1, n_waves):
+        target = cum[-1] * w / n_waves
+        pos = int(np.searchsorted(cum[dim_start], target))
+        pos = min(max(pos, 0), len(dim_start) - 1)
+        cuts.append(dim_start[pos])
+    cuts.append(len(grp))
     cuts = np.maximum.accumulate(np.clip(np.array(cuts), 0, len(grp)))
     L.wave_grp = cuts.astype(np.int32)
     nxt = np.full(max(len(grp), 1), -1, dtype=np.int32)
@@ -201,6 +209,42 @@ def build_newton_layout(nf: NewtonForm, n_waves: int = N_WAVES) -> NewtonLayout:
             if g == a or grp[g - 1][2] != grp[g][2]:
                 nd = grp[g][2]
     L.grp_next_dim = nxt
+
+    # staged kernel: groups padded to multiples of 4 nodes, 32-byte group records
+    s_pad, q_pad, rec = [], [], []
+    for a, b, j, k in grp:
+        n0 = len(s_pad)
+        s_pad.extend(L.node_s32[a:b].tolist())
+        q_pad.extend(L.node_q[a:b].tolist())
+        while len(s_pad) % 4:
+            s_pad.append(0.0)
+            q_pad.append(0)
+        rec.append((n0, len(s_pad), j, k, L.dim_xi[j, k - 1], L.dim_c[j, k]))
+    L.n_nodes_pad = len(s_pad)
+    L.node_s32p = np.array(s_pad or [0.0] * 4, dtype=np.float32)
+    assert max(q_pad or [0]) < 65536
+    L.node_q16p = np.array(q_pad or [0] * 4, dtype=np.uint16)
+    recdt = np.dtype([("n0", "<i4"), ("n1", "<i4"), ("j", "<i4"), ("k", "<i4"), ("a", "<f8"), ("c", "<f8")])
+    L.grp_rec = np.array(rec or [(0, 0, 0, 1, 0.0, 0.0)], dtype=recdt)
+    L.grp_j = np.array([r[2] for r in rec] or [0], dtype=np.int32)
+    # h table of the internal nodes' factors: per dim rows k = 1..kmax
+    hs_k = {}
+    for q in range(L.n_internal):
+        j, k = int(L.int_dim[q]), int(L.int_deg[q])
+        hs_k[j] = max(hs_k.get(j, 0), k)
+    hs_dims = sorted(hs_k)
+    rows, r = {}, 0
+    for j in hs_dims:
+        rows[j] = r
+        r += hs_k[j]
+    L.n_hs_dims, L.n_hs_rows = len(hs_dims), r
+    L.hs_dim = np.array(hs_dims or [0], dtype=np.int32)
+    L.hs_kmax = np.array([hs_k[j] for j in hs_dims] or [0], dtype=np.int32)
+    L.hs_row = np.array([rows[j] for j in hs_dims] or [0], dtype=np.int32)
+    irec = [int(L.int_parent[q]) | ((rows[int(L.int_dim[q])] + int(L.int_deg[q]) - 1) << 16)
+            for q in range(L.n_internal)]
+    assert L.n_internal + 1 < 65536 and r < 65536
+    L.int_rec = np.array(irec or [0], dtype=np.int32)
     return L
 
 
