@@ -172,14 +172,15 @@ typedef struct sbi_newton_desc {
   const double* node_s64;     /* [n_nodes] surpluses, FP64 */
   const int32_t* node_q;      /* [n_nodes] table row of the node's parent */
   /* staged mixed kernel (all optional: NULL selects the unstaged kernel) */
-  int32_t n_nodes_pad;        /* nodes with every group padded to a multiple of 4 */
+  int32_t n_nodes_pad;        /* nodes with every group padded to an even count */
   int32_t n_hs_dims;          /* dims of the internal nodes' factors */
   int32_t n_hs_rows;          /* rows of the per-tile h table */
   int32_t _pad;
-  const float* node_s32p;     /* [n_nodes_pad] surpluses FP32, padded groups (pads are 0) */
-  const uint16_t* node_q16p;  /* [n_nodes_pad] parent table rows, padded groups */
-  const void* grp_rec;        /* [n_groups] 32-byte records {int32 n0, n1, dim, deg;
-                                 double xi_{deg-1}, c_deg} over the padded node ranges */
+  const int32_t* node_sq;     /* [2 * n_nodes_pad] pairs {c_deg * surplus as FP32 bits, byte offset
+                                 of the parent's table row (row * 256)}; groups padded to even
+                                 counts with zero pairs */
+  const void* grp_rec;        /* [n_groups] 16-byte records {int32 n0 | n1 << 16, dim;
+                                 double xi_{deg-1}} over the padded node ranges */
   const int32_t* grp_j;       /* [n_groups] dim of each group */
   const int32_t* hs_dim;      /* [n_hs_dims] */
   const int32_t* hs_kmax;     /* [n_hs_dims] max deg of the dim among internal factors */

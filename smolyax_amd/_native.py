@@ -52,7 +52,7 @@ class NewtonDesc(ctypes.Structure):
         ("grp_next_dim", c_vp), ("wave_grp", c_vp), ("node_s32", c_vp), ("node_s64", c_vp),
         ("node_q", c_vp),
         ("n_nodes_pad", c_i32), ("n_hs_dims", c_i32), ("n_hs_rows", c_i32), ("_pad", c_i32),
-        ("node_s32p", c_vp), ("node_q16p", c_vp), ("grp_rec", c_vp), ("grp_j", c_vp),
+        ("node_sq", c_vp), ("grp_rec", c_vp), ("grp_j", c_vp),
         ("hs_dim", c_vp), ("hs_kmax", c_vp), ("hs_row", c_vp), ("int_rec", c_vp),
     ]
 

@@ -74,9 +74,11 @@ struct XWin {
   double v[kXW];
 };
 __device__ __forceinline__ XWin load_window(const double* __restrict__ xp, int j0, int d_in) {
+  // unconditional loads (index clamped into the row): a fixed number of loads in flight lets the
+  // compiler wait for exactly the older window instead of draining every outstanding load
   XWin w;
 #pragma unroll
-  for (int i = 0; i < kXW; ++i) w.v[i] = (j0 + i < d_in) ? xp[j0 + i] : 0.0;
+  for (int i = 0; i < kXW; ++i) w.v[i] = xp[min(j0 + i, d_in - 1)];
   return w;
 }
 
@@ -288,9 +290,20 @@ __global__ __launch_bounds__(kTile* kWaves) void newton_mixed_staged_kernel(
   XWin xb = load_window(xp, wj0 + kXW, d_in);
 
   // 1a. h values of the internal nodes' factors: h_k(x_j) for k = 1..kmax(j), FP64 -> FP32
+  double xh[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int i = wave + m * kWaves;
+    xh[m] = (i < d.n_hs_dims) ? xp[hs_dim[i]] : 0.0;
+  }
   for (int i = wave; i < d.n_hs_dims; i += kWaves) {
     const int j = hs_dim[i], km = hs_kmax[i], row = hs_row[i];
-    const double xj = xp[j];
+    const int m = (i - wave) / kWaves;
+    double xj = xh[0];
+#pragma unroll
+    for (int mm = 1; mm < 4; ++mm)
+      if (m == mm) xj = xh[mm];
+    if (m >= 4) xj = xp[j];
     cptr<double> xi = dim_xi + (int64_t)j * d.kmax1;
     cptr<double> cj = dim_c + (int64_t)j * d.kmax1;
     double pr = 1.0;
@@ -312,67 +325,91 @@ __global__ __launch_bounds__(kTile* kWaves) void newton_mixed_staged_kernel(
     __syncthreads();
   }
 
-  // 2. stage surpluses, parent rows and group records into region R
-  const int n_pad = d.n_nodes_pad;
-  lds_f32* S = R;
-  const __attribute__((address_space(3))) uint16_t* Q =
-      (const __attribute__((address_space(3))) uint16_t*)(S + n_pad);
-  const int q_bytes = ((n_pad * 2) + 15) & ~15;
-  lds_void* G = (lds_void*)((__attribute__((address_space(3))) char*)(S + n_pad) + q_bytes);
-  stage_to_lds(d.node_s32p, (lds_void*)S, n_pad * 4, wave, lane);
-  stage_to_lds(d.node_q16p, (lds_void*)Q, q_bytes, wave, lane);
-  stage_to_lds(d.grp_rec, G, d.n_groups * 32, wave, lane);
+  // 2. stage node pairs {c_deg * surplus (FP32 bits), parent byte offset} and group records
+  const int n_pad = d.n_nodes_pad;  // even
+  lds_void* SQ = (lds_void*)R;
+  lds_void* G = (lds_void*)((__attribute__((address_space(3))) char*)SQ + (size_t)n_pad * 8);
+  stage_to_lds(d.node_sq, SQ, n_pad * 8, wave, lane);
+  stage_to_lds(d.grp_rec, G, d.n_groups * 16, wave, lane);
   __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed
   __syncthreads();
   const __attribute__((address_space(3))) v4i* GI = (const __attribute__((address_space(3))) v4i*)G;
-  const __attribute__((address_space(3))) v2d* GD = (const __attribute__((address_space(3))) v2d*)G;
-  const __attribute__((address_space(3))) v4f* S4 = (const __attribute__((address_space(3))) v4f*)S;
-  const __attribute__((address_space(3))) v2u* Q4 = (const __attribute__((address_space(3))) v2u*)Q;
+  const __attribute__((address_space(3))) v4i* N2 = (const __attribute__((address_space(3))) v4i*)SQ;
+  const __attribute__((address_space(3))) char* Wb = (const __attribute__((address_space(3))) char*)W;
+  const int lane4 = lane * 4;
 
-  // 3. streamed groups in increasing dim order, window by window (static window slots)
+  // 3. streamed groups in increasing dim order, window by window (static window slots).  Two
+  // coordinate windows alternate roles (A is consumed while B is in flight and vice versa), so
+  // no register copy ever waits on a load issued for a later window.  The next group's record
+  // is read while the current group is processed; nodes are handled in batches of 8.
   double acc = 0.0;
   int g = g_begin;
-  while (g < g_end) {
+  // group record: {n0 | n1 << 16, dim, lo(a), hi(a)}, a = xi_{deg-1} (FP64)
+  v4i ri = (g < g_end) ? GI[g] : v4i{0, -1, 0, 0};
+
+  auto wld = [&](int off) { return *(const __attribute__((address_space(3))) float*)(Wb + off + lane4); };
+  auto consume = [&](const XWin& xw, const int w0) {
 #pragma unroll
     for (int u = 0; u < kXW; ++u) {
-      const int j = wj0 + u;
-      const double xj = xa.v[u];
+      const int j = w0 + u;
+      const double xj = xw.v[u];
       double pr = 1.0;
-      while (g < g_end) {
-        const v4i ri = GI[2 * g];
-        if (ri.z != j) break;
-        const v2d rd = GD[2 * g + 1];
-        pr *= (xj - rd.x);
-        const double h = pr * rd.y;
-        float t = 0.0f;
-        for (int i4 = ri.x >> 2; i4 < (ri.y >> 2); ++i4) {
-          const v4f s4 = S4[i4];
-          const v2u q4 = Q4[i4];
-          const float w0 = W[(q4.x & 0xffff) * kTile + lane];
-          const float w1 = W[(q4.x >> 16) * kTile + lane];
-          const float w2 = W[(q4.y & 0xffff) * kTile + lane];
-          const float w3 = W[(q4.y >> 16) * kTile + lane];
-          t = fmaf(s4.x, w0, t);
-          t = fmaf(s4.y, w1, t);
-          t = fmaf(s4.z, w2, t);
-          t = fmaf(s4.w, w3, t);
+      while (g < g_end && ri.y == j) {
+        const int gn = g + 1;
+        const v4i ri_n = (gn < g_end) ? GI[gn] : v4i{0, -1, 0, 0};
+        const double a = __hiloint2double(ri.w, ri.z);
+        pr *= (xj - a);
+        float t0 = 0.0f, t1 = 0.0f;
+        // node pairs: {s_a, off_a, s_b, off_b}; groups start and end on even nodes
+        const int e2 = ((unsigned)ri.x >> 16) >> 1;
+        int i2 = (ri.x & 0xffff) >> 1;
+        for (; i2 + 4 <= e2; i2 += 4) {
+          const v4i p0 = N2[i2], p1 = N2[i2 + 1], p2 = N2[i2 + 2], p3 = N2[i2 + 3];
+          t0 = fmaf(__int_as_float(p0.x), wld(p0.y), t0);
+          t1 = fmaf(__int_as_float(p0.z), wld(p0.w), t1);
+          t0 = fmaf(__int_as_float(p1.x), wld(p1.y), t0);
+          t1 = fmaf(__int_as_float(p1.z), wld(p1.w), t1);
+          t0 = fmaf(__int_as_float(p2.x), wld(p2.y), t0);
+          t1 = fmaf(__int_as_float(p2.z), wld(p2.w), t1);
+          t0 = fmaf(__int_as_float(p3.x), wld(p3.y), t0);
+          t1 = fmaf(__int_as_float(p3.z), wld(p3.w), t1);
         }
-        acc = fma(h, (double)t, acc);
-        ++g;
+        for (; i2 < e2; ++i2) {
+          const v4i p0 = N2[i2];
+          t0 = fmaf(__int_as_float(p0.x), wld(p0.y), t0);
+          t1 = fmaf(__int_as_float(p0.z), wld(p0.w), t1);
+        }
+        acc = fma(pr, (double)(t0 + t1), acc);
+        g = gn;
+        ri = ri_n;
       }
     }
-    if (g < g_end) {
-      const int jn = grp_j[g];
-      if (jn >= wj0 + 2 * kXW) {
-        wj0 = jn & ~(kXW - 1);
-        xa = load_window(xp, wj0, d_in);
-        xb = load_window(xp, wj0 + kXW, d_in);
-      } else {
-        wj0 += kXW;
-        xa = xb;
-        xb = load_window(xp, wj0 + kXW, d_in);
-      }
+  };
+
+  int wa = wj0, wb = wj0 + kXW;
+  while (g < g_end) {
+    consume(xa, wa);
+    if (g >= g_end) break;
+    if (ri.y >= wb + kXW) {  // gap past window B: restart both windows at the next dim
+      wa = ri.y & ~(kXW - 1);
+      wb = wa + kXW;
+      xa = load_window(xp, wa, d_in);
+      xb = load_window(xp, wb, d_in);
+      continue;
     }
+    wa = wb + kXW;
+    xa = load_window(xp, wa, d_in);  // refill A two windows ahead, in place
+    consume(xb, wb);
+    if (g >= g_end) break;
+    if (ri.y >= wa + kXW) {
+      wa = ri.y & ~(kXW - 1);
+      wb = wa + kXW;
+      xa = load_window(xp, wa, d_in);
+      xb = load_window(xp, wb, d_in);
+      continue;
+    }
+    wb = wa + kXW;
+    xb = load_window(xp, wb, d_in);  // refill B, in place
   }
 
   // 4. deterministic cross-wave reduction in FP64 (W area reused)
@@ -421,11 +458,10 @@ extern "C" int sbi_newton_plan_create(const sbi_newton_desc* desc, sbi_newton_pl
   {
     const size_t w = (size_t)(d.n_internal + 1) * kTile * 4;
     const size_t hs = (size_t)d.n_hs_rows * kTile * 4;
-    const size_t st = (size_t)d.n_nodes_pad * 4 + (((size_t)d.n_nodes_pad * 2 + 15) & ~(size_t)15) +
-                      (size_t)d.n_groups * 32;
+    const size_t st = (size_t)d.n_nodes_pad * 8 + (size_t)d.n_groups * 16;  // pairs + records
     size_t tot = w + (hs > st ? hs : st);
     if (tot < red) tot = red;
-    p->lds_staged = (d.node_s32p && d.node_q16p && d.grp_rec && d.hs_dim && d.n_nodes_pad % 4 == 0)
+    p->lds_staged = (d.node_sq && d.grp_rec && d.hs_dim && d.n_nodes_pad % 2 == 0)
                         ? tot
                         : (size_t)-1;
   }

@@ -180,8 +180,7 @@ class SmolyakBarycentricInterpolator:
             "node_s32": _dev_tensor(NL.node_s32, f32, dev),
             "node_s64": _dev_tensor(NL.node_s64, f64, dev),
             "node_q": _dev_tensor(NL.node_q, i32, dev),
-            "node_s32p": _dev_tensor(NL.node_s32p, f32, dev),
-            "node_q16p": torch.as_tensor(NL.node_q16p.view(np.int16).copy()).to(dev),
+            "node_sq": _dev_tensor(NL.node_sq, i32, dev),
             "grp_rec": torch.as_tensor(NL.grp_rec.view(np.uint8).copy()).to(dev),
             "grp_j": _dev_tensor(NL.grp_j, i32, dev),
             "hs_dim": _dev_tensor(NL.hs_dim, i32, dev),

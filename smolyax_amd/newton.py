@@ -210,23 +210,26 @@ def build_newton_layout(nf: NewtonForm, n_waves: int = N_WAVES) -> NewtonLayout:
                 nd = grp[g][2]
     L.grp_next_dim = nxt
 
-    # staged kernel: groups padded to multiples of 4 nodes, 32-byte group records
+    # staged kernel: groups padded to even counts; pairs {c_deg * s (FP32 bits), parent row
+    # byte offset}; 16-byte group records
     s_pad, q_pad, rec = [], [], []
     for a, b, j, k in grp:
         n0 = len(s_pad)
-        s_pad.extend(L.node_s32[a:b].tolist())
-        q_pad.extend(L.node_q[a:b].tolist())
-        while len(s_pad) % 4:
+        s_pad.extend((L.node_s64[a:b] * L.dim_c[j, k]).tolist())
+        q_pad.extend((L.node_q[a:b].astype(np.int64) * 256).tolist())
+        if len(s_pad) % 2:
             s_pad.append(0.0)
             q_pad.append(0)
-        rec.append((n0, len(s_pad), j, k, L.dim_xi[j, k - 1], L.dim_c[j, k]))
+        rec.append((n0 | (len(s_pad) << 16), j, L.dim_xi[j, k - 1]))
     L.n_nodes_pad = len(s_pad)
-    L.node_s32p = np.array(s_pad or [0.0] * 4, dtype=np.float32)
-    assert max(q_pad or [0]) < 65536
-    L.node_q16p = np.array(q_pad or [0] * 4, dtype=np.uint16)
-    recdt = np.dtype([("n0", "<i4"), ("n1", "<i4"), ("j", "<i4"), ("k", "<i4"), ("a", "<f8"), ("c", "<f8")])
-    L.grp_rec = np.array(rec or [(0, 0, 0, 1, 0.0, 0.0)], dtype=recdt)
-    L.grp_j = np.array([r[2] for r in rec] or [0], dtype=np.int32)
+    assert L.n_nodes_pad < 65536, "too many nodes for the staged kernel records"
+    sq = np.zeros((max(L.n_nodes_pad, 2), 2), dtype=np.int32)
+    sq[: L.n_nodes_pad, 0] = np.array(s_pad, dtype=np.float32).view(np.int32)
+    sq[: L.n_nodes_pad, 1] = np.array(q_pad, dtype=np.int32)
+    L.node_sq = sq.reshape(-1)
+    recdt = np.dtype([("n", "<u4"), ("j", "<i4"), ("a", "<f8")])
+    L.grp_rec = np.array(rec or [(0, 0, 0.0)], dtype=recdt)
+    L.grp_j = np.array([r[1] for r in rec] or [0], dtype=np.int32)
     # h table of the internal nodes' factors: per dim rows k = 1..kmax
     hs_k = {}
     for q in range(L.n_internal):
