@@ -186,6 +186,18 @@ typedef struct sbi_newton_desc {
   const int32_t* hs_kmax;     /* [n_hs_dims] max deg of the dim among internal factors */
   const int32_t* hs_row;      /* [n_hs_dims] first h-table row of the dim */
   const int32_t* int_rec;     /* [n_internal] parent row | (h-table row << 16) */
+  /* basis matrix and adjoint (GEMM-shaped paths, any d_out; optional) */
+  int32_t basis_n;            /* rows of H^T: all nodes of Lambda including the root */
+  int32_t n_tab_dims;         /* dims with a factor in some node */
+  int32_t n_tab_rows;         /* rows of the per-tile factor table: sum over dims of max deg */
+  int32_t _pad3;
+  const void* basis_desc;     /* [basis_n] 16 bytes: 8 x u16 factor-table rows (0xffff: none) */
+  const int32_t* tab_dim;     /* [n_tab_dims] */
+  const int32_t* tab_row;     /* [n_tab_dims] first table row of the dim (row of k is row+k-1) */
+  const int32_t* tab_kmax;    /* [n_tab_dims] */
+  const int32_t* vjp_slot_off;/* [n_tab_rows + 1] CSR over factor rows of the adjoint entries */
+  const void* vjp_ent;        /* 16 bytes per entry: int32 node row of G^T, 6 x u16 table rows
+                                 of the node's other factors (0xffff: none) */
 } sbi_newton_desc;
 
 typedef struct sbi_newton_plan sbi_newton_plan;
@@ -199,6 +211,20 @@ int sbi_newton_eval_mixed(const sbi_newton_plan* plan, const double* x, int64_t 
 /* Same in FP64 throughout. */
 int sbi_newton_eval_f64(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
                         double* y, void* stream);
+
+/* Basis matrix H^T[basis_n, ldh] of the Newton form at x[P, ldx] (dtype 0: FP64, 1: FP32,
+ * 2: BF16 rounded from FP32 values); y = H S is then one dense GEMM (large d_out). */
+int sbi_newton_basis(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx, void* HT,
+                     int64_t ldh, int32_t dtype, void* stream);
+/* Adjoint of the Newton form (the VJP of __call__, gradient path of interpolation.py:306-345):
+ * gx[P, d_in] = sum_nu G[p, nu] d H[p, nu] / dx  with G^T[basis_n, ldg] = S V^T given
+ * (dtype 0: FP64, 1: FP32).  Deterministic (no atomics). */
+int sbi_newton_vjp(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
+                   const void* GT, int64_t ldg, int32_t dtype, double* gx, void* stream);
+/* Barycentric entry weights W^T[k_raw, ldw] (FP64) at x[P, ldx]: y = offset + W F, the
+ * reference's per-block contraction (barycentric.py:69-123) as one dense GEMM (large d_out). */
+int sbi_bary_basis_f64(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx, double* WT,
+                       int64_t ldw, void* stream);
 
 #ifdef __cplusplus
 }

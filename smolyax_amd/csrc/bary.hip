@@ -510,6 +510,12 @@ int sbi_bary_plan_create(const sbi_bary_desc* desc, sbi_plan** out) {
   return SBI_OK;
 }
 
+int sbi_bary_plan_desc(const sbi_plan* plan, const sbi_bary_desc** out) {
+  if (!plan || !out || plan->kind != 1) return SBI_ERR_INVALID;
+  *out = &plan->bary;
+  return SBI_OK;
+}
+
 int sbi_plan_destroy(sbi_plan* plan) {
   delete plan;
   return SBI_OK;

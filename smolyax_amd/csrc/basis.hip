@@ -1,0 +1,383 @@
+// Basis matrices and the adjoint for the GEMM-shaped paths (large d_out), gfx950.
+//
+// Newton form (nested nodes): y[p, :] = sum_nu H[p, nu] S[nu, :] with
+//   H[p, nu] = prod_{(j,k) in nu} h^{(j)}_k(x_{p,j})     (H[p, root] = 1).
+// newton_basis_kernel writes H^T (nodes x points, points contiguous) so that the d_out
+// contraction Y = H S is one dense GEMM (done by the caller's BLAS on MFMA in FP64, FP32 or
+// BF16 with FP32 accumulation).
+//
+// Adjoint (vector-Jacobian product) of the same form: with G = V S^T (a GEMM, V = dL/dy),
+//   dL/dx_{p,j} = sum_k h'^{(j)}_k(x_{p,j}) A[p, (j,k)],
+//   A[p, (j,k)] = sum_{nu containing (j,k)} G[p, nu] prod_{other factors (j',k') of nu} h_{k'}(x_{p,j'}).
+// newton_vjp_kernel evaluates it per tile from G^T, one dim per wave at a time (no atomics: each
+// (p, j) is written once).
+//
+// Barycentric form (the reference's algorithm): bary_basis_kernel writes the per-entry weights
+// W^T[e, p] = prod_i Lhat_i(x_{p, s_i})[mu_i] of every block entry e (K_raw rows) so that
+// y = offset + W F (zeta folded into F) is a dense FP64 GEMM.
+//
+// Tile = 64 points, one per lane; the 1D factors h_k(x_j) of the tile live in an LDS table.
+
+#include <hip/hip_bf16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/smolyax_amd.h"
+#include "common.h"
+
+#define SBI_HIP_CHECK(expr)                                                              \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess) {                                                              \
+      sbi::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));                 \
+      return SBI_ERR_HIP;                                                                \
+    }                                                                                    \
+  } while (0)
+
+namespace {
+
+constexpr int kTile = 64;
+constexpr int kBWaves = 4;
+constexpr int kMaxLds = 160 * 1024;
+
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T*;
+template <typename T>
+__device__ __forceinline__ cptr<T> C(const T* p) {
+  return (cptr<T>)(p);
+}
+
+template <typename T>
+struct lds_t;
+template <>
+struct lds_t<float> {
+  typedef __attribute__((address_space(3))) float type;
+};
+template <>
+struct lds_t<double> {
+  typedef __attribute__((address_space(3))) double type;
+};
+
+// Table of h_k(x_j) (and optionally h'_k(x_j)) for all table dims of the tile.  Row of (j, k):
+// tab_row[i] + k - 1 for table dim i.  FP64 arithmetic, stored as T.
+template <typename T>
+__device__ void build_h_table(const sbi_newton_desc& d, const double* __restrict__ xp,
+                              typename lds_t<T>::type* tab, typename lds_t<T>::type* dtab,
+                              int wave, int lane, int nwaves) {
+  cptr<int32_t> tdim = C(d.tab_dim), trow = C(d.tab_row), tk = C(d.tab_kmax);
+  cptr<double> dim_xi = C(d.dim_xi), dim_c = C(d.dim_c);
+  for (int i = wave; i < d.n_tab_dims; i += nwaves) {
+    const int j = tdim[i], row = trow[i], km = tk[i];
+    const double xj = xp[j];
+    cptr<double> xi = dim_xi + (int64_t)j * d.kmax1;
+    cptr<double> c = dim_c + (int64_t)j * d.kmax1;
+    // pr = prod_{r<k} (x - xi_r);  dpr = d/dx pr
+    double pr = 1.0, dpr = 0.0;
+    for (int k = 1; k <= km; ++k) {
+      const double dx = xj - xi[k - 1];
+      dpr = dpr * dx + pr;
+      pr *= dx;
+      tab[(row + k - 1) * kTile + lane] = (T)(pr * c[k]);
+      if (dtab) dtab[(row + k - 1) * kTile + lane] = (T)(dpr * c[k]);
+    }
+  }
+}
+
+__device__ __forceinline__ void store_out(double* p, double v) { *p = v; }
+__device__ __forceinline__ void store_out(float* p, double v) { *p = (float)v; }
+__device__ __forceinline__ void store_out(__hip_bfloat16* p, double v) {
+  *p = __float2bfloat16((float)v);
+}
+
+// H^T[nu, p] for a chunk of nodes.  Node descriptor: 8 x u16 table rows (0xffff = none).
+template <typename T, typename OUT>
+__global__ __launch_bounds__(kTile* kBWaves) void newton_basis_kernel(sbi_newton_desc d,
+                                                                      const double* __restrict__ x,
+                                                                      int64_t P, int64_t ldx,
+                                                                      OUT* __restrict__ HT,
+                                                                      int64_t ldh, int chunk) {
+  extern __shared__ double lds_raw[];
+  typedef typename lds_t<T>::type LT;
+  LT* tab = (LT*)lds_raw;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t p_raw = (int64_t)blockIdx.x * kTile + lane;
+  const bool valid = p_raw < P;
+  const int64_t p = valid ? p_raw : (P - 1);
+  build_h_table<T>(d, x + p * ldx, tab, nullptr, wave, lane, kBWaves);
+  __syncthreads();
+
+  const int n0 = blockIdx.y * chunk;
+  const int n1 = min(d.basis_n, n0 + chunk);
+  const uint4* __restrict__ desc = reinterpret_cast<const uint4*>(d.basis_desc);
+  // each wave: 64-node groups, descriptors one per lane, broadcast with v_readlane
+  for (int base = n0 + wave * 64; base < n1; base += kBWaves * 64) {
+    const int mine = base + lane;
+    const uint4 dv = (mine < n1) ? desc[mine] : make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu,
+                                                             0xffffffffu);
+    const int cnt = min(64, n1 - base);
+    for (int i = 0; i < cnt; ++i) {
+      const uint32_t w[4] = {(uint32_t)__builtin_amdgcn_readlane((int)dv.x, i),
+                             (uint32_t)__builtin_amdgcn_readlane((int)dv.y, i),
+                             (uint32_t)__builtin_amdgcn_readlane((int)dv.z, i),
+                             (uint32_t)__builtin_amdgcn_readlane((int)dv.w, i)};
+      T v = (T)1;
+#pragma unroll
+      for (int f = 0; f < 8; ++f) {
+        const uint32_t r = (w[f >> 1] >> ((f & 1) * 16)) & 0xffff;
+        if (r != 0xffff) v *= tab[r * kTile + lane];
+      }
+      if (valid) store_out(HT + (int64_t)(base + i) * ldh + p, (double)v);
+    }
+  }
+}
+
+// dL/dx from G^T (nodes x points).  Per table dim (one wave at a time): for each of its k's the
+// list of (node, other factor rows) entries; A accumulates in FP64.
+template <typename T>
+__global__ __launch_bounds__(kTile* kBWaves) void newton_vjp_kernel(sbi_newton_desc d,
+                                                                    const double* __restrict__ x,
+                                                                    int64_t P, int64_t ldx,
+                                                                    const T* __restrict__ GT,
+                                                                    int64_t ldg,
+                                                                    double* __restrict__ gx) {
+  extern __shared__ double lds_raw[];
+  typedef typename lds_t<T>::type LT;
+  LT* tab = (LT*)lds_raw;
+  LT* dtab = tab + (size_t)d.n_tab_rows * kTile;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t p_raw = (int64_t)blockIdx.x * kTile + lane;
+  const bool valid = p_raw < P;
+  const int64_t p = valid ? p_raw : (P - 1);
+  build_h_table<T>(d, x + p * ldx, tab, dtab, wave, lane, kBWaves);
+  __syncthreads();
+
+  cptr<int32_t> tdim = C(d.tab_dim), trow = C(d.tab_row), tk = C(d.tab_kmax),
+                slot_off = C(d.vjp_slot_off);
+  const uint4* __restrict__ ent = reinterpret_cast<const uint4*>(d.vjp_ent);
+  for (int i = wave; i < d.n_tab_dims; i += kBWaves) {
+    const int j = tdim[i], row = trow[i], km = tk[i];
+    double gj = 0.0;
+    for (int k = 1; k <= km; ++k) {
+      const int s = row + k - 1;
+      double A = 0.0;
+      const int e0 = slot_off[s], e1 = slot_off[s + 1];
+      for (int base = e0; base < e1; base += 64) {
+        const int mine = base + lane;
+        const uint4 ev = (mine < e1) ? ent[mine] : make_uint4(0u, 0xffffffffu, 0xffffffffu, 0xffffffffu);
+        const int cnt = min(64, e1 - base);
+        for (int q = 0; q < cnt; ++q) {
+          const int node = __builtin_amdgcn_readlane((int)ev.x, q);
+          const uint32_t w1 = (uint32_t)__builtin_amdgcn_readlane((int)ev.y, q);
+          const uint32_t w2 = (uint32_t)__builtin_amdgcn_readlane((int)ev.z, q);
+          const uint32_t w3 = (uint32_t)__builtin_amdgcn_readlane((int)ev.w, q);
+          const uint32_t ws[3] = {w1, w2, w3};
+          double v = (double)GT[(int64_t)node * ldg + p];
+#pragma unroll
+          for (int f = 0; f < 6; ++f) {
+            const uint32_t r = (ws[f >> 1] >> ((f & 1) * 16)) & 0xffff;
+            if (r != 0xffff) v *= (double)tab[r * kTile + lane];
+          }
+          A += v;
+        }
+      }
+      gj = fma((double)dtab[s * kTile + lane], A, gj);
+    }
+    if (valid) gx[p * d.d_in + j] = gj;
+  }
+}
+
+// Zero the gradient of dims that no factor touches (written separately so every entry of gx is
+// defined).
+__global__ void fill_zero_kernel(double* __restrict__ g, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) g[i] = 0.0;
+}
+
+// W^T[e, p] for the entries of a chunk of barycentric blocks.  Factor order on the device:
+// prefix factors, then the streamed one; entries in C order (last factor fastest).  Each factor
+// value Lhat_mu(x) = w_mu prod_{nu != mu} (x - xi_nu) is formed on the fly (exact one-hot at nodes).
+__device__ __forceinline__ double lagrange_first_form(double xv, cptr<double> xi, cptr<double> w,
+                                                      int deg, int mu) {
+  double v = w[mu];
+  bool hit_other = false, hit_self = false;
+  for (int q = 0; q <= deg; ++q) {
+    const double dq = xv - xi[q];
+    if (q == mu) {
+      hit_self = (dq == 0.0);
+    } else {
+      hit_other |= (dq == 0.0);
+      v *= dq;
+    }
+  }
+  if (hit_self) return 1.0;
+  if (hit_other) return 0.0;
+  return v;
+}
+
+__global__ __launch_bounds__(kTile) void bary_basis_kernel(sbi_bary_desc d,
+                                                          const double* __restrict__ x, int64_t P,
+                                                          int64_t ldx, double* __restrict__ WT,
+                                                          int64_t ldw, int chunk) {
+  const int lane = threadIdx.x;
+  const int64_t p_raw = (int64_t)blockIdx.x * kTile + lane;
+  const bool valid = p_raw < P;
+  const int64_t p = valid ? p_raw : (P - 1);
+  const double* __restrict__ xp = x + p * ldx;
+  cptr<int32_t> blk_fac = C(d.blk_fac), fac_slot = C(d.fac_slot), last = C(d.blk_last_slot),
+                slot_dim = C(d.slot_dim), slot_deg = C(d.slot_deg);
+  cptr<int64_t> foff = C(d.blk_foff);
+  cptr<double> nodes = C(d.slot_nodes), wts = C(d.slot_w);
+  const int b0 = blockIdx.y * chunk;
+  const int b1 = min(d.n_blocks, b0 + chunk);
+  for (int b = b0; b < b1; ++b) {
+    const int f0 = blk_fac[b], npre = blk_fac[b + 1] - f0, ls = last[b];
+    const int nf = npre + (ls >= 0 ? 1 : 0);
+    int slots[16];
+    double xs[16];
+    int64_t nent = 1;
+    for (int i = 0; i < nf && i < 16; ++i) {
+      slots[i] = (i < npre) ? fac_slot[f0 + i] : ls;
+      xs[i] = xp[slot_dim[slots[i]]];
+      nent *= slot_deg[slots[i]] + 1;
+    }
+    const int64_t r0 = foff[b];
+    for (int64_t e = 0; e < nent; ++e) {
+      int64_t rem = e;
+      double wgt = 1.0;
+      for (int i = nf - 1; i >= 0; --i) {
+        const int s = slots[i];
+        const int di1 = slot_deg[s] + 1;
+        const int mu = (int)(rem % di1);
+        rem /= di1;
+        wgt *= lagrange_first_form(xs[i], nodes + (int64_t)s * d.dmax1, wts + (int64_t)s * d.dmax1,
+                                   di1 - 1, mu);
+      }
+      if (valid) WT[(r0 + e) * ldw + p] = wgt;
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int sbi_bary_basis_f64(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx,
+                                  double* WT, int64_t ldw, void* stream);
+
+extern "C" int sbi_newton_plan_desc(const sbi_newton_plan* plan, const sbi_newton_desc** out);
+
+template <typename T, typename OUT>
+static int basis_launch(const sbi_newton_desc& d, const double* x, int64_t P, int64_t ldx, OUT* HT,
+                        int64_t ldh, hipStream_t st) {
+  const size_t lds = (size_t)d.n_tab_rows * kTile * sizeof(T);
+  if (lds > (size_t)kMaxLds) {
+    sbi::set_error("sbi_newton_basis: the tile's factor table exceeds LDS");
+    return SBI_ERR_CAPACITY;
+  }
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)newton_basis_kernel<T, OUT>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
+    attr = true;
+  }
+  const int chunk = 2048;
+  dim3 grid((unsigned)((P + kTile - 1) / kTile), (unsigned)((d.basis_n + chunk - 1) / chunk));
+  hipLaunchKernelGGL((newton_basis_kernel<T, OUT>), grid, dim3(kTile * kBWaves), lds, st, d, x, P,
+                     ldx, HT, ldh, chunk);
+  SBI_HIP_CHECK(hipGetLastError());
+  return SBI_OK;
+}
+
+extern "C" int sbi_newton_basis(const sbi_newton_plan* plan, const double* x, int64_t P,
+                                int64_t ldx, void* HT, int64_t ldh, int32_t dtype, void* stream) {
+  const sbi_newton_desc* dp = nullptr;
+  if (sbi_newton_plan_desc(plan, &dp) != SBI_OK || !x || !HT || P < 0 || ldh < P) {
+    sbi::set_error("sbi_newton_basis: invalid argument");
+    return SBI_ERR_INVALID;
+  }
+  const sbi_newton_desc& d = *dp;
+  if (!d.basis_desc || !d.tab_dim || ldx < d.d_in) {
+    sbi::set_error("sbi_newton_basis: plan has no basis tables");
+    return SBI_ERR_INVALID;
+  }
+  if (P == 0) return SBI_OK;
+  hipStream_t st = (hipStream_t)stream;
+  switch (dtype) {
+    case 0: return basis_launch<double, double>(d, x, P, ldx, (double*)HT, ldh, st);
+    case 1: return basis_launch<float, float>(d, x, P, ldx, (float*)HT, ldh, st);
+    case 2: return basis_launch<float, __hip_bfloat16>(d, x, P, ldx, (__hip_bfloat16*)HT, ldh, st);
+    default:
+      sbi::set_error("sbi_newton_basis: dtype must be 0 (f64), 1 (f32) or 2 (bf16)");
+      return SBI_ERR_INVALID;
+  }
+}
+
+template <typename T>
+static int vjp_launch(const sbi_newton_desc& d, const double* x, int64_t P, int64_t ldx,
+                      const T* GT, int64_t ldg, double* gx, hipStream_t st) {
+  const size_t lds = 2 * (size_t)d.n_tab_rows * kTile * sizeof(T);
+  if (lds > (size_t)kMaxLds) {
+    sbi::set_error("sbi_newton_vjp: the tile's factor tables exceed LDS");
+    return SBI_ERR_CAPACITY;
+  }
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)newton_vjp_kernel<T>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
+    attr = true;
+  }
+  const int64_t n = P * d.d_in;
+  hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, gx, n);
+  SBI_HIP_CHECK(hipGetLastError());
+  dim3 grid((unsigned)((P + kTile - 1) / kTile));
+  hipLaunchKernelGGL(newton_vjp_kernel<T>, grid, dim3(kTile * kBWaves), lds, st, d, x, P, ldx, GT,
+                     ldg, gx);
+  SBI_HIP_CHECK(hipGetLastError());
+  return SBI_OK;
+}
+
+extern "C" int sbi_newton_vjp(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
+                              const void* GT, int64_t ldg, int32_t dtype, double* gx,
+                              void* stream) {
+  const sbi_newton_desc* dp = nullptr;
+  if (sbi_newton_plan_desc(plan, &dp) != SBI_OK || !x || !GT || !gx || P < 0 || ldg < P) {
+    sbi::set_error("sbi_newton_vjp: invalid argument");
+    return SBI_ERR_INVALID;
+  }
+  const sbi_newton_desc& d = *dp;
+  if (!d.vjp_ent || !d.vjp_slot_off || !d.tab_dim || ldx < d.d_in) {
+    sbi::set_error("sbi_newton_vjp: plan has no adjoint tables");
+    return SBI_ERR_INVALID;
+  }
+  if (P == 0) return SBI_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == 0) return vjp_launch<double>(d, x, P, ldx, (const double*)GT, ldg, gx, st);
+  if (dtype == 1) return vjp_launch<float>(d, x, P, ldx, (const float*)GT, ldg, gx, st);
+  sbi::set_error("sbi_newton_vjp: dtype must be 0 (f64) or 1 (f32)");
+  return SBI_ERR_INVALID;
+}
+
+extern "C" int sbi_bary_plan_desc(const sbi_plan* plan, const sbi_bary_desc** out);
+
+extern "C" int sbi_bary_basis_f64(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx,
+                                  double* WT, int64_t ldw, void* stream) {
+  const sbi_bary_desc* dp = nullptr;
+  if (sbi_bary_plan_desc(plan, &dp) != SBI_OK || !x || !WT || P < 0 || ldw < P) {
+    sbi::set_error("sbi_bary_basis_f64: invalid argument");
+    return SBI_ERR_INVALID;
+  }
+  const sbi_bary_desc& d = *dp;
+  if (ldx < d.d_in || d.max_n > 16) {
+    sbi::set_error("sbi_bary_basis_f64: invalid argument (at most 16 active dims per block)");
+    return SBI_ERR_INVALID;
+  }
+  if (P == 0 || d.n_blocks == 0) return SBI_OK;
+  const int chunk = 64;
+  dim3 grid((unsigned)((P + kTile - 1) / kTile), (unsigned)((d.n_blocks + chunk - 1) / chunk));
+  hipLaunchKernelGGL(bary_basis_kernel, grid, dim3(kTile), 0, (hipStream_t)stream, d, x, P, ldx,
+                     WT, ldw, chunk);
+  SBI_HIP_CHECK(hipGetLastError());
+  return SBI_OK;
+}

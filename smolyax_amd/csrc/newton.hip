@@ -437,10 +437,10 @@ extern "C" int sbi_newton_plan_create(const sbi_newton_desc* desc, sbi_newton_pl
     return SBI_ERR_INVALID;
   }
   const sbi_newton_desc& d = *desc;
-  if (d.d_in <= 0 || d.d_out != 1 || d.n_nodes < 0 || d.n_internal < 0 || d.n_groups < 0 ||
+  if (d.d_in <= 0 || d.d_out < 1 || d.n_nodes < 0 || d.n_internal < 0 || d.n_groups < 0 ||
       !d.s_root || !d.wave_grp || !d.grp || !d.node_q || !d.node_s32 || !d.node_s64 ||
       d.kmax1 <= 0) {
-    sbi::set_error("sbi_newton_plan_create: invalid descriptor (this kernel takes d_out == 1)");
+    sbi::set_error("sbi_newton_plan_create: invalid descriptor");
     return SBI_ERR_INVALID;
   }
   if (d.n_split != kWaves) {
@@ -474,6 +474,12 @@ extern "C" int sbi_newton_plan_create(const sbi_newton_desc* desc, sbi_newton_pl
   return SBI_OK;
 }
 
+extern "C" int sbi_newton_plan_desc(const sbi_newton_plan* plan, const sbi_newton_desc** out) {
+  if (!plan || !out) return SBI_ERR_INVALID;
+  *out = &plan->d;
+  return SBI_OK;
+}
+
 extern "C" int sbi_newton_plan_destroy(sbi_newton_plan* plan) {
   delete plan;
   return SBI_OK;
@@ -482,8 +488,8 @@ extern "C" int sbi_newton_plan_destroy(sbi_newton_plan* plan) {
 template <typename T>
 static int newton_launch(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
                          double* y, void* stream) {
-  if (!plan || !x || !y || P < 0 || ldx < plan->d.d_in) {
-    sbi::set_error("sbi_newton_eval: invalid argument");
+  if (!plan || !x || !y || P < 0 || ldx < plan->d.d_in || plan->d.d_out != 1) {
+    sbi::set_error("sbi_newton_eval: invalid argument (the streaming kernels take d_out == 1)");
     return SBI_ERR_INVALID;
   }
   const size_t lds = sizeof(T) == 4 ? plan->lds_f32 : plan->lds_f64;
@@ -510,8 +516,8 @@ static int newton_launch(const sbi_newton_plan* plan, const double* x, int64_t P
 
 extern "C" int sbi_newton_eval_mixed(const sbi_newton_plan* plan, const double* x, int64_t P,
                                      int64_t ldx, double* y, void* stream) {
-  if (!plan || !x || !y || P < 0 || ldx < plan->d.d_in) {
-    sbi::set_error("sbi_newton_eval_mixed: invalid argument");
+  if (!plan || !x || !y || P < 0 || ldx < plan->d.d_in || plan->d.d_out != 1) {
+    sbi::set_error("sbi_newton_eval_mixed: invalid argument (the streaming kernel takes d_out == 1)");
     return SBI_ERR_INVALID;
   }
   if (plan->lds_staged > (size_t)kMaxLds) return newton_launch<float>(plan, x, P, ldx, y, stream);

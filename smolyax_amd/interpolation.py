@@ -16,7 +16,8 @@ from . import _native as N
 from . import indices, newton, packing
 from .nodes import Generator
 
-PRECISIONS = ("fp64", "mixed", "newton-fp64")
+PRECISIONS = ("fp64", "mixed", "newton-fp64", "bf16")
+GEMM_MIN_DOUT = 16  # from this d_out on, the d_out contraction is one dense GEMM
 
 
 def _dev_tensor(a, dtype, device):
@@ -48,9 +49,12 @@ class SmolyakBarycentricInterpolator:
         """Keyword-only, as interpolation.py:51-113.  Extensions: `device` (torch device) and
         `precision`, the default evaluation path of __call__:
           "fp64"        barycentric combination technique in FP64 (the reference's algorithm);
-          "mixed"       hierarchical Newton form, FP32 surpluses / products / group sums with FP64
-                        node differences and accumulation (nested nodes, d_out == 1);
-          "newton-fp64" hierarchical Newton form in FP64 (nested nodes, d_out == 1).
+          "mixed"       hierarchical Newton form with FP32 surpluses and products and FP64 node
+                        differences; d_out == 1: FP64 accumulation across groups; larger d_out:
+                        FP32 GEMM on MFMA (nested nodes);
+          "newton-fp64" hierarchical Newton form in FP64 (nested nodes);
+          "bf16"        hierarchical Newton form, BF16 basis and surpluses with FP32 accumulation
+                        on MFMA (nested nodes).
         """
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {PRECISIONS}")
@@ -90,14 +94,16 @@ class SmolyakBarycentricInterpolator:
         self._F_host = F
         self._offset_host = offset
         self._nlayout = None
-        if self._is_nested and self._d_out == 1:
+        self._nform = None
+        if self._is_nested:
             nf = newton.NewtonForm(self._k, self._t, self._node_gen, self._d_out)
             vals = np.array([np.broadcast_to(np.asarray(f_evals[nu], dtype=np.float64).reshape(-1),
                                              (self._d_out,)) for nu in nf.nus])
             nf.set_values(vals)
+            self._nform = nf
             self._nlayout = newton.build_newton_layout(nf)
         elif self._precision != "fp64":
-            raise ValueError(f"precision {self._precision!r} needs nested nodes and d_out == 1")
+            raise ValueError(f"precision {self._precision!r} needs nested nodes (Leja)")
         self._upload()
         if self._n_inputs is not None:
             x0 = torch.rand((self._n_inputs, self._d_in), dtype=torch.float64, device=self._device)
@@ -187,13 +193,22 @@ class SmolyakBarycentricInterpolator:
             "hs_kmax": _dev_tensor(NL.hs_kmax, i32, dev),
             "hs_row": _dev_tensor(NL.hs_row, i32, dev),
             "int_rec": _dev_tensor(NL.int_rec, i32, dev),
+            "basis_desc": torch.as_tensor(NL.basis_desc.copy()).to(dev),
+            "tab_dim": _dev_tensor(NL.tab_dim, i32, dev),
+            "tab_row": _dev_tensor(NL.tab_row, i32, dev),
+            "tab_kmax": _dev_tensor(NL.tab_kmax, i32, dev),
+            "vjp_slot_off": _dev_tensor(NL.vjp_slot_off, i32, dev),
+            "vjp_ent": torch.as_tensor(NL.vjp_ent.copy()).to(dev),
         }
+        # surpluses in the order of the basis rows (nf.nus), for the GEMM-shaped paths
+        self._S = {"f64": _dev_tensor(self._nform.s, f64, dev)}
         self._ndev = t
         d = N.NewtonDesc()
         d.d_in, d.d_out = self._d_in, self._d_out
         d.n_nodes, d.n_internal, d.n_groups = NL.n_nodes, NL.n_internal, NL.n_groups
         d.n_split, d.n_levels, d.kmax1 = newton.N_WAVES, NL.n_levels, NL.kmax1
         d.n_nodes_pad, d.n_hs_dims, d.n_hs_rows = NL.n_nodes_pad, NL.n_hs_dims, NL.n_hs_rows
+        d.basis_n, d.n_tab_dims, d.n_tab_rows = NL.basis_n, NL.n_tab_dims, NL.n_tab_rows
         for name, ten in t.items():
             setattr(d, name, ten.data_ptr())
         self._ndesc = d
@@ -242,21 +257,109 @@ class SmolyakBarycentricInterpolator:
     def evaluate(self, x, precision: str = None) -> torch.Tensor:
         """Evaluate on a chosen path (see the constructor's `precision`)."""
         precision = precision or self._precision
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {PRECISIONS}")
         x = self._validate_input(x)
         P = x.shape[0]
-        y = torch.empty((P, self._d_out), dtype=torch.float64, device=self._device)
+        if precision != "fp64" and self._nplan is None:
+            raise ValueError(f"precision {precision!r} needs nested nodes (Leja)")
+        # the Newton streaming kernels take d_out == 1; the barycentric one small d_out
+        gemm = (self._d_out >= GEMM_MIN_DOUT or precision == "bf16"
+                or (precision != "fp64" and self._d_out > 1))
+        if not gemm:
+            y = torch.empty((P, self._d_out), dtype=torch.float64, device=self._device)
+            if precision == "fp64":
+                N.check(N.fn["sbi_bary_eval_f64"](self._plan, x.data_ptr(), P, x.stride(0),
+                                                  y.data_ptr(), self._stream()), "eval")
+            else:
+                fn = "sbi_newton_eval_mixed" if precision == "mixed" else "sbi_newton_eval_f64"
+                N.check(N.fn[fn](self._nplan, x.data_ptr(), P, x.stride(0), y.data_ptr(),
+                                 self._stream()), precision)
+            return y
         if precision == "fp64":
-            N.check(N.fn["sbi_bary_eval_f64"](self._plan, x.data_ptr(), P, x.stride(0), y.data_ptr(),
-                                              self._stream()), "eval")
-        elif precision in ("mixed", "newton-fp64"):
-            if self._nplan is None:
-                raise ValueError(f"precision {precision!r} needs nested nodes and d_out == 1")
-            fn = "sbi_newton_eval_mixed" if precision == "mixed" else "sbi_newton_eval_f64"
-            N.check(N.fn[fn](self._nplan, x.data_ptr(), P, x.stride(0), y.data_ptr(),
-                             self._stream()), precision)
-        else:
-            raise ValueError(f"precision must be one of {PRECISIONS}")
+            return self._eval_bary_gemm(x)
+        return self._eval_newton_gemm(x, precision)
+
+    def _point_chunk(self, rows, itemsize, P):
+        """Points per chunk so that the basis matrix stays within memory_limit GiB."""
+        per_point = max(1, rows) * itemsize
+        return max(64, min(P, int(self._memory_limit * 1024**3 // per_point) // 64 * 64 or 64))
+
+    def _eval_bary_gemm(self, x):
+        """y = offset + W F: barycentric entry weights (HIP kernel) then one FP64 GEMM."""
+        P = x.shape[0]
+        K = self._layout.k_raw
+        F = self._dev["F"][:K]
+        y = torch.empty((P, self._d_out), dtype=torch.float64, device=self._device)
+        pc = self._point_chunk(K, 8, P)
+        for a in range(0, P, pc):
+            b = min(P, a + pc)
+            xc = x[a:b]
+            WT = torch.empty((K, b - a), dtype=torch.float64, device=self._device)
+            N.check(N.fn["sbi_bary_basis_f64"](self._plan, xc.data_ptr(), b - a, xc.stride(0),
+                                               WT.data_ptr(), b - a, self._stream()), "bary_basis")
+            torch.mm(WT.t(), F, out=y[a:b])
+        y += self._dev["offset"]
         return y
+
+    def _surpluses(self, kind):
+        S = self._S.get(kind)
+        if S is None:
+            dt = {"f32": torch.float32, "bf16": torch.bfloat16}[kind]
+            S = self._S["f64"].to(dt)
+            self._S[kind] = S
+        return S
+
+    def _eval_newton_gemm(self, x, precision):
+        """y = H S: Newton basis matrix (HIP kernel) then one dense GEMM on MFMA."""
+        P = x.shape[0]
+        Nn = self._nlayout.basis_n
+        kind, code, dt = {"newton-fp64": ("f64", 0, torch.float64), "mixed": ("f32", 1, torch.float32),
+                          "bf16": ("bf16", 2, torch.bfloat16)}[precision]
+        S = self._surpluses(kind)
+        y = torch.empty((P, self._d_out), dtype=torch.float64, device=self._device)
+        pc = self._point_chunk(Nn, torch.finfo(dt).bits // 8, P)
+        for a in range(0, P, pc):
+            b = min(P, a + pc)
+            xc = x[a:b]
+            HT = torch.empty((Nn, b - a), dtype=dt, device=self._device)
+            N.check(N.fn["sbi_newton_basis"](self._nplan, xc.data_ptr(), b - a, xc.stride(0),
+                                             HT.data_ptr(), b - a, code, self._stream()), "basis")
+            if kind == "bf16":
+                y[a:b] = torch.mm(HT.t(), S, out_dtype=torch.float32)
+            elif kind == "f32":
+                y[a:b] = torch.mm(HT.t(), S)
+            else:
+                torch.mm(HT.t(), S, out=y[a:b])
+        return y
+
+    def vjp(self, x, v, precision: str = None) -> torch.Tensor:
+        """Vector-Jacobian product g[p, j] = sum_m v[p, m] dy_m/dx_j (FP64 result).
+
+        Nested nodes: hand-written adjoint of the Newton form after one GEMM G^T = S V^T
+        ("mixed" runs the GEMM and the adjoint tables in FP32).  Non-nested nodes: contraction of
+        the barycentric Jacobian (interpolation.py:306-345).
+        """
+        precision = precision or self._precision
+        x = self._validate_input(x)
+        P = x.shape[0]
+        v = torch.as_tensor(v, device=self._device).reshape(P, self._d_out)
+        if self._nplan is None:
+            J = self.gradient(x)
+            return torch.einsum("pm,pmj->pj", v.to(torch.float64), J)
+        kind, code = ("f32", 1) if precision in ("mixed", "bf16") else ("f64", 0)
+        S = self._surpluses(kind)
+        GT = torch.mm(S, v.to(S.dtype).t()).contiguous()
+        gx = torch.empty((P, self._d_in), dtype=torch.float64, device=self._device)
+        N.check(N.fn["sbi_newton_vjp"](self._nplan, x.data_ptr(), P, x.stride(0), GT.data_ptr(),
+                                       GT.stride(0), code, gx.data_ptr(), self._stream()), "vjp")
+        return gx
+
+    def differentiable(self, x: torch.Tensor, precision: str = None) -> torch.Tensor:
+        """__call__ as a torch.autograd.Function: backward runs the hand-written adjoint."""
+        from .autograd import SmolyakEval
+
+        return SmolyakEval.apply(x, self, precision or self._precision)
 
     def gradient(self, x) -> torch.Tensor:
         """interpolation.py:306-345: Jacobian J[P, d_out, d_in] in float64."""

@@ -158,6 +158,7 @@ def build_newton_layout(nf: NewtonForm, n_waves: int = N_WAVES) -> NewtonLayout:
     L.n_nodes = len(nodes)
     L.node_q = np.array([row[v[:-1]] for v in nodes] or [0], dtype=np.int32)
     s = np.array([nf.s[idx[v]] for v in nodes]).reshape(len(nodes), -1) if nodes else np.zeros((1, 1))
+    # streaming kernels (d_out == 1) read column 0; the GEMM paths use nf.s directly
     L.node_s64 = np.ascontiguousarray(s[:, 0], dtype=np.float64)
     L.node_s32 = L.node_s64.astype(np.float32)
     # groups by last factor (j, k), in node order
@@ -248,7 +249,47 @@ def build_newton_layout(nf: NewtonForm, n_waves: int = N_WAVES) -> NewtonLayout:
             for q in range(L.n_internal)]
     assert L.n_internal + 1 < 65536 and r < 65536
     L.int_rec = np.array(irec or [0], dtype=np.int32)
+    build_basis_tables(nf, L)
     return L
+
+
+def build_basis_tables(nf: NewtonForm, L: NewtonLayout) -> None:
+    """Descriptors of the basis matrix H^T (rows in nf.nus order) and of the adjoint."""
+    tab_dims = [j for j in range(nf.d_in) if nf.maxdeg[j] > 0]
+    row0, r = {}, 0
+    for j in tab_dims:
+        row0[j] = r
+        r += int(nf.maxdeg[j])
+    assert r < 65535, "factor table too large"
+    L.n_tab_dims, L.n_tab_rows = len(tab_dims), r
+    L.tab_dim = np.array(tab_dims or [0], dtype=np.int32)
+    L.tab_row = np.array([row0[j] for j in tab_dims] or [0], dtype=np.int32)
+    L.tab_kmax = np.array([int(nf.maxdeg[j]) for j in tab_dims] or [0], dtype=np.int32)
+    L.basis_n = len(nf.nus)
+    desc = np.full((max(L.basis_n, 1), 8), 0xFFFF, dtype=np.uint16)
+    per_row = [[] for _ in range(max(r, 1))]
+    for i, nu in enumerate(nf.nus):
+        if len(nu) > 8:
+            raise ValueError("basis descriptors hold at most 8 factors per node")
+        rows = [row0[j] + k - 1 for j, k in nu]
+        desc[i, : len(rows)] = rows
+        for a, ra in enumerate(rows):
+            others = rows[:a] + rows[a + 1:]
+            if len(others) > 6:
+                raise ValueError("adjoint entries hold at most 6 other factors")
+            per_row[ra].append((i, others))
+    L.basis_desc = desc.reshape(-1).view(np.uint8)
+    off = [0]
+    ent = []
+    for ra in range(r):
+        for node, others in per_row[ra]:
+            e = np.full(8, 0xFFFF, dtype=np.uint16)
+            e[0:2] = np.array([node], dtype=np.int32).view(np.uint16)
+            e[2: 2 + len(others)] = others
+            ent.append(e)
+        off.append(len(ent))
+    L.vjp_slot_off = np.array(off, dtype=np.int32)
+    L.vjp_ent = (np.stack(ent) if ent else np.full((1, 8), 0xFFFF, dtype=np.uint16)).reshape(-1).view(np.uint8)
 
 
 def eval_newton_layout(L: NewtonLayout, x, fp32=False):

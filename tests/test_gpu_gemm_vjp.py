@@ -1,0 +1,111 @@
+"""GPU: GEMM-shaped evaluation (large d_out) and the hand-written adjoint.
+
+Tolerances (max |diff| / max |ref|): FP64 paths 1e-11 (the oracle's own rounding is amplified by
+the Smolyak coefficients); mixed (FP32 GEMM) 1e-5; bf16 (BF16 operands, FP32 accumulation) 5e-2;
+adjoint FP64 vs the oracle's Jacobian contracted with v: 1e-10; central finite differences with
+h = 1e-6 in FP64: 1e-6 relative.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import indices as OI
+from oracle import nodes as ON
+from oracle import smolyak as OS
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def make(d, d_out, m, seed=0, kind="leja"):
+    from smolyax_amd import nodes as SN
+    from smolyax_amd.interpolation import SmolyakBarycentricInterpolator
+
+    rng = np.random.default_rng(seed)
+    k = np.log((2 + np.arange(d)) / np.log(2))
+    ong, sng = (ON.leja(dim=d), SN.Leja(dim=d)) if kind == "leja" else (ON.gauss_hermite(dim=d), SN.GaussHermite(dim=d))
+    t = OI.find_approximate_threshold(k, m, ong.is_nested)
+    A = rng.standard_normal((d_out, d)) / (np.arange(d) + 1.0) ** 1.5
+    b = rng.uniform(0, 2 * np.pi, d_out)
+    f = lambda x: np.sin(np.asarray(x) @ A.T + b)
+    ref = OS.Interpolant(d_out=d_out, node_gen=ong, k=k, t=t, f=f)
+    ip = SmolyakBarycentricInterpolator(d_out=d_out, node_gen=sng, k=k, t=t)
+    ip.set_f(f=f, vectorized=True)
+    return ref, ip, rng
+
+
+@pytest.mark.parametrize("d,d_out,m", [(6, 32, 200), (20, 64, 800), (50, 40, 1500)])
+def test_gemm_paths(gpu, d, d_out, m):
+    ref, ip, rng = make(d, d_out, m, seed=d)
+    x = rng.uniform(-1, 1, (200, d))
+    yr = ref(x)
+    assert rel(ip.evaluate(x, "fp64").cpu().numpy(), yr) < 1e-11
+    assert rel(ip.evaluate(x, "newton-fp64").cpu().numpy(), yr) < 1e-11
+    assert rel(ip.evaluate(x, "mixed").cpu().numpy(), yr) < 1e-5
+    assert rel(ip.evaluate(x, "bf16").cpu().numpy(), yr) < 5e-2
+
+
+def test_gemm_point_chunks(gpu):
+    """Chunking over points (memory_limit) gives the same result."""
+    ref, ip, rng = make(8, 24, 300, seed=3)
+    x = rng.uniform(-1, 1, (500, 8))
+    y_all = ip.evaluate(x, "newton-fp64").cpu().numpy()
+    ip._memory_limit = 1e-5  # forces chunks of 64 points
+    y_ch = ip.evaluate(x, "newton-fp64").cpu().numpy()
+    y_b = ip.evaluate(x, "fp64").cpu().numpy()
+    assert np.array_equal(y_all, y_ch)
+    assert rel(y_b, ref(x)) < 1e-11
+
+
+@pytest.mark.parametrize("d,d_out,m", [(3, 2, 40), (12, 20, 600), (40, 16, 1500)])
+def test_vjp_matches_oracle_jacobian(gpu, d, d_out, m):
+    ref, ip, rng = make(d, d_out, m, seed=10 + d)
+    x = rng.uniform(-0.95, 0.95, (64, d))
+    v = rng.standard_normal((64, d_out))
+    g_ref = np.einsum("pm,pmj->pj", v, ref.gradient(x))
+    g64 = ip.vjp(x, v, "fp64").cpu().numpy()
+    g32 = ip.vjp(x, v, "mixed").cpu().numpy()
+    assert rel(g64, g_ref) < 1e-10
+    assert rel(g32, g_ref) < 1e-5
+
+
+def test_vjp_finite_differences(gpu):
+    """Adjoint of w.y against central differences with h = 1e-6 in FP64."""
+    d, d_out = 10, 8
+    ref, ip, rng = make(d, d_out, 500, seed=4)
+    x = rng.uniform(-0.9, 0.9, (16, d))
+    w = rng.standard_normal(d_out)
+    g = ip.vjp(x, np.tile(w, (16, 1)), "fp64").cpu().numpy()
+    h = 1e-6
+    fd = np.zeros_like(g)
+    for j in range(d):
+        e = np.zeros(d)
+        e[j] = h
+        yp = ip.evaluate(x + e, "newton-fp64").cpu().numpy() @ w
+        ym = ip.evaluate(x - e, "newton-fp64").cpu().numpy() @ w
+        fd[:, j] = (yp - ym) / (2 * h)
+    assert rel(g, fd) < 1e-6
+
+
+def test_autograd_function(gpu):
+    d, d_out = 8, 4
+    ref, ip, rng = make(d, d_out, 300, seed=5)
+    x = torch.tensor(rng.uniform(-0.9, 0.9, (32, d)), device="cuda", requires_grad=True)
+    y = ip.differentiable(x, "newton-fp64")
+    w = torch.tensor(rng.standard_normal((32, d_out)), device="cuda")
+    (y * w).sum().backward()
+    g_ref = np.einsum("pm,pmj->pj", w.cpu().numpy(), ref.gradient(x.detach().cpu().numpy()))
+    assert rel(x.grad.cpu().numpy(), g_ref) < 1e-10
+
+
+def test_vjp_non_nested_uses_jacobian(gpu):
+    ref, ip, rng = make(4, 3, 60, seed=6, kind="gh")
+    x = rng.standard_normal((20, 4)) / np.sqrt(2)
+    v = rng.standard_normal((20, 3))
+    g_ref = np.einsum("pm,pmj->pj", v, ref.gradient(x))
+    assert rel(ip.vjp(x, v).cpu().numpy(), g_ref) < 1e-10
