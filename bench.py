@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-points", type=int, default=1024)
     ap.add_argument("--oracle-points", type=int, default=256)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-requests", type=int, default=600)
     return ap.parse_args()
 
 
@@ -58,6 +60,134 @@ def timed_loop(fn, steps, warmup, torch):
     return (t1 - t0), float(np.mean(ev_ms)), float(np.median(ev_ms))
 
 
+def bench_gemm_config(args, torch, W, cfg):
+    """Large-d_out workloads: every path is a basis kernel + one dense GEMM (MFMA)."""
+    from oracle import nodes as ON
+    from oracle import smolyak as OS
+    from smolyax_amd import indices as SI
+    from smolyax_amd import nodes as SN
+    from smolyax_amd.interpolation import SmolyakBarycentricInterpolator
+
+    d, d_out, m, P = cfg["d_in"], cfg["d_out"], cfg["m"], cfg["P"]
+    f = cfg["f"]()
+    k = W.k_rule(d)
+    ts = time.perf_counter()
+    t = SI.find_approximate_threshold(k, m, True)
+    ip = SmolyakBarycentricInterpolator(d_out=d_out, node_gen=SN.Leja(dim=d), k=k, t=t,
+                                        memory_limit=16.0)
+    ip.set_f(f=lambda X: f(X), vectorized=True)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - ts
+    x_np = W.points_uniform(P, d, seed=0)
+    x = torch.as_tensor(x_np, device="cuda")
+    results, outputs = {}, {}
+    for name in ("bf16", "mixed", "newton-fp64", "fp64"):
+        steps = args.steps if name != "fp64" else max(2, args.steps // 10)
+        box = {}
+
+        def step():
+            box["y"] = ip.evaluate(x, precision=name)
+
+        wall, ev_mean_ms, ev_med_ms = timed_loop(step, steps, min(args.warmup, 2), torch)
+        outputs[name] = box["y"].cpu().numpy()
+        results[name] = {"value": P * steps / wall, "ms_per_step": wall / steps * 1e3,
+                         "steps": steps}
+    accuracy = {}
+
+    def errs(a, b):
+        diff = a - b
+        return {"max_rel": float(np.max(np.abs(diff)) / np.max(np.abs(b))),
+                "rel_rmse": float(np.sqrt(np.mean(diff ** 2) / np.mean(b ** 2)))}
+
+    for name in results:
+        accuracy[name] = {"vs_gpu_fp64_all_points": errs(outputs[name], outputs["fp64"])}
+    out_extra = {}
+    if cfg.get("vjp"):
+        rng = np.random.default_rng(2)
+        v = torch.as_tensor(rng.standard_normal((P, d_out)), device="cuda")
+        for name in ("mixed", "fp64"):
+            box = {}
+
+            def vstep():
+                box["g"] = ip.vjp(x, v, precision=name)
+
+            wall, _, _ = timed_loop(vstep, args.steps, min(args.warmup, 2), torch)
+            results["vjp-" + name] = {"value": P * args.steps / wall,
+                                      "ms_per_step": wall / args.steps * 1e3}
+            outputs["vjp-" + name] = box["g"].cpu().numpy()
+        accuracy["vjp-mixed"] = {"vs_gpu_fp64_all_points": errs(outputs["vjp-mixed"],
+                                                                outputs["vjp-fp64"])}
+        if cfg.get("oracle_points"):
+            no = cfg["oracle_points"]
+            ref = OS.Interpolant(d_out=d_out, node_gen=ON.leja(dim=d), k=k, t=t, f=f)
+            yr = ref(x_np[:no])
+            gr = np.einsum("pm,pmj->pj", v.cpu().numpy()[:no], ref.gradient(x_np[:no]))
+            for name in ("bf16", "mixed", "newton-fp64", "fp64"):
+                accuracy[name]["vs_oracle"] = errs(outputs[name][:no], yr)
+            accuracy["vjp-fp64"] = {"vs_oracle": errs(outputs["vjp-fp64"][:no], gr)}
+            accuracy["vjp-mixed"]["vs_oracle"] = errs(outputs["vjp-mixed"][:no], gr)
+            accuracy["oracle_points"] = no
+    L, NL = ip.layout, ip._nlayout
+    flops_newton = 2.0 * d_out * NL.basis_n * P
+    flops_bary = 2.0 * d_out * L.k_raw * P
+    t_mixed = results["mixed"]["ms_per_step"] * 1e-3
+    roofline = {"bound": "mfma", "achieved": flops_newton / t_mixed / 1e12, "peak": 157.3,
+                "unit": "TFLOP/s", "frac": flops_newton / t_mixed / 1e12 / 157.3, "traffic": None,
+                "kernel": "newton basis + FP32 GEMM (whole step)"}
+    extra = {name: {"tflops_alg": (flops_bary if name == "fp64" else flops_newton)
+                    / (r["ms_per_step"] * 1e-3) / 1e12}
+             for name, r in results.items() if not name.startswith("vjp")}
+    out = {
+        "metric": "point-evals/s, d_in=1e3 n=1e4 (mixed vs FP64, <=1e-6 rel); serve p50/p99 ms",
+        "value": results["mixed"]["value"], "unit": "point-evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": results["mixed"]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (mixed path)", "data": "synthetic",
+        "config": {"workload": args.config, "d_in": d, "d_out": d_out, "points": P,
+                   "nodes": int(ip.n_f_evals), "k_raw": int(L.k_raw), "t": t},
+        "roofline": roofline, "paths": results, "path_tflops": extra, "accuracy": accuracy,
+        "cpu_baseline": None, "setup_s": setup_s, **out_extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def _log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def bench_serving(torch, W, n_requests=600, qps=None, seed=0):
+    """Serving workload: 32 resident Gauss-Hermite interpolants, Poisson arrivals, FP32."""
+    from smolyax_amd import serving
+
+    ts = time.perf_counter()
+    ips = W.build_serving_interpolants(W.serving_specs(32, seed=seed), progress=_log)
+    names = sorted(ips)
+    for n in names:  # first touch of every interpolant outside the timed run
+        ips[n].evaluate(torch.zeros((4, ips[n].d_in), dtype=torch.float64, device="cuda"))
+        torch.cuda.synchronize()
+        _log(f"serve: first evaluation of {n} done")
+    setup_s = time.perf_counter() - ts
+    # capacity: all requests submitted at once (closed loop)
+    srv = serving.Server(ips, precision="mixed", n_streams=1)
+    reqs, inputs = serving.run_poisson(srv, names, qps=1e9, n_requests=n_requests, seed=seed,
+                                       time_limit_s=120.0, progress=_log)
+    cap = srv.stats(reqs)
+    _log(f"serve: capacity run done, {cap['requests'] / cap['span_s']:.0f} requests/s")
+    load_qps = qps if qps is not None else 0.7 * cap["requests"] / cap["span_s"]
+    srv = serving.Server(ips, precision="mixed", n_streams=1)
+    reqs, _ = serving.run_poisson(srv, names, qps=load_qps, n_requests=n_requests, seed=seed,
+                                  inputs=inputs, time_limit_s=120.0, progress=_log)
+    st = srv.stats(reqs)
+    _log(f"serve: loaded run done, p50 {st['p50_ms']:.3f} ms p99 {st['p99_ms']:.3f} ms")
+    return {"qps": load_qps, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
+            "mean_ms": st["mean_ms"], "point_evals_per_s": st["point_evals_per_s"],
+            "requests": st["requests"], "capacity_point_evals_per_s": cap["point_evals_per_s"],
+            "capacity_qps": cap["requests"] / cap["span_s"], "precision": "mixed (FP32)",
+            "interpolants": len(names), "setup_s": setup_s,
+            "workload": "32 GH interpolants, d_in logU[8,500], nodes logU[1e2,2e4], d_out "
+                        "logU[1,1024], batch logU[1,4096], Poisson arrivals at 70% of capacity"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -76,7 +206,15 @@ def main():
     from smolyax_amd.interpolation import SmolyakBarycentricInterpolator
     from smolyax_amd import indices as SI
 
+    if args.config == "c5_serve":
+        out = {"metric": "serve p50/p99 ms (point-evals/s)", "config": {"workload": "c5_serve"},
+               "serve": bench_serving(torch, W, n_requests=args.serve_requests), "n_gpus": 1}
+        out["value"] = out["serve"]["point_evals_per_s"]
+        print(json.dumps(out), flush=True)
+        return
     cfg = W.CONFIGS[args.config]
+    if cfg["d_out"] > 1:
+        return bench_gemm_config(args, torch, W, cfg)
     d, d_out, m, P = cfg["d_in"], cfg["d_out"], cfg["m"], cfg["P"]
     f = cfg["f"]()
     k = W.k_rule(d)
@@ -167,6 +305,7 @@ def main():
                "sample": f"{nc} of the {P} points x {reps} calls, FP64 torch-CPU restatement of the "
                          f"reference's padded-group einsum evaluation"}
 
+    serve = None if args.no_serve else bench_serving(torch, W, n_requests=args.serve_requests)
     value = results["mixed"]["value"]
     out = {
         "metric": "point-evals/s, d_in=1e3 n=1e4 (mixed vs FP64, <=1e-6 rel); serve p50/p99 ms",
@@ -191,6 +330,7 @@ def main():
         "paths": results,
         "speedup_mixed_vs_fp64": results["mixed"]["value"] / results["fp64"]["value"],
         "accuracy": accuracy,
+        "serve": serve,
         "setup_s": setup_s,
     }
     print(json.dumps(out), flush=True)

@@ -126,7 +126,9 @@ int sbi_plan_destroy(sbi_plan* plan);
 
 /* interpolation.py:264-304 __call__ with barycentric.py:69-123 per block:
  * y[P, d_out] (FP64) = offset + sum_blocks zeta * tensor-product interpolant at x[P, ldx].
- * x, y device pointers; stream = hipStream_t (NULL = default stream). */
+ * x, y device pointers; stream = hipStream_t (NULL = default stream).  Returns SBI_ERR_CAPACITY
+ * when the prefix basis table of a 64-point tile exceeds LDS (more than 320 rows); the GEMM form
+ * (sbi_bary_basis_f64 + a dense GEMM) evaluates such plans. */
 int sbi_bary_eval_f64(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx, double* y,
                       void* stream);
 
@@ -225,6 +227,13 @@ int sbi_newton_vjp(const sbi_newton_plan* plan, const double* x, int64_t P, int6
  * reference's per-block contraction (barycentric.py:69-123) as one dense GEMM (large d_out). */
 int sbi_bary_basis_f64(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx, double* WT,
                        int64_t ldw, void* stream);
+/* Same weights (FP64 node differences and products) stored in FP32 for the FP32 contraction. */
+int sbi_bary_basis_f32(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx, float* WT,
+                       int64_t ldw, void* stream);
+
+/* Descriptor accessors of the plan objects (shared by the library's translation units). */
+int sbi_bary_plan_desc(const sbi_plan* plan, const sbi_bary_desc** out);
+int sbi_newton_plan_desc(const sbi_newton_plan* plan, const sbi_newton_desc** out);
 
 #ifdef __cplusplus
 }

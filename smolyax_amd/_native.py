@@ -91,9 +91,15 @@ SIGNATURES = {
     "sbi_newton_basis": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_i32, c_vp]),
     "sbi_newton_vjp": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_i32, c_vp, c_vp]),
     "sbi_bary_basis_f64": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp]),
+    "sbi_bary_basis_f32": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp]),
+    "sbi_bary_plan_desc": (c_i32, [c_vp, c_vp]),
+    "sbi_newton_plan_desc": (c_i32, [c_vp, c_vp]),
 }
 
 fn = {name: _sig(name, r, a) for name, (r, a) in SIGNATURES.items()}
+
+
+SBI_OK, SBI_ERR_INVALID, SBI_ERR_HIP, SBI_ERR_UNSUPPORTED, SBI_ERR_CAPACITY = 0, -1, -2, -3, -4
 
 
 class NativeError(RuntimeError):

@@ -497,11 +497,8 @@ int sbi_bary_plan_create(const sbi_bary_desc* desc, sbi_plan** out) {
   const size_t table = (size_t)d.n_prefix_vals * kTile * sizeof(double);
   const size_t red = (size_t)kWaves * 4 * kTile * sizeof(double);
   const size_t lds = table > red ? table : red;
-  if (lds > (size_t)kMaxLds) {
-    sbi::set_error("sbi_bary_plan_create: prefix basis table exceeds LDS (" +
-                   std::to_string(d.n_prefix_vals) + " rows > 320)");
-    return SBI_ERR_CAPACITY;
-  }
+  // a plan whose prefix table exceeds LDS is still valid for the basis / jacobian / integral
+  // entry points; sbi_bary_eval_f64 then reports SBI_ERR_CAPACITY
   sbi_plan* p = new sbi_plan();
   p->kind = 1;
   p->bary = d;
@@ -544,6 +541,11 @@ extern "C" int sbi_bary_eval_f64(const sbi_plan* plan, const double* x, int64_t 
   if (!plan || plan->kind != 1 || !x || !y || P < 0 || ldx < plan->bary.d_in) {
     sbi::set_error("sbi_bary_eval_f64: invalid argument");
     return SBI_ERR_INVALID;
+  }
+  if (plan->eval_lds_bytes > (size_t)kMaxLds) {
+    sbi::set_error("sbi_bary_eval_f64: prefix basis table exceeds LDS (" +
+                   std::to_string(plan->bary.n_prefix_vals) + " rows > 320)");
+    return SBI_ERR_CAPACITY;
   }
   if (P == 0) return SBI_OK;
   hipStream_t st = (hipStream_t)stream;

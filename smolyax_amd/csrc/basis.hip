@@ -146,22 +146,29 @@ __global__ __launch_bounds__(kTile* kBWaves) void newton_vjp_kernel(sbi_newton_d
   extern __shared__ double lds_raw[];
   typedef typename lds_t<T>::type LT;
   LT* tab = (LT*)lds_raw;
-  LT* dtab = tab + (size_t)d.n_tab_rows * kTile;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t p_raw = (int64_t)blockIdx.x * kTile + lane;
   const bool valid = p_raw < P;
   const int64_t p = valid ? p_raw : (P - 1);
-  build_h_table<T>(d, x + p * ldx, tab, dtab, wave, lane, kBWaves);
+  const double* __restrict__ xp = x + p * ldx;
+  build_h_table<T>(d, xp, tab, nullptr, wave, lane, kBWaves);
   __syncthreads();
 
   cptr<int32_t> tdim = C(d.tab_dim), trow = C(d.tab_row), tk = C(d.tab_kmax),
                 slot_off = C(d.vjp_slot_off);
+  cptr<double> dim_xi = C(d.dim_xi), dim_c = C(d.dim_c);
   const uint4* __restrict__ ent = reinterpret_cast<const uint4*>(d.vjp_ent);
   for (int i = wave; i < d.n_tab_dims; i += kBWaves) {
     const int j = tdim[i], row = trow[i], km = tk[i];
-    double gj = 0.0;
+    const double xj = xp[j];
+    cptr<double> xi = dim_xi + (int64_t)j * d.kmax1;
+    cptr<double> cj = dim_c + (int64_t)j * d.kmax1;
+    double gj = 0.0, pr = 1.0, dpr = 0.0;  // d/dx prod_{r<k} (x - xi_r), FP64
     for (int k = 1; k <= km; ++k) {
+      const double dx = xj - xi[k - 1];
+      dpr = dpr * dx + pr;
+      pr *= dx;
       const int s = row + k - 1;
       double A = 0.0;
       const int e0 = slot_off[s], e1 = slot_off[s + 1];
@@ -184,7 +191,7 @@ __global__ __launch_bounds__(kTile* kBWaves) void newton_vjp_kernel(sbi_newton_d
           A += v;
         }
       }
-      gj = fma((double)dtab[s * kTile + lane], A, gj);
+      gj = fma(dpr * cj[k], A, gj);
     }
     if (valid) gx[p * d.d_in + j] = gj;
   }
@@ -218,9 +225,10 @@ __device__ __forceinline__ double lagrange_first_form(double xv, cptr<double> xi
   return v;
 }
 
+template <typename OUT>
 __global__ __launch_bounds__(kTile) void bary_basis_kernel(sbi_bary_desc d,
                                                           const double* __restrict__ x, int64_t P,
-                                                          int64_t ldx, double* __restrict__ WT,
+                                                          int64_t ldx, OUT* __restrict__ WT,
                                                           int64_t ldw, int chunk) {
   const int lane = threadIdx.x;
   const int64_t p_raw = (int64_t)blockIdx.x * kTile + lane;
@@ -256,15 +264,12 @@ __global__ __launch_bounds__(kTile) void bary_basis_kernel(sbi_bary_desc d,
         wgt *= lagrange_first_form(xs[i], nodes + (int64_t)s * d.dmax1, wts + (int64_t)s * d.dmax1,
                                    di1 - 1, mu);
       }
-      if (valid) WT[(r0 + e) * ldw + p] = wgt;
+      if (valid) WT[(r0 + e) * ldw + p] = (OUT)wgt;
     }
   }
 }
 
 }  // namespace
-
-extern "C" int sbi_bary_basis_f64(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx,
-                                  double* WT, int64_t ldw, void* stream);
 
 extern "C" int sbi_newton_plan_desc(const sbi_newton_plan* plan, const sbi_newton_desc** out);
 
@@ -317,9 +322,9 @@ extern "C" int sbi_newton_basis(const sbi_newton_plan* plan, const double* x, in
 template <typename T>
 static int vjp_launch(const sbi_newton_desc& d, const double* x, int64_t P, int64_t ldx,
                       const T* GT, int64_t ldg, double* gx, hipStream_t st) {
-  const size_t lds = 2 * (size_t)d.n_tab_rows * kTile * sizeof(T);
+  const size_t lds = (size_t)d.n_tab_rows * kTile * sizeof(T);
   if (lds > (size_t)kMaxLds) {
-    sbi::set_error("sbi_newton_vjp: the tile's factor tables exceed LDS");
+    sbi::set_error("sbi_newton_vjp: the tile's factor table exceeds LDS (use FP32: mixed)");
     return SBI_ERR_CAPACITY;
   }
   static bool attr = false;
@@ -361,23 +366,34 @@ extern "C" int sbi_newton_vjp(const sbi_newton_plan* plan, const double* x, int6
 
 extern "C" int sbi_bary_plan_desc(const sbi_plan* plan, const sbi_bary_desc** out);
 
-extern "C" int sbi_bary_basis_f64(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx,
-                                  double* WT, int64_t ldw, void* stream) {
+template <typename OUT>
+static int bary_basis_launch(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx, OUT* WT,
+                             int64_t ldw, void* stream) {
   const sbi_bary_desc* dp = nullptr;
   if (sbi_bary_plan_desc(plan, &dp) != SBI_OK || !x || !WT || P < 0 || ldw < P) {
-    sbi::set_error("sbi_bary_basis_f64: invalid argument");
+    sbi::set_error("sbi_bary_basis: invalid argument");
     return SBI_ERR_INVALID;
   }
   const sbi_bary_desc& d = *dp;
   if (ldx < d.d_in || d.max_n > 16) {
-    sbi::set_error("sbi_bary_basis_f64: invalid argument (at most 16 active dims per block)");
+    sbi::set_error("sbi_bary_basis: invalid argument (at most 16 active dims per block)");
     return SBI_ERR_INVALID;
   }
   if (P == 0 || d.n_blocks == 0) return SBI_OK;
   const int chunk = 64;
   dim3 grid((unsigned)((P + kTile - 1) / kTile), (unsigned)((d.n_blocks + chunk - 1) / chunk));
-  hipLaunchKernelGGL(bary_basis_kernel, grid, dim3(kTile), 0, (hipStream_t)stream, d, x, P, ldx,
-                     WT, ldw, chunk);
+  hipLaunchKernelGGL(bary_basis_kernel<OUT>, grid, dim3(kTile), 0, (hipStream_t)stream, d, x, P,
+                     ldx, WT, ldw, chunk);
   SBI_HIP_CHECK(hipGetLastError());
   return SBI_OK;
+}
+
+extern "C" int sbi_bary_basis_f64(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx,
+                                  double* WT, int64_t ldw, void* stream) {
+  return bary_basis_launch<double>(plan, x, P, ldx, WT, ldw, stream);
+}
+
+extern "C" int sbi_bary_basis_f32(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx,
+                                  float* WT, int64_t ldw, void* stream) {
+  return bary_basis_launch<float>(plan, x, P, ldx, WT, ldw, stream);
 }

@@ -16,7 +16,7 @@ from . import _native as N
 from . import indices, newton, packing
 from .nodes import Generator
 
-PRECISIONS = ("fp64", "mixed", "newton-fp64", "bf16")
+PRECISIONS = ("fp64", "mixed", "newton-fp64", "bf16", "bary-fp32")
 GEMM_MIN_DOUT = 16  # from this d_out on, the d_out contraction is one dense GEMM
 
 
@@ -54,7 +54,9 @@ class SmolyakBarycentricInterpolator:
                         FP32 GEMM on MFMA (nested nodes);
           "newton-fp64" hierarchical Newton form in FP64 (nested nodes);
           "bf16"        hierarchical Newton form, BF16 basis and surpluses with FP32 accumulation
-                        on MFMA (nested nodes).
+                        on MFMA (nested nodes);
+          "bary-fp32"   barycentric form with FP64 node differences, FP32 entry weights and FP32
+                        contraction on MFMA (any nodes).  For non-nested nodes "mixed" means this.
         """
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {PRECISIONS}")
@@ -102,7 +104,7 @@ class SmolyakBarycentricInterpolator:
             nf.set_values(vals)
             self._nform = nf
             self._nlayout = newton.build_newton_layout(nf)
-        elif self._precision != "fp64":
+        elif self._precision in ("newton-fp64", "bf16"):
             raise ValueError(f"precision {self._precision!r} needs nested nodes (Leja)")
         self._upload()
         if self._n_inputs is not None:
@@ -259,8 +261,12 @@ class SmolyakBarycentricInterpolator:
         precision = precision or self._precision
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {PRECISIONS}")
+        if precision == "mixed" and self._nplan is None:
+            precision = "bary-fp32"
         x = self._validate_input(x)
         P = x.shape[0]
+        if precision == "bary-fp32":
+            return self._eval_bary_gemm(x, fp32=True)
         if precision != "fp64" and self._nplan is None:
             raise ValueError(f"precision {precision!r} needs nested nodes (Leja)")
         # the Newton streaming kernels take d_out == 1; the barycentric one small d_out
@@ -269,8 +275,11 @@ class SmolyakBarycentricInterpolator:
         if not gemm:
             y = torch.empty((P, self._d_out), dtype=torch.float64, device=self._device)
             if precision == "fp64":
-                N.check(N.fn["sbi_bary_eval_f64"](self._plan, x.data_ptr(), P, x.stride(0),
-                                                  y.data_ptr(), self._stream()), "eval")
+                rc = N.fn["sbi_bary_eval_f64"](self._plan, x.data_ptr(), P, x.stride(0),
+                                               y.data_ptr(), self._stream())
+                if rc == N.SBI_ERR_CAPACITY:  # prefix table too large for LDS: GEMM form
+                    return self._eval_bary_gemm(x)
+                N.check(rc, "eval")
             else:
                 fn = "sbi_newton_eval_mixed" if precision == "mixed" else "sbi_newton_eval_f64"
                 N.check(N.fn[fn](self._nplan, x.data_ptr(), P, x.stride(0), y.data_ptr(),
@@ -285,20 +294,31 @@ class SmolyakBarycentricInterpolator:
         per_point = max(1, rows) * itemsize
         return max(64, min(P, int(self._memory_limit * 1024**3 // per_point) // 64 * 64 or 64))
 
-    def _eval_bary_gemm(self, x):
-        """y = offset + W F: barycentric entry weights (HIP kernel) then one FP64 GEMM."""
+    def _eval_bary_gemm(self, x, fp32=False, out=None):
+        """y = offset + W F: barycentric entry weights (HIP kernel) then one GEMM (FP64 / FP32)."""
         P = x.shape[0]
         K = self._layout.k_raw
-        F = self._dev["F"][:K]
-        y = torch.empty((P, self._d_out), dtype=torch.float64, device=self._device)
-        pc = self._point_chunk(K, 8, P)
+        if fp32:
+            F = self._dev.get("F32")
+            if F is None:
+                F = self._dev["F"][:K].to(torch.float32)
+                self._dev["F32"] = F
+            dt, fn = torch.float32, "sbi_bary_basis_f32"
+        else:
+            F, dt, fn = self._dev["F"][:K], torch.float64, "sbi_bary_basis_f64"
+        y = out if out is not None else torch.empty((P, self._d_out), dtype=torch.float64,
+                                                     device=self._device)
+        pc = self._point_chunk(K, 4 if fp32 else 8, P)
         for a in range(0, P, pc):
             b = min(P, a + pc)
             xc = x[a:b]
-            WT = torch.empty((K, b - a), dtype=torch.float64, device=self._device)
-            N.check(N.fn["sbi_bary_basis_f64"](self._plan, xc.data_ptr(), b - a, xc.stride(0),
-                                               WT.data_ptr(), b - a, self._stream()), "bary_basis")
-            torch.mm(WT.t(), F, out=y[a:b])
+            WT = torch.empty((K, b - a), dtype=dt, device=self._device)
+            N.check(N.fn[fn](self._plan, xc.data_ptr(), b - a, xc.stride(0), WT.data_ptr(), b - a,
+                             self._stream()), "bary_basis")
+            if fp32:
+                y[a:b] = torch.mm(WT.t(), F)
+            else:
+                torch.mm(WT.t(), F, out=y[a:b])
         y += self._dev["offset"]
         return y
 

@@ -1,0 +1,65 @@
+"""GPU: continuous-batching server.  Every request's result equals the same request evaluated
+alone within the FP32 path's rounding (1e-3 of max |y|: the library GEMM may reduce in another
+order for another batch size, and the barycentric FP32 form amplifies rounding by the Smolyak
+coefficients), and the FP64 result within 1e-2; admission respects the HBM budget; overlapping
+streams complete every request."""
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def small_specs():
+    from smolyax_amd import workloads as W
+
+    specs = W.serving_specs(n=6, seed=3)
+    for sp in specs:  # keep the test quick
+        sp["d_in"] = min(sp["d_in"], 40)
+        sp["nodes"] = min(sp["nodes"], 800)
+        sp["d_out"] = min(sp["d_out"], 64)
+    return specs
+
+
+def test_results_same_alone_and_in_mix(gpu):
+    from smolyax_amd import serving
+    from smolyax_amd import workloads as W
+
+    ips = W.build_serving_interpolants(small_specs())
+    names = sorted(ips)
+    srv = serving.Server(ips, precision="mixed", n_streams=1, max_batch_points=2048)
+    reqs, inputs = serving.run_poisson(srv, names, qps=20000.0, n_requests=120, batch_hi=512, seed=1)
+    assert len(reqs) == 120 and all(r.done is not None for r in reqs)
+    assert any(r.batch_points > r.x.shape[0] for r in reqs), "no request was batched with others"
+    torch.cuda.synchronize()
+    for r in reqs:
+        alone = srv.ips[r.name].evaluate(r.x, precision="mixed")
+        ref64 = srv.ips[r.name].evaluate(r.x, precision="fp64")
+        scale = float(ref64.abs().max()) + 1e-30
+        assert float((r.y - alone).abs().max()) / scale < 1e-3
+        assert float((r.y - ref64).abs().max()) / scale < 1e-2
+    st = srv.stats(reqs)
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+
+
+def test_admission_by_hbm_budget(gpu):
+    from smolyax_amd import serving
+    from smolyax_amd import workloads as W
+
+    ips = W.build_serving_interpolants(small_specs()[:2])
+    names = sorted(ips)
+    one = max(ips[n].layout.k_raw * 4 + ips[n].d_out * 8 + ips[n].d_in * 8 for n in names)
+    srv = serving.Server(ips, precision="mixed", n_streams=1, hbm_budget_bytes=300 * one)
+    with pytest.raises(ValueError):
+        srv.submit(names[0], torch.zeros((1000, ips[names[0]].d_in), dtype=torch.float64, device="cuda"))
+    xs = [torch.randn((200, ips[names[i % 2]].d_in), dtype=torch.float64, device="cuda") for i in range(8)]
+    reqs = [srv.submit(names[i % 2], xs[i]) for i in range(8)]
+    peak = 0
+    while srv.step():
+        peak = max(peak, srv.reserved)
+        assert srv.reserved <= srv.budget
+    assert peak <= 300 * one
+    for r in reqs:
+        alone = srv.ips[r.name].evaluate(r.x, precision="mixed")
+        assert float((r.y - alone).abs().max()) <= 1e-3 * (float(alone.abs().max()) + 1e-30)
