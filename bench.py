@@ -168,13 +168,13 @@ def bench_serving(torch, W, n_requests=600, qps=None, seed=0):
         _log(f"serve: first evaluation of {n} done")
     setup_s = time.perf_counter() - ts
     # capacity: all requests submitted at once (closed loop)
-    srv = serving.Server(ips, precision="mixed", n_streams=1)
+    srv = serving.Server(ips, precision="mixed", n_streams=4)
     reqs, inputs = serving.run_poisson(srv, names, qps=1e9, n_requests=n_requests, seed=seed,
                                        time_limit_s=120.0, progress=_log)
     cap = srv.stats(reqs)
     _log(f"serve: capacity run done, {cap['requests'] / cap['span_s']:.0f} requests/s")
     load_qps = qps if qps is not None else 0.7 * cap["requests"] / cap["span_s"]
-    srv = serving.Server(ips, precision="mixed", n_streams=1)
+    srv = serving.Server(ips, precision="mixed", n_streams=4)
     reqs, _ = serving.run_poisson(srv, names, qps=load_qps, n_requests=n_requests, seed=seed,
                                   inputs=inputs, time_limit_s=120.0, progress=_log)
     st = srv.stats(reqs)

@@ -231,6 +231,20 @@ int sbi_bary_basis_f64(const sbi_plan* plan, const double* x, int64_t P, int64_t
 int sbi_bary_basis_f32(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx, float* WT,
                        int64_t ldw, void* stream);
 
+/* Coefficient-by-basis contraction on the matrix cores for many outputs:
+ *   Y[p, j] = offset[j] + sum_k A[k, p] B[k, j]   (p < P, j < n, k < K; offset may be NULL)
+ * with A the basis matrix written by sbi_newton_basis / sbi_bary_basis_* (K x P, row stride lda)
+ * and B the coefficients (K x n, row stride ldb), both of dtype 0 (FP64, FP64 MFMA) or 1 (FP32,
+ * FP32 MFMA with the partial sums added into FP64 every 256 terms); Y is FP64 (row stride ldy).
+ * Replaces the dense GEMM that follows the per-point weights in the reference's evaluation
+ * (barycentric.py:69-123 contracts F with the weight tensors block by block).  Each row of Y
+ * depends only on its own column of A, summed in a fixed order, so a point's result is the same
+ * bit for bit whatever else is in the batch; launches are independent and may run concurrently on
+ * several streams. */
+int sbi_contract(const void* A, int64_t lda, const void* B, int64_t ldb, const double* offset,
+                 double* Y, int64_t ldy, int64_t P, int64_t n, int64_t K, int32_t dtype,
+                 void* stream);
+
 /* Descriptor accessors of the plan objects (shared by the library's translation units). */
 int sbi_bary_plan_desc(const sbi_plan* plan, const sbi_bary_desc** out);
 int sbi_newton_plan_desc(const sbi_newton_plan* plan, const sbi_newton_desc** out);

@@ -16,7 +16,7 @@ OUT = os.path.join(HERE, "libsmolyax_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["bary.hip", "newton.hip", "basis.hip"]
+HIP_SOURCES = ["bary.hip", "newton.hip", "basis.hip", "gemm.hip"]
 HOST_SOURCES = ["indices.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function"]

@@ -37,15 +37,6 @@ void set_error(const std::string& msg) { g_err = msg; }
 
 }  // namespace sbi
 
-#define SBI_HIP_CHECK(expr)                                                              \
-  do {                                                                                   \
-    hipError_t _e = (expr);                                                              \
-    if (_e != hipSuccess) {                                                              \
-      sbi::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));                 \
-      return SBI_ERR_HIP;                                                                \
-    }                                                                                    \
-  } while (0)
-
 namespace {
 
 constexpr int kTile = 64;       // points per tile (one per lane)

@@ -27,15 +27,6 @@
 #include "../../include/smolyax_amd.h"
 #include "common.h"
 
-#define SBI_HIP_CHECK(expr)                                                              \
-  do {                                                                                   \
-    hipError_t _e = (expr);                                                              \
-    if (_e != hipSuccess) {                                                              \
-      sbi::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));                 \
-      return SBI_ERR_HIP;                                                                \
-    }                                                                                    \
-  } while (0)
-
 namespace {
 
 constexpr int kTile = 64;

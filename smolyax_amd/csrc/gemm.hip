@@ -1,0 +1,228 @@
+// Coefficient-by-basis contraction on the matrix cores (gfx950 MFMA), for many outputs.
+//
+//   Y[p, j] = offset[j] + sum_k A[k, p] * B[k, j]        p < P, j < n, k < K
+//
+// A is the basis matrix written by the basis kernels (K x P, points contiguous: the Newton basis
+// H^T or the barycentric entry weights W^T); B is the coefficient matrix (K x n, outputs
+// contiguous: Newton surpluses or zeta-scaled node values).  Y is FP64, row stride ldy.
+//
+// Work split: a workgroup of 4 waves owns a 64-point x (32*FN)-output tile of Y; each wave a
+// 32 x 16*FN sub-tile made of 2 x FN MFMA 16x16 fragments.  Every workgroup is independent (no
+// inter-workgroup waits, no atomics), so any number of these launches may run concurrently on
+// several streams, and each Y row is a function of its own column of A only, summed in a fixed
+// order: a point's result does not depend on P, on its position in the batch or on the batch's
+// other points (bit for bit).
+//
+// k order: MFMA 16x16x4 takes k = lane/16 from lane group g; within a 16-wide chunk of k, step s
+// of the four MFMAs takes k = k0 + 4g + s, so each lane walks four consecutive rows of A and B.
+// A and B are read with one dword (FP32) or qword (FP64) per lane; the 16 lanes of a group read
+// 16 consecutive points (outputs) of one row: 64- (128-) byte segments.  Edges are handled by
+// clamped loads (rows beyond P or n are computed and discarded) and, in the last chunk, by zeroing
+// terms with k >= K.
+//
+// Accumulation: FP64 operands use the FP64 MFMA (v_mfma_f64_16x16x4).  FP32 operands use the FP32
+// MFMA (v_mfma_f32_16x16x4) into FP32 partial sums that are added into FP64 accumulators every
+// kFlushK terms of k, so the FP32 accumulation error stays that of a 256-term sum and the result's
+// error is dominated by the FP32 rounding of the operands.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/smolyax_amd.h"
+#include "common.h"
+
+namespace {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int kWaves = 4;
+constexpr int kBlockM = 64;   // points per workgroup
+constexpr int kChunk = 16;    // k per main-loop iteration
+constexpr int kFlushK = 256;  // FP32 partial sums are flushed into FP64 every kFlushK terms
+
+template <typename T>
+struct Acc;
+template <>
+struct Acc<float> {
+  using type = v4f;
+};
+template <>
+struct Acc<double> {
+  using type = v4d;
+};
+
+__device__ __forceinline__ v4f mfma(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <typename T, int FN>
+__global__ __launch_bounds__(64 * kWaves) void contract_kernel(
+    const T* __restrict__ A, int64_t lda, const T* __restrict__ B, int64_t ldb,
+    const double* __restrict__ offset, double* __restrict__ Y, int64_t ldy, int64_t P, int64_t n,
+    int64_t K) {
+  using AccT = typename Acc<T>::type;
+  constexpr bool kF32 = sizeof(T) == 4;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int li = lane & 15, g = lane >> 4;
+
+  const int64_t m_base = (int64_t)blockIdx.x * kBlockM + wm * 32;
+  const int64_t n_base = (int64_t)blockIdx.y * (32 * FN) + wn * (16 * FN);
+  int64_t pcol[2], jcol[FN];
+#pragma unroll
+  for (int fm = 0; fm < 2; ++fm) pcol[fm] = min(m_base + fm * 16 + li, P - 1);
+#pragma unroll
+  for (int fn = 0; fn < FN; ++fn) jcol[fn] = min(n_base + fn * 16 + li, n - 1);
+
+  AccT acc[2][FN];
+  double acc64[2][FN][4];
+#pragma unroll
+  for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+    for (int fn = 0; fn < FN; ++fn) {
+      acc[fm][fn] = AccT{0, 0, 0, 0};
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc64[fm][fn][r] = 0.0;
+    }
+
+  const int64_t K16 = K & ~(int64_t)(kChunk - 1);
+  T a[2][4], b[FN][4];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int64_t k = k0 + 4 * g + s;
+      const T* __restrict__ ar = A + k * lda;
+      const T* __restrict__ br = B + k * ldb;
+#pragma unroll
+      for (int fm = 0; fm < 2; ++fm) a[fm][s] = ar[pcol[fm]];
+#pragma unroll
+      for (int fn = 0; fn < FN; ++fn) b[fn][s] = br[jcol[fn]];
+    }
+  };
+  auto step = [&]() {
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn) acc[fm][fn] = mfma(a[fm][s], b[fn][s], acc[fm][fn]);
+  };
+  auto flush = [&]() {
+#pragma unroll
+    for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+      for (int fn = 0; fn < FN; ++fn) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc64[fm][fn][r] += (double)acc[fm][fn][r];
+        acc[fm][fn] = AccT{0, 0, 0, 0};
+      }
+  };
+
+  if (K16 > 0) {
+    load(0);
+    for (int64_t k0 = 0; k0 < K16; k0 += kChunk) {
+      T ca[2][4], cb[FN][4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+#pragma unroll
+        for (int fm = 0; fm < 2; ++fm) ca[fm][s] = a[fm][s];
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn) cb[fn][s] = b[fn][s];
+      }
+      if (k0 + kChunk < K16) load(k0 + kChunk);  // next chunk in flight during this one
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+          for (int fn = 0; fn < FN; ++fn)
+            acc[fm][fn] = mfma(ca[fm][s], cb[fn][s], acc[fm][fn]);
+      if (kF32 && ((k0 + kChunk) % kFlushK) == 0) flush();
+    }
+  }
+  if (K16 < K) {  // last partial chunk: rows k >= K contribute zero
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int64_t k = K16 + 4 * g + s;
+      const bool in = k < K;
+      const int64_t kc = in ? k : K - 1;
+#pragma unroll
+      for (int fm = 0; fm < 2; ++fm) {
+        const T v = A[kc * lda + pcol[fm]];
+        a[fm][s] = in ? v : T(0);
+      }
+#pragma unroll
+      for (int fn = 0; fn < FN; ++fn) {
+        const T v = B[kc * ldb + jcol[fn]];
+        b[fn][s] = in ? v : T(0);
+      }
+    }
+    step();
+  }
+  if (kF32) flush();
+
+#pragma unroll
+  for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+    for (int fn = 0; fn < FN; ++fn) {
+      const int64_t col = n_base + fn * 16 + li;
+      if (col >= n) continue;
+      const double off = offset ? offset[col] : 0.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        // result layout: FP32 16x16x4 holds row 4g + r in register r of lane group g, FP64
+        // 16x16x4 holds row g + 4r (column = lane % 16 in both)
+        const int64_t row = m_base + fm * 16 + (kF32 ? 4 * g + r : g + 4 * r);
+        const double v = kF32 ? acc64[fm][fn][r] : (double)acc[fm][fn][r];
+        if (row < P) Y[row * ldy + col] = off + v;
+      }
+    }
+}
+
+template <typename T>
+int contract_launch(const T* A, int64_t lda, const T* B, int64_t ldb, const double* offset,
+                    double* Y, int64_t ldy, int64_t P, int64_t n, int64_t K, hipStream_t st) {
+  const dim3 block(64 * kWaves);
+  const unsigned gm = (unsigned)((P + kBlockM - 1) / kBlockM);
+  if (n <= 32) {
+    hipLaunchKernelGGL((contract_kernel<T, 1>), dim3(gm, (unsigned)((n + 31) / 32)), block, 0, st,
+                       A, lda, B, ldb, offset, Y, ldy, P, n, K);
+  } else if (n <= 64) {
+    hipLaunchKernelGGL((contract_kernel<T, 2>), dim3(gm, (unsigned)((n + 63) / 64)), block, 0, st,
+                       A, lda, B, ldb, offset, Y, ldy, P, n, K);
+  } else {
+    hipLaunchKernelGGL((contract_kernel<T, 4>), dim3(gm, (unsigned)((n + 127) / 128)), block, 0,
+                       st, A, lda, B, ldb, offset, Y, ldy, P, n, K);
+  }
+  SBI_HIP_CHECK(hipGetLastError());
+  return SBI_OK;
+}
+
+}  // namespace
+
+extern "C" int sbi_contract(const void* A, int64_t lda, const void* B, int64_t ldb,
+                            const double* offset, double* Y, int64_t ldy, int64_t P, int64_t n,
+                            int64_t K, int32_t dtype, void* stream) {
+  if (P < 0 || n < 0 || K < 0 || (P > 0 && n > 0 && (!Y || ldy < n)) ||
+      (P > 0 && n > 0 && K > 0 && (!A || !B || lda < P || ldb < n)) ||
+      P > (int64_t)65535 * kBlockM || n > (int64_t)65535 * 32) {
+    sbi::set_error("sbi_contract: invalid argument");
+    return SBI_ERR_INVALID;
+  }
+  if (P == 0 || n == 0) return SBI_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == 0)
+    return contract_launch<double>((const double*)A, lda, (const double*)B, ldb, offset, Y, ldy,
+                                   P, n, K, st);
+  if (dtype == 1)
+    return contract_launch<float>((const float*)A, lda, (const float*)B, ldb, offset, Y, ldy, P,
+                                  n, K, st);
+  sbi::set_error("sbi_contract: dtype must be 0 (f64) or 1 (f32)");
+  return SBI_ERR_INVALID;
+}

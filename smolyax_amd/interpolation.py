@@ -294,8 +294,18 @@ class SmolyakBarycentricInterpolator:
         per_point = max(1, rows) * itemsize
         return max(64, min(P, int(self._memory_limit * 1024**3 // per_point) // 64 * 64 or 64))
 
+    def _contract(self, A, B, offset, y):
+        """y[P, n] = offset + A^T B on the matrix cores (sbi_contract: FP64 or FP32 MFMA with
+        FP64 accumulation; rows independent of the batch)."""
+        code = {torch.float64: 0, torch.float32: 1}[A.dtype]
+        N.check(N.fn["sbi_contract"](A.data_ptr(), A.stride(0), B.data_ptr(), B.stride(0),
+                                     None if offset is None else offset.data_ptr(), y.data_ptr(),
+                                     y.stride(0), y.shape[0], y.shape[1], A.shape[0], code,
+                                     self._stream()), "contract")
+
     def _eval_bary_gemm(self, x, fp32=False, out=None):
-        """y = offset + W F: barycentric entry weights (HIP kernel) then one GEMM (FP64 / FP32)."""
+        """y = offset + W F: barycentric entry weights (HIP kernel), then the contraction with the
+        zeta-scaled values on the matrix cores (FP64, or FP32 operands with FP64 accumulation)."""
         P = x.shape[0]
         K = self._layout.k_raw
         if fp32:
@@ -315,11 +325,7 @@ class SmolyakBarycentricInterpolator:
             WT = torch.empty((K, b - a), dtype=dt, device=self._device)
             N.check(N.fn[fn](self._plan, xc.data_ptr(), b - a, xc.stride(0), WT.data_ptr(), b - a,
                              self._stream()), "bary_basis")
-            if fp32:
-                y[a:b] = torch.mm(WT.t(), F)
-            else:
-                torch.mm(WT.t(), F, out=y[a:b])
-        y += self._dev["offset"]
+            self._contract(WT, F, self._dev["offset"], y[a:b])
         return y
 
     def _surpluses(self, kind):
@@ -331,7 +337,9 @@ class SmolyakBarycentricInterpolator:
         return S
 
     def _eval_newton_gemm(self, x, precision):
-        """y = H S: Newton basis matrix (HIP kernel) then one dense GEMM on MFMA."""
+        """y = H S: Newton basis matrix (HIP kernel), then the contraction with the surpluses on
+        the matrix cores: sbi_contract for FP64 and FP32 (FP64 accumulation); BF16 operands use
+        the library GEMM (FP32 accumulation)."""
         P = x.shape[0]
         Nn = self._nlayout.basis_n
         kind, code, dt = {"newton-fp64": ("f64", 0, torch.float64), "mixed": ("f32", 1, torch.float32),
@@ -347,10 +355,8 @@ class SmolyakBarycentricInterpolator:
                                              HT.data_ptr(), b - a, code, self._stream()), "basis")
             if kind == "bf16":
                 y[a:b] = torch.mm(HT.t(), S, out_dtype=torch.float32)
-            elif kind == "f32":
-                y[a:b] = torch.mm(HT.t(), S)
             else:
-                torch.mm(HT.t(), S, out=y[a:b])
+                self._contract(HT, S, None, y[a:b])
         return y
 
     def vjp(self, x, v, precision: str = None) -> torch.Tensor:

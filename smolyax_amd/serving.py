@@ -5,9 +5,13 @@ on one host thread keeps up to `n_streams` batches in flight, one per HIP stream
 overlaps on the device.  Whenever a stream's batch completes (polled through HIP events), new
 work is admitted in arrival order: the oldest pending request picks the interpolant, and every
 pending request for the same interpolant joins its batch up to `max_batch_points` points and the
-free HBM budget (the batch's basis matrix, output and inputs are counted before launch).  Each
-request's rows are independent, so a request's result does not depend on what it was batched with
-beyond the rounding of the FP32 contraction (checked by tests/test_gpu_serving.py).
+free HBM budget (the batch's basis matrix, output and inputs are counted before launch).  The
+basis kernels and the matrix-core contraction (sbi_contract) compute every point independently of
+the batch it is in, so a request's result is the same bit for bit alone and in the mix, and their
+workgroups never wait on each other, so batches on different streams may overlap freely.  The BF16
+precision runs on the library GEMM instead, whose kernels may rely on all workgroups of a launch
+being resident (stream-K): two such launches overlapping on different streams can deadlock, so
+that precision is served on one stream.
 
 Latency = completion (event observed) - arrival; throughput = point evaluations per second.
 """
@@ -41,8 +45,10 @@ class _Slot:
 
 
 class Server:
-    def __init__(self, interpolants, precision="mixed", n_streams=1, hbm_budget_bytes=8 << 30,
+    def __init__(self, interpolants, precision="mixed", n_streams=4, hbm_budget_bytes=8 << 30,
                  max_batch_points=8192):
+        if precision == "bf16" and n_streams > 1:
+            raise ValueError("precision 'bf16' uses the library GEMM: serve it on one stream")
         self.ips = dict(interpolants)
         self.precision = precision
         self.max_batch_points = int(max_batch_points)

@@ -6,6 +6,7 @@ max |y|), the precision target of the mixed path.
 
 import numpy as np
 import pytest
+import torch
 
 from oracle import indices as OI
 from oracle import nodes as ON
@@ -48,9 +49,16 @@ def test_newton_paths(gpu, d, m, dom):
 
 
 def test_newton_rejects_non_nested(gpu):
+    """The Newton-form precisions need nested nodes; "mixed" on non-nested nodes is the FP32
+    barycentric contraction."""
     from smolyax_amd import nodes as SN
     from smolyax_amd.interpolation import SmolyakBarycentricInterpolator
 
-    with pytest.raises(ValueError):
-        SmolyakBarycentricInterpolator(d_out=1, node_gen=SN.GaussHermite(dim=2), k=[1.0, 1.5], t=3.0,
-                                       f=lambda x: x[0], precision="mixed")
+    for prec in ("newton-fp64", "bf16"):
+        with pytest.raises(ValueError):
+            SmolyakBarycentricInterpolator(d_out=1, node_gen=SN.GaussHermite(dim=2), k=[1.0, 1.5],
+                                           t=3.0, f=lambda x: x[0], precision=prec)
+    ip = SmolyakBarycentricInterpolator(d_out=1, node_gen=SN.GaussHermite(dim=2), k=[1.0, 1.5],
+                                        t=3.0, f=lambda x: x[0], precision="mixed")
+    x = torch.randn((50, 2), dtype=torch.float64, device="cuda")
+    assert float((ip(x)[:, 0] - x[:, 0]).abs().max()) < 1e-5
