@@ -40,6 +40,25 @@ def parse():
     return ap.parse_args()
 
 
+TRAFFIC_FILES = {("newton_mixed_staged_kernel", "c2_exp_d1000"):
+                 "profiles/newton_mixed_staged_c2_d1000_traffic.md"}
+
+
+def measured_traffic(kernel, config):
+    """HBM bytes per launch of `kernel` on `config` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-calibrated), or None when no summary covers it."""
+    rel = TRAFFIC_FILES.get((kernel, config))
+    path = os.path.join(ROOT, rel) if rel else None
+    if not path or not os.path.exists(path):
+        return None
+    key = f"traffic_bytes_per_launch {kernel}:"
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith(key):
+                return float(line.split(":")[1])
+    return None
+
+
 def timed_loop(fn, steps, warmup, torch):
     """Warm up, then time `steps` calls; also per-call HIP event durations on the current stream."""
     for _ in range(warmup):
@@ -272,15 +291,18 @@ def main():
                           "vs_gpu_fp64_all_points": errs(outputs[name], outputs["fp64"])}
     accuracy["oracle_points"] = int(no)
 
-    # rooflines.  Mixed (headline) kernel: HBM bytes per launch = x (FP64) + surpluses (FP32)
-    # + parent rows (int32) + y; the FP64 barycentric kernel: 2 * d_out * K_raw flops per point.
+    # rooflines on the algorithmic work of the reference's evaluation: bytes per launch =
+    # x (8 B per point and dimension) + F read once (4 B per entry on the FP32 path, d_out * K_raw
+    # entries) + y (8 B per point and output); flops = 2 * d_out * K_raw per point.
     NL = ip._nlayout
     t_mixed = results["mixed"]["kernel_ms"] * 1e-3
-    bytes_mixed = 8.0 * P * d + (4.0 + 4.0) * NL.n_nodes + 8.0 * P * d_out
+    bytes_mixed = 8.0 * P * d + 4.0 * d_out * L.k_raw + 8.0 * P * d_out
+    staged = N.fn["sbi_newton_mixed_staged"](ip._nplan) == 1
+    kname = "newton_mixed_staged_kernel" if staged else "newton_eval_kernel<float>"
     roofline = {"bound": "hbm", "achieved": bytes_mixed / t_mixed / 1e9, "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": bytes_mixed / t_mixed / 1e9 / PEAK_HBM_GBS, "traffic": None,
-                "kernel": "newton_eval_kernel<float>", "bytes_per_launch": bytes_mixed,
-                "avg_launch_ms": results["mixed"]["kernel_ms"]}
+                "unit": "GB/s", "frac": bytes_mixed / t_mixed / 1e9 / PEAK_HBM_GBS,
+                "traffic": measured_traffic(kname, args.config), "kernel": kname,
+                "bytes_per_launch": bytes_mixed, "avg_launch_ms": results["mixed"]["kernel_ms"]}
     t_b = results["fp64"]["kernel_ms"] * 1e-3
     flops = 2.0 * d_out * L.k_raw * P
     roof_fp64 = {"bound": "valu", "kernel": "bary_eval_kernel<1>", "achieved": flops / t_b / 1e12,

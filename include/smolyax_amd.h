@@ -210,6 +210,9 @@ int sbi_newton_plan_destroy(sbi_newton_plan* plan);
  * x - xi, Newton factors and accumulation across groups. */
 int sbi_newton_eval_mixed(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
                           double* y, void* stream);
+/* 1 if sbi_newton_eval_mixed runs the LDS-staged kernel for this plan (node tables fit in LDS),
+ * 0 if it runs the streaming kernel that reads them from global memory. */
+int sbi_newton_mixed_staged(const sbi_newton_plan* plan);
 /* Same in FP64 throughout. */
 int sbi_newton_eval_f64(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
                         double* y, void* stream);

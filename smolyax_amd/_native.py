@@ -92,6 +92,7 @@ SIGNATURES = {
     "sbi_newton_vjp": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_i32, c_vp, c_vp]),
     "sbi_bary_basis_f64": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp]),
     "sbi_bary_basis_f32": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp]),
+    "sbi_newton_mixed_staged": (c_i32, [c_vp]),
     "sbi_contract": (c_i32, [c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64,
                              c_i32, c_vp]),
     "sbi_bary_plan_desc": (c_i32, [c_vp, c_vp]),

@@ -465,6 +465,11 @@ extern "C" int sbi_newton_plan_create(const sbi_newton_desc* desc, sbi_newton_pl
   return SBI_OK;
 }
 
+extern "C" int sbi_newton_mixed_staged(const sbi_newton_plan* plan) {
+  if (!plan) return SBI_ERR_INVALID;
+  return plan->lds_staged <= (size_t)kMaxLds ? 1 : 0;
+}
+
 extern "C" int sbi_newton_plan_desc(const sbi_newton_plan* plan, const sbi_newton_desc** out) {
   if (!plan || !out) return SBI_ERR_INVALID;
   *out = &plan->d;
