@@ -106,13 +106,15 @@ SBI_OK, SBI_ERR_INVALID, SBI_ERR_HIP, SBI_ERR_UNSUPPORTED, SBI_ERR_CAPACITY = 0,
 
 
 class NativeError(RuntimeError):
-    pass
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 def check(rc, what=""):
     if rc != 0:
         msg = fn["sbi_last_error"]().decode(errors="replace")
-        raise NativeError(f"{what} failed (code {rc}): {msg}")
+        raise NativeError(f"{what} failed (code {rc}): {msg}", rc)
 
 
 def f64_ptr(a):

@@ -16,7 +16,7 @@
 //   * blocks are sorted by their last slot, so each run of blocks shares one streamed basis that
 //     is computed once per run into registers; all block metadata and F values are wave-uniform
 //     and arrive through scalar loads, so 64 points share every F read;
-//   * the 8 waves of a workgroup split the runs (cost-balanced on the host) and reduce their
+//   * the 16 waves of a workgroup split the runs (cost-balanced on the host) and reduce their
 //     partial sums through LDS in a fixed order (deterministic, no atomics).
 
 #include <hip/hip_runtime.h>

@@ -228,9 +228,11 @@ __global__ __launch_bounds__(kTile* kWaves) void newton_eval_kernel(sbi_newton_d
 // windows, which are requested one window ahead.  LDS layout:
 //   [W table: (n_internal+1) x 64 floats][region R]
 //   region R, phase 1: h table of the internal nodes' factors (n_hs_rows x 64 floats)
-//   region R, phase 2: surpluses (n_pad floats) | parent rows (n_pad u16) | group records
-// Group records (32 bytes): first node, end node, dim, deg, a = xi_{deg-1}, c = c_deg; node
-// ranges are padded to multiples of 4 so each batch of 4 is one aligned broadcast read.
+//   region R, phase 2: node pairs (8 bytes per node: c_deg * surplus as FP32 bits, byte offset of
+//                      the parent's W row) | group records (16 bytes: first node | end node << 16,
+//                      dim, a = xi_{deg-1} as two words)
+// Every group starts and ends on an even node (padding nodes have surplus 0), so two nodes are
+// one aligned 16-byte broadcast read.
 // ------------------------------------------------------------------------------------------
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef double v2d __attribute__((ext_vector_type(2)));

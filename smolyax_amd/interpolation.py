@@ -74,6 +74,7 @@ class SmolyakBarycentricInterpolator:
         self._device = torch.device(device) if device is not None else None
         self._plan = None
         self._nplan = None
+        self._newton_off = "set_f has not been called"
         self._layout = None
         if f is not None:
             self.set_f(f=f)
@@ -97,15 +98,21 @@ class SmolyakBarycentricInterpolator:
         self._offset_host = offset
         self._nlayout = None
         self._nform = None
+        self._newton_off = None  # why the Newton form is unavailable (None: available)
         if self._is_nested:
-            nf = newton.NewtonForm(self._k, self._t, self._node_gen, self._d_out)
-            vals = np.array([np.broadcast_to(np.asarray(f_evals[nu], dtype=np.float64).reshape(-1),
-                                             (self._d_out,)) for nu in nf.nus])
-            nf.set_values(vals)
-            self._nform = nf
-            self._nlayout = newton.build_newton_layout(nf)
-        elif self._precision in ("newton-fp64", "bf16"):
-            raise ValueError(f"precision {self._precision!r} needs nested nodes (Leja)")
+            try:
+                nf = newton.NewtonForm(self._k, self._t, self._node_gen, self._d_out)
+                vals = np.array([np.broadcast_to(np.asarray(f_evals[nu], dtype=np.float64)
+                                                 .reshape(-1), (self._d_out,)) for nu in nf.nus])
+                nf.set_values(vals)
+                nl = newton.build_newton_layout(nf)
+                self._nform, self._nlayout = nf, nl
+            except newton.CapacityError as e:
+                self._newton_off = str(e)
+        else:
+            self._newton_off = "the Newton (hierarchical) form needs nested nodes (Leja)"
+        if self._newton_off and self._precision in ("newton-fp64", "bf16"):
+            raise ValueError(f"precision {self._precision!r}: {self._newton_off}")
         self._upload()
         if self._n_inputs is not None:
             x0 = torch.rand((self._n_inputs, self._d_in), dtype=torch.float64, device=self._device)
@@ -218,7 +225,12 @@ class SmolyakBarycentricInterpolator:
             N.fn["sbi_newton_plan_destroy"](self._nplan)
             self._nplan = None
         plan = ctypes.c_void_p()
-        N.check(N.fn["sbi_newton_plan_create"](ctypes.byref(d), ctypes.byref(plan)), "newton_plan")
+        rc = N.fn["sbi_newton_plan_create"](ctypes.byref(d), ctypes.byref(plan))
+        if rc == N.SBI_ERR_CAPACITY:  # too many internal nodes for the on-chip table
+            self._newton_off = N.fn["sbi_last_error"]().decode(errors="replace")
+            self._nform = self._nlayout = None
+            return
+        N.check(rc, "newton_plan")
         self._nplan = plan
 
     def __del__(self):
@@ -265,10 +277,12 @@ class SmolyakBarycentricInterpolator:
             precision = "bary-fp32"
         x = self._validate_input(x)
         P = x.shape[0]
+        if P == 0:  # empty batch: nothing to launch (an empty tensor may have no data pointer)
+            return torch.empty((0, self._d_out), dtype=torch.float64, device=self._device)
         if precision == "bary-fp32":
             return self._eval_bary_gemm(x, fp32=True)
         if precision != "fp64" and self._nplan is None:
-            raise ValueError(f"precision {precision!r} needs nested nodes (Leja)")
+            raise ValueError(f"precision {precision!r}: {self._newton_off}")
         # the Newton streaming kernels take d_out == 1; the barycentric one small d_out
         gemm = (self._d_out >= GEMM_MIN_DOUT or precision == "bf16"
                 or (precision != "fp64" and self._d_out > 1))
@@ -370,6 +384,8 @@ class SmolyakBarycentricInterpolator:
         x = self._validate_input(x)
         P = x.shape[0]
         v = torch.as_tensor(v, device=self._device).reshape(P, self._d_out)
+        if P == 0:
+            return torch.empty((0, self._d_in), dtype=torch.float64, device=self._device)
         if self._nplan is None:
             J = self.gradient(x)
             return torch.einsum("pm,pmj->pj", v.to(torch.float64), J)
@@ -392,6 +408,8 @@ class SmolyakBarycentricInterpolator:
         x = self._validate_input(x)
         P = x.shape[0]
         J = torch.empty((P, self._d_out, self._d_in), dtype=torch.float64, device=self._device)
+        if P == 0:
+            return J
         N.check(N.fn["sbi_bary_jacobian_f64"](self._plan, x.data_ptr(), P, x.stride(0), J.data_ptr(),
                                               self._stream()), "jacobian")
         return J

@@ -81,6 +81,10 @@ def hierarchize(nus, values, node_gen):
     return s
 
 
+class CapacityError(ValueError):
+    """The multi-index set exceeds a size limit of the Newton-form layout."""
+
+
 class NewtonForm:
     """Host description of the Newton form: multi-indices, surpluses and per-dim node tables."""
 
@@ -223,7 +227,8 @@ def build_newton_layout(nf: NewtonForm, n_waves: int = N_WAVES) -> NewtonLayout:
             q_pad.append(0)
         rec.append((n0 | (len(s_pad) << 16), j, L.dim_xi[j, k - 1]))
     L.n_nodes_pad = len(s_pad)
-    assert L.n_nodes_pad < 65536, "too many nodes for the staged kernel records"
+    if L.n_nodes_pad >= 65536:
+        raise CapacityError("too many nodes for the staged kernel records (at most 65535)")
     sq = np.zeros((max(L.n_nodes_pad, 2), 2), dtype=np.int32)
     sq[: L.n_nodes_pad, 0] = np.array(s_pad, dtype=np.float32).view(np.int32)
     sq[: L.n_nodes_pad, 1] = np.array(q_pad, dtype=np.int32)
@@ -260,7 +265,8 @@ def build_basis_tables(nf: NewtonForm, L: NewtonLayout) -> None:
     for j in tab_dims:
         row0[j] = r
         r += int(nf.maxdeg[j])
-    assert r < 65535, "factor table too large"
+    if r >= 65535:
+        raise CapacityError("factor table too large (at most 65534 rows)")
     L.n_tab_dims, L.n_tab_rows = len(tab_dims), r
     L.tab_dim = np.array(tab_dims or [0], dtype=np.int32)
     L.tab_row = np.array([row0[j] for j in tab_dims] or [0], dtype=np.int32)
@@ -270,13 +276,13 @@ def build_basis_tables(nf: NewtonForm, L: NewtonLayout) -> None:
     per_row = [[] for _ in range(max(r, 1))]
     for i, nu in enumerate(nf.nus):
         if len(nu) > 8:
-            raise ValueError("basis descriptors hold at most 8 factors per node")
+            raise CapacityError("basis descriptors hold at most 8 factors per node")
         rows = [row0[j] + k - 1 for j, k in nu]
         desc[i, : len(rows)] = rows
         for a, ra in enumerate(rows):
             others = rows[:a] + rows[a + 1:]
             if len(others) > 6:
-                raise ValueError("adjoint entries hold at most 6 other factors")
+                raise CapacityError("adjoint entries hold at most 6 other factors")
             per_row[ra].append((i, others))
     L.basis_desc = desc.reshape(-1).view(np.uint8)
     off = [0]

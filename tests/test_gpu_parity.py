@@ -171,3 +171,30 @@ def test_f_evals_reuse(gpu):
         assert len(calls) == 0 and ip2.n_f_evals_new == 0
         x = np.random.default_rng(0).uniform(-0.5, 0.5, (9, d))
         assert rel(ip(x).cpu().numpy(), ip2(x).cpu().numpy()) == 0.0
+
+
+def test_input_shapes_and_errors(gpu):
+    """interpolation.py:253-262: (d_in,) is one point; empty batches give empty results; a wrong
+    d_in and use before set_f raise AssertionError.  Every precision path on the same inputs."""
+    from smolyax_amd import nodes as SN
+    from smolyax_amd.interpolation import SmolyakBarycentricInterpolator
+
+    d, d_out = 4, 3
+    f = lambda x: np.array([np.sum(x), np.prod(1 + x), np.exp(x[0])])
+    ref, ip, ong = build_pair(d, d_out, 80, f)
+    x1 = np.array([0.1, -0.3, 0.7, 0.2])
+    for prec in ("fp64", "mixed", "newton-fp64", "bf16", "bary-fp32"):
+        y0 = ip.evaluate(np.zeros((0, d)), prec)
+        assert tuple(y0.shape) == (0, d_out)
+        y1 = ip.evaluate(x1, prec).cpu().numpy()
+        assert y1.shape == (1, d_out)
+        assert rel(y1, ref(x1[None, :])) <= {"fp64": TOL, "newton-fp64": 1e-12}.get(prec, 1e-2)
+    assert tuple(ip.gradient(np.zeros((0, d))).shape) == (0, d_out, d)
+    assert tuple(ip.gradient(x1).shape) == (1, d_out, d)
+    with pytest.raises(AssertionError):
+        ip(np.zeros((2, d + 1)))
+    fresh = SmolyakBarycentricInterpolator(d_out=1, node_gen=SN.Leja(dim=d), k=[1.0] * d, t=2.0)
+    with pytest.raises(AssertionError):
+        fresh(x1)
+    with pytest.raises(AssertionError):
+        fresh.integral()

@@ -58,3 +58,19 @@ def test_hierarchize_polynomial_reproduction():
     nf.set_values(f(X).reshape(-1, 1))
     nz = {nu for nu, s in zip(nf.nus, nf.s[:, 0]) if abs(s) > 1e-13}
     assert nz <= {(), ((0, 1),), ((0, 1), (1, 1))}
+
+
+def test_newton_layout_limits_raise_capacity_error():
+    """A node with more than 8 factors cannot be described for the basis kernel: the host raises
+    CapacityError (set_f then keeps the barycentric paths only)."""
+    from types import SimpleNamespace
+
+    from smolyax_amd import newton
+
+    d = 10
+    nus = [(), tuple((j, 1) for j in range(9))]
+    nf = SimpleNamespace(d_in=d, maxdeg=np.array([1] * 9 + [0]), nus=nus)
+    L = SimpleNamespace()
+    with pytest.raises(newton.CapacityError):
+        newton.build_basis_tables(nf, L)
+    assert issubclass(newton.CapacityError, ValueError)
