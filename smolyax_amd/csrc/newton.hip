@@ -374,33 +374,27 @@ __global__ __launch_bounds__(kTile* kWaves) void newton_mixed_staged_kernel(
         }
         acc = fma(pr, (double)(t0 + t1), acc);
         g = gn;
-        ri = ri_n;
+        // the record read at the start of this group goes to scalar registers only now, so its
+        // LDS latency overlaps the group's work instead of stalling at the read
+        int r0 = __builtin_amdgcn_readfirstlane(ri_n.x), r1 = __builtin_amdgcn_readfirstlane(ri_n.y);
+        int r2 = __builtin_amdgcn_readfirstlane(ri_n.z), r3 = __builtin_amdgcn_readfirstlane(ri_n.w);
+        asm volatile("" : "+s"(r0), "+s"(r1), "+s"(r2), "+s"(r3));
+        ri = v4i{r0, r1, r2, r3};
       }
     }
   };
 
+  // windows advance sequentially over the wave's dim range (a wave's dims are contiguous runs of
+  // active dims; a window without groups costs one refill).  No other path loads a window, so
+  // the two slots keep their registers and no LDS read waits on an outstanding coordinate load.
   int wa = wj0, wb = wj0 + kXW;
   while (g < g_end) {
     consume(xa, wa);
     if (g >= g_end) break;
-    if (ri.y >= wb + kXW) {  // gap past window B: restart both windows at the next dim
-      wa = ri.y & ~(kXW - 1);
-      wb = wa + kXW;
-      xa = load_window(xp, wa, d_in);
-      xb = load_window(xp, wb, d_in);
-      continue;
-    }
     wa = wb + kXW;
     xa = load_window(xp, wa, d_in);  // refill A two windows ahead, in place
     consume(xb, wb);
     if (g >= g_end) break;
-    if (ri.y >= wa + kXW) {
-      wa = ri.y & ~(kXW - 1);
-      wb = wa + kXW;
-      xa = load_window(xp, wa, d_in);
-      xb = load_window(xp, wb, d_in);
-      continue;
-    }
     wb = wa + kXW;
     xb = load_window(xp, wb, d_in);  // refill B, in place
   }

@@ -132,6 +132,11 @@ class NewtonForm:
 
 
 N_WAVES = 16  # waves per point tile (kWaves in newton.hip)
+# Wave split of the streaming kernels: a group costs its nodes plus GROUP_COST node equivalents
+# (latency of its dependent on-chip reads), a dim DIM_COST; tuned on the d_in = 1000 workload
+# (balancing by nodes alone left the last wave with 140 one-node groups and twice the run time).
+GROUP_COST = 20.0
+DIM_COST = 2.0
 
 
 class NewtonLayout:
@@ -192,8 +197,11 @@ def build_newton_layout(nf: NewtonForm, n_waves: int = N_WAVES) -> NewtonLayout:
             cache[key] = v
         L.dim_xi[j], L.dim_c[j] = v
     # cost-balanced split of the groups over the waves, cut only at dim boundaries (a wave owns
-    # all groups of its dims, so the incremental Newton product starts at k = 1)
-    cost = np.array([(b - a) + 6.0 for a, b, _, _ in grp]) if grp else np.zeros(0)
+    # all groups of its dims, so the incremental Newton product starts at k = 1).  A group costs
+    # its nodes plus GROUP_COST (its chain of dependent on-chip reads), a new dim DIM_COST.
+    new_dim = [g == 0 or grp[g - 1][2] != grp[g][2] for g in range(len(grp))]
+    cost = np.array([(b - a) + GROUP_COST + (DIM_COST if nd else 0.0)
+                     for (a, b, _, _), nd in zip(grp, new_dim)]) if grp else np.zeros(0)
     dim_start = [g for g in range(len(grp)) if g == 0 or grp[g - 1][2] != grp[g][2]] + [len(grp)]
     cum = np.concatenate([[0.0], np.cumsum(cost)])
     cuts = [0]
