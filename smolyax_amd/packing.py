@@ -20,6 +20,12 @@ from . import barycentric, indices
 STREAM_LMAX = 12     # max (deg + 1) of a streamed factor (kLMax in bary.hip)
 SLOT_ROW_MIN = 12    # slot rows are padded to at least this many entries
 N_WAVES = 16         # waves per point tile (kWaves in bary.hip)
+# Wave split of the barycentric eval kernel, in F entries: a block costs its entries plus
+# BLOCK_COST, a run start SEG_COST more (the streamed basis and the run's metadata are fetched
+# anew); tuned on the d_in = 1000 workload, where balancing by entries alone left the last wave
+# with 490 runs and twice the run time.
+BLOCK_COST = 4.0
+SEG_COST = 35.0
 
 
 @dataclass
@@ -211,8 +217,11 @@ def build_layout(blocks: Blocks, node_gen, d_in: int, d_out: int) -> Layout:
     L.blk_desc = desc.astype(np.uint32).view(np.int32).reshape(-1)
 
     # cost-balanced split of the blocks over the waves of a tile; segments = runs cut at the
-    # wave boundaries (a wave recomputes the streamed basis at each segment start)
-    bcost = np.diff(L.blk_foff).astype(np.float64) + 8.0
+    # wave boundaries (a wave recomputes the streamed basis at each segment start, so a run start
+    # costs SEG_COST entries more)
+    bcost = np.diff(L.blk_foff).astype(np.float64) + BLOCK_COST
+    if nb:
+        bcost[L.run_start[:-1][L.run_start[:-1] < nb]] += SEG_COST
     cum = np.concatenate([[0.0], np.cumsum(bcost)]) if nb else np.zeros(1)
     cuts = [0]
     for w in range(1, N_WAVES):
