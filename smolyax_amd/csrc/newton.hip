@@ -280,7 +280,8 @@ __global__ __launch_bounds__(kTile* kWaves) void newton_mixed_staged_kernel(
   const int g_end = wave_grp[wave + 1];
   int wj0 = (g_begin < g_end) ? (grp_j[g_begin] & ~(kXW - 1)) : 0;
   XWin xa = load_window(xp, wj0, d_in);
-  XWin xb = load_window(xp, wj0 + kXW, d_in);
+  XWin xb = load_window(xp, min(wj0 + kXW, (g_begin < g_end) ? (grp_j[g_end - 1] & ~(kXW - 1)) : 0),
+                        d_in);
 
   // 1a. h values of the internal nodes' factors: h_k(x_j) for k = 1..kmax(j), FP64 -> FP32
   double xh[4];
@@ -387,16 +388,19 @@ __global__ __launch_bounds__(kTile* kWaves) void newton_mixed_staged_kernel(
   // windows advance sequentially over the wave's dim range (a wave's dims are contiguous runs of
   // active dims; a window without groups costs one refill).  No other path loads a window, so
   // the two slots keep their registers and no LDS read waits on an outstanding coordinate load.
+  // Refills past the wave's last window re-read that window (cache hits) instead of fetching the
+  // next wave's coordinates.
+  const int wlast = (g_end > g_begin) ? (grp_j[g_end - 1] & ~(kXW - 1)) : 0;
   int wa = wj0, wb = wj0 + kXW;
   while (g < g_end) {
     consume(xa, wa);
     if (g >= g_end) break;
     wa = wb + kXW;
-    xa = load_window(xp, wa, d_in);  // refill A two windows ahead, in place
+    xa = load_window(xp, min(wa, wlast), d_in);  // refill A two windows ahead, in place
     consume(xb, wb);
     if (g >= g_end) break;
     wb = wa + kXW;
-    xb = load_window(xp, wb, d_in);  // refill B, in place
+    xb = load_window(xp, min(wb, wlast), d_in);  // refill B, in place
   }
 
   // 4. deterministic cross-wave reduction in FP64 (W area reused)
