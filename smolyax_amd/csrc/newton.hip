@@ -72,6 +72,29 @@ __device__ __forceinline__ XWin load_window(const double* __restrict__ xp, int j
   for (int i = 0; i < kXW; ++i) w.v[i] = xp[min(j0 + i, d_in - 1)];
   return w;
 }
+// Same window with two coordinates per 16-byte load (row stride even, x 16-byte aligned,
+// d_in >= 2): half the load instructions, each of which touches one cache line per lane.  Pairs
+// past the row end re-read the last full pair (even index, in bounds); those values are unused.
+typedef double xv2d __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ XWin load_window2(const double* __restrict__ xp, int j0, int d_in) {
+  const int jmax = (d_in - 2) & ~1;
+  XWin w;
+#pragma unroll
+  for (int q = 0; q < kXW / 2; ++q) {
+    const xv2d v = *(const xv2d*)(xp + min(j0 + 2 * q, jmax));
+    w.v[2 * q] = v.x;
+    w.v[2 * q + 1] = v.y;
+  }
+  return w;
+}
+template <bool kPairs>
+__device__ __forceinline__ XWin load_win(const double* __restrict__ xp, int j0, int d_in) {
+  if constexpr (kPairs) {
+    return load_window2(xp, j0, d_in);
+  } else {
+    return load_window(xp, j0, d_in);
+  }
+}
 
 // h_k(x) = c_k prod_{r<k} (x - xi_r) in FP64
 __device__ __forceinline__ double newton_h(double x, cptr<double> xi, cptr<double> c, int k) {
@@ -256,6 +279,7 @@ __device__ __forceinline__ void stage_to_lds(const void* src, lds_void* dst, int
   }
 }
 
+template <bool kPairs>
 __global__ __launch_bounds__(kTile* kWaves) void newton_mixed_staged_kernel(
     sbi_newton_desc d, const double* __restrict__ x, int64_t P, int64_t ldx, double* __restrict__ y) {
   extern __shared__ double lds_raw[];
@@ -279,9 +303,9 @@ __global__ __launch_bounds__(kTile* kWaves) void newton_mixed_staged_kernel(
   const int g_begin = wave_grp[wave];
   const int g_end = wave_grp[wave + 1];
   int wj0 = (g_begin < g_end) ? (grp_j[g_begin] & ~(kXW - 1)) : 0;
-  XWin xa = load_window(xp, wj0, d_in);
-  XWin xb = load_window(xp, min(wj0 + kXW, (g_begin < g_end) ? (grp_j[g_end - 1] & ~(kXW - 1)) : 0),
-                        d_in);
+  const int wlast = (g_end > g_begin) ? (grp_j[g_end - 1] & ~(kXW - 1)) : 0;
+  XWin xa = load_win<kPairs>(xp, wj0, d_in);
+  XWin xb = load_win<kPairs>(xp, min(wj0 + kXW, wlast), d_in);
 
   // 1a. h values of the internal nodes' factors: h_k(x_j) for k = 1..kmax(j), FP64 -> FP32
   double xh[4];
@@ -342,9 +366,12 @@ __global__ __launch_bounds__(kTile* kWaves) void newton_mixed_staged_kernel(
   v4i ri = (g < g_end) ? GI[g] : v4i{0, -1, 0, 0};
 
   auto wld = [&](int off) { return *(const __attribute__((address_space(3))) float*)(Wb + off + lane4); };
-  auto consume = [&](const XWin& xw, const int w0) {
+  // after_first runs once the window's first dim is done: the refill of the other window is
+  // issued there, so no wait at the start of a window covers a load issued just before it
+  auto consume = [&](const XWin& xw, const int w0, auto&& after_first) {
 #pragma unroll
     for (int u = 0; u < kXW; ++u) {
+      if (u == 1) after_first();
       const int j = w0 + u;
       const double xj = xw.v[u];
       double pr = 1.0;
@@ -390,17 +417,18 @@ __global__ __launch_bounds__(kTile* kWaves) void newton_mixed_staged_kernel(
   // the two slots keep their registers and no LDS read waits on an outstanding coordinate load.
   // Refills past the wave's last window re-read that window (cache hits) instead of fetching the
   // next wave's coordinates.
-  const int wlast = (g_end > g_begin) ? (grp_j[g_end - 1] & ~(kXW - 1)) : 0;
   int wa = wj0, wb = wj0 + kXW;
+  consume(xa, wa, [] {});
   while (g < g_end) {
-    consume(xa, wa);
+    consume(xb, wb, [&] {  // A was consumed: refill it two windows ahead, in place
+      wa = wb + kXW;
+      xa = load_win<kPairs>(xp, min(wa, wlast), d_in);
+    });
     if (g >= g_end) break;
-    wa = wb + kXW;
-    xa = load_window(xp, min(wa, wlast), d_in);  // refill A two windows ahead, in place
-    consume(xb, wb);
-    if (g >= g_end) break;
-    wb = wa + kXW;
-    xb = load_window(xp, min(wb, wlast), d_in);  // refill B, in place
+    consume(xa, wa, [&] {  // refill B, in place
+      wb = wa + kXW;
+      xb = load_win<kPairs>(xp, min(wb, wlast), d_in);
+    });
   }
 
   // 4. deterministic cross-wave reduction in FP64 (W area reused)
@@ -520,13 +548,21 @@ extern "C" int sbi_newton_eval_mixed(const sbi_newton_plan* plan, const double* 
   if (P == 0) return SBI_OK;
   static bool attr = false;
   if (!attr) {
-    (void)hipFuncSetAttribute((const void*)newton_mixed_staged_kernel,
+    (void)hipFuncSetAttribute((const void*)newton_mixed_staged_kernel<true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
+    (void)hipFuncSetAttribute((const void*)newton_mixed_staged_kernel<false>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
     attr = true;
   }
   dim3 grid((unsigned)((P + kTile - 1) / kTile));
-  hipLaunchKernelGGL(newton_mixed_staged_kernel, grid, dim3(kTile * kWaves), plan->lds_staged,
-                     (hipStream_t)stream, plan->d, x, P, ldx, y);
+  // 16-byte coordinate loads need an even row stride, a 16-byte aligned x and d_in >= 2
+  const bool pairs = (ldx % 2 == 0) && ((uintptr_t)x % 16 == 0) && plan->d.d_in >= 2;
+  if (pairs)
+    hipLaunchKernelGGL(newton_mixed_staged_kernel<true>, grid, dim3(kTile * kWaves),
+                       plan->lds_staged, (hipStream_t)stream, plan->d, x, P, ldx, y);
+  else
+    hipLaunchKernelGGL(newton_mixed_staged_kernel<false>, grid, dim3(kTile * kWaves),
+                       plan->lds_staged, (hipStream_t)stream, plan->d, x, P, ldx, y);
   SBI_HIP_CHECK(hipGetLastError());
   return SBI_OK;
 }

@@ -195,7 +195,6 @@ class SmolyakBarycentricInterpolator:
             "node_s32": _dev_tensor(NL.node_s32, f32, dev),
             "node_s64": _dev_tensor(NL.node_s64, f64, dev),
             "node_q": _dev_tensor(NL.node_q, i32, dev),
-            "node_sq": _dev_tensor(NL.node_sq, i32, dev),
             "grp_rec": torch.as_tensor(NL.grp_rec.view(np.uint8).copy()).to(dev),
             "grp_j": _dev_tensor(NL.grp_j, i32, dev),
             "hs_dim": _dev_tensor(NL.hs_dim, i32, dev),
@@ -209,6 +208,8 @@ class SmolyakBarycentricInterpolator:
             "vjp_slot_off": _dev_tensor(NL.vjp_slot_off, i32, dev),
             "vjp_ent": torch.as_tensor(NL.vjp_ent.copy()).to(dev),
         }
+        if NL.node_sq is not None:  # staged kernel tables (else the unstaged kernel runs)
+            t["node_sq"] = _dev_tensor(NL.node_sq, i32, dev)
         # surpluses in the order of the basis rows (nf.nus), for the GEMM-shaped paths
         self._S = {"f64": _dev_tensor(self._nform.s, f64, dev)}
         self._ndev = t

@@ -233,14 +233,18 @@ def build_newton_layout(nf: NewtonForm, n_waves: int = N_WAVES) -> NewtonLayout:
         if len(s_pad) % 2:
             s_pad.append(0.0)
             q_pad.append(0)
-        rec.append((n0 | (len(s_pad) << 16), j, L.dim_xi[j, k - 1]))
+        rec.append(((n0 & 0xFFFF) | ((len(s_pad) & 0xFFFF) << 16), j, L.dim_xi[j, k - 1]))
     L.n_nodes_pad = len(s_pad)
-    if L.n_nodes_pad >= 65536:
-        raise CapacityError("too many nodes for the staged kernel records (at most 65535)")
-    sq = np.zeros((max(L.n_nodes_pad, 2), 2), dtype=np.int32)
-    sq[: L.n_nodes_pad, 0] = np.array(s_pad, dtype=np.float32).view(np.int32)
-    sq[: L.n_nodes_pad, 1] = np.array(q_pad, dtype=np.int32)
-    L.node_sq = sq.reshape(-1)
+    # the staged kernel needs node ranges below 65536 and c_deg * s inside FP32's range (c_deg
+    # grows like 2^deg on Leja nodes); otherwise its tables are left out and the plan runs the
+    # unstaged kernel, which keeps c_deg in the FP64 factor
+    folded = np.asarray(s_pad, dtype=np.float64)
+    L.node_sq = None
+    if L.n_nodes_pad < 65536 and np.all(np.abs(folded) < 1e37):
+        sq = np.zeros((max(L.n_nodes_pad, 2), 2), dtype=np.int32)
+        sq[: L.n_nodes_pad, 0] = folded.astype(np.float32).view(np.int32)
+        sq[: L.n_nodes_pad, 1] = np.array(q_pad, dtype=np.int32)
+        L.node_sq = sq.reshape(-1)
     recdt = np.dtype([("n", "<u4"), ("j", "<i4"), ("a", "<f8")])
     L.grp_rec = np.array(rec or [(0, 0, 0.0)], dtype=recdt)
     L.grp_j = np.array([r[1] for r in rec] or [0], dtype=np.int32)

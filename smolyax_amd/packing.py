@@ -204,13 +204,13 @@ def build_layout(blocks: Blocks, node_gen, d_in: int, d_out: int) -> Layout:
         npre = f1 - f0
         ls = L.blk_last_slot[pos]
         dl = int(L.slot_deg[ls]) if ls >= 0 else 0
-        if npre > 3:
-            packed = 7 | (dl << 3)
+        dg = [int(L.fac_deg[f0 + i]) for i in range(min(npre, 3))] + [0] * (3 - min(npre, 3))
+        r = [int(L.fac_row[f0 + i]) for i in range(min(npre, 3))] + [0] * (3 - min(npre, 3))
+        assert dl < STREAM_LMAX
+        if npre > 3 or max(dg) >= 32 or max(r) >= 65536:
+            packed = 7 | (dl << 3)  # generic record: the kernel reads fac_deg / fac_row
             r = [0, 0, 0]
         else:
-            dg = [int(L.fac_deg[f0 + i]) for i in range(npre)] + [0] * (3 - npre)
-            r = [int(L.fac_row[f0 + i]) for i in range(npre)] + [0] * (3 - npre)
-            assert max(dg) < 32 and dl < 32 and max(r) < 65536
             packed = npre | (dl << 3) | (dg[0] << 8) | (dg[1] << 13) | (dg[2] << 18)
         assert L.blk_foff[pos] < 2**31 and ls + 1 < 65536
         desc[pos] = [L.blk_foff[pos], packed, r[0] | (r[1] << 16), r[2] | ((ls + 1) << 16)]

@@ -62,3 +62,22 @@ def test_newton_rejects_non_nested(gpu):
                                         t=3.0, f=lambda x: x[0], precision="mixed")
     x = torch.randn((50, 2), dtype=torch.float64, device="cuda")
     assert float((ip(x)[:, 0] - x[:, 0]).abs().max()) < 1e-5
+
+
+@pytest.mark.parametrize("d", [1, 6, 40])
+def test_mixed_strided_points_same_bits(gpu, d):
+    """x as a view with an odd row stride (8-byte coordinate loads) gives the same bits as the
+    contiguous copy (16-byte loads where d_in >= 2)."""
+    from smolyax_amd import nodes as SN
+    from smolyax_amd.interpolation import SmolyakBarycentricInterpolator
+
+    k = np.log((2 + np.arange(d)) / np.log(2))
+    t = OI.find_approximate_threshold(k, 300, True)
+    f = lambda x: np.exp(-np.sum(x / (1 + np.arange(d))))
+    ip = SmolyakBarycentricInterpolator(d_out=1, node_gen=SN.Leja(dim=d), k=k, t=t, f=f)
+    big = torch.rand((300, d + 1 + d % 2), dtype=torch.float64, device="cuda") * 2 - 1
+    xv = big[:, :d]
+    assert xv.stride(0) % 2 == 1 or d == 1
+    y_view = ip.evaluate(xv, "mixed")
+    y_cont = ip.evaluate(xv.contiguous(), "mixed")
+    assert torch.equal(y_view, y_cont)
