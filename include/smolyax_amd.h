@@ -221,6 +221,16 @@ int sbi_newton_eval_f64(const sbi_newton_plan* plan, const double* x, int64_t P,
  * 2: BF16 rounded from FP32 values); y = H S is then one dense GEMM (large d_out). */
 int sbi_newton_basis(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx, void* HT,
                      int64_t ldh, int32_t dtype, void* stream);
+/* Rows [row0, row1) of the same basis matrix into HT rows 0 .. row1 - row0 - 1. */
+int sbi_newton_basis_rows(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
+                          int32_t row0, int32_t row1, void* HT, int64_t ldh, int32_t dtype,
+                          void* stream);
+/* n_rows rows of the basis matrix computed directly in FP64 from the coordinates (no factor
+ * table), row r = prod over q < 8 of h_{fac[16r+2q+1]}(x_{fac[16r+2q]}) (fac: device int32
+ * [16 * n_rows], dim < 0 = no factor; (dim, deg) outside the plan's tables contribute 1).  For the
+ * few heaviest rows, which the mixed contraction keeps in FP64 (sbi_contract_split). */
+int sbi_newton_rows_f64(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
+                        const int32_t* fac, int32_t n_rows, double* HT, int64_t ldh, void* stream);
 /* Adjoint of the Newton form (the VJP of __call__, gradient path of interpolation.py:306-345):
  * gx[P, d_in] = sum_nu G[p, nu] d H[p, nu] / dx  with G^T[basis_n, ldg] = S V^T given
  * (dtype 0: FP64, 1: FP32).  Deterministic (no atomics). */
@@ -247,6 +257,16 @@ int sbi_bary_basis_f32(const sbi_plan* plan, const double* x, int64_t P, int64_t
 int sbi_contract(const void* A, int64_t lda, const void* B, int64_t ldb, const double* offset,
                  double* Y, int64_t ldy, int64_t P, int64_t n, int64_t K, int32_t dtype,
                  void* stream);
+
+/* Mixed contraction: Y = offset + A64^T B64 (K64 rows, FP64 MFMA) + A32^T B32 (K32 rows, FP32
+ * MFMA, partial sums added into FP64 every 256 terms), with the same layouts as sbi_contract.
+ * The few rows that carry most of the result (the constant and low-degree terms) go in the FP64
+ * head so that their rounding does not dominate the FP32 remainder.  Same determinism and
+ * concurrency guarantees as sbi_contract. */
+int sbi_contract_split(const double* A64, int64_t lda64, const double* B64, int64_t ldb64,
+                       int64_t K64, const float* A32, int64_t lda32, const float* B32,
+                       int64_t ldb32, int64_t K32, const double* offset, double* Y, int64_t ldy,
+                       int64_t P, int64_t n, void* stream);
 
 /* Descriptor accessors of the plan objects (shared by the library's translation units). */
 int sbi_bary_plan_desc(const sbi_plan* plan, const sbi_bary_desc** out);

@@ -93,6 +93,11 @@ SIGNATURES = {
     "sbi_bary_basis_f64": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp]),
     "sbi_bary_basis_f32": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp]),
     "sbi_newton_mixed_staged": (c_i32, [c_vp]),
+    "sbi_newton_basis_rows": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_i32, c_i32, c_vp, c_i64, c_i32,
+                                      c_vp]),
+    "sbi_newton_rows_f64": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i32, c_vp, c_i64, c_vp]),
+    "sbi_contract_split": (c_i32, [c_vp, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64,
+                                   c_vp, c_vp, c_i64, c_i64, c_i64, c_vp]),
     "sbi_contract": (c_i32, [c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64,
                              c_i32, c_vp]),
     "sbi_bary_plan_desc": (c_i32, [c_vp, c_vp]),

@@ -88,7 +88,8 @@ __global__ __launch_bounds__(kTile* kBWaves) void newton_basis_kernel(sbi_newton
                                                                       const double* __restrict__ x,
                                                                       int64_t P, int64_t ldx,
                                                                       OUT* __restrict__ HT,
-                                                                      int64_t ldh, int chunk) {
+                                                                      int64_t ldh, int chunk,
+                                                                      int row0, int row1) {
   extern __shared__ double lds_raw[];
   typedef typename lds_t<T>::type LT;
   LT* tab = (LT*)lds_raw;
@@ -100,8 +101,8 @@ __global__ __launch_bounds__(kTile* kBWaves) void newton_basis_kernel(sbi_newton
   build_h_table<T>(d, x + p * ldx, tab, nullptr, wave, lane, kBWaves);
   __syncthreads();
 
-  const int n0 = blockIdx.y * chunk;
-  const int n1 = min(d.basis_n, n0 + chunk);
+  const int n0 = row0 + blockIdx.y * chunk;
+  const int n1 = min(row1, n0 + chunk);
   const uint4* __restrict__ desc = reinterpret_cast<const uint4*>(d.basis_desc);
   // each wave: 64-node groups, descriptors one per lane, broadcast with v_readlane
   for (int base = n0 + wave * 64; base < n1; base += kBWaves * 64) {
@@ -120,8 +121,39 @@ __global__ __launch_bounds__(kTile* kBWaves) void newton_basis_kernel(sbi_newton
         const uint32_t r = (w[f >> 1] >> ((f & 1) * 16)) & 0xffff;
         if (r != 0xffff) v *= tab[r * kTile + lane];
       }
-      if (valid) store_out(HT + (int64_t)(base + i) * ldh + p, (double)v);
+      if (valid) store_out(HT + (int64_t)(base + i - row0) * ldh + p, (double)v);
     }
+  }
+}
+
+// Rows of H^T computed directly in FP64 from the coordinates (no factor table): each row is a
+// product of at most 8 factors h_k(x_j), given as (dim, deg) pairs (dim < 0: none).  Used for the
+// few heaviest rows, which the mixed contraction keeps in FP64.
+__global__ __launch_bounds__(256) void newton_rows_f64_kernel(sbi_newton_desc d,
+                                                             const double* __restrict__ x,
+                                                             int64_t P, int64_t ldx,
+                                                             const int32_t* __restrict__ fac,
+                                                             int n_rows, double* __restrict__ HT,
+                                                             int64_t ldh) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= P) return;
+  const double* __restrict__ xp = x + p * ldx;
+  cptr<int32_t> f = C(fac);
+  cptr<double> dim_xi = C(d.dim_xi), dim_c = C(d.dim_c);
+  for (int r = 0; r < n_rows; ++r) {
+    double v = 1.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int j = f[16 * r + 2 * q], k = f[16 * r + 2 * q + 1];
+      if (j >= 0 && j < d.d_in && k >= 0 && k < d.kmax1) {  // (dim, deg) in the plan's tables
+        const double xj = xp[j];
+        cptr<double> xi = dim_xi + (int64_t)j * d.kmax1;
+        double h = dim_c[(int64_t)j * d.kmax1 + k];
+        for (int m = 0; m < k; ++m) h *= (xj - xi[m]);
+        v *= h;
+      }
+    }
+    HT[(int64_t)r * ldh + p] = v;
   }
 }
 
@@ -266,7 +298,7 @@ extern "C" int sbi_newton_plan_desc(const sbi_newton_plan* plan, const sbi_newto
 
 template <typename T, typename OUT>
 static int basis_launch(const sbi_newton_desc& d, const double* x, int64_t P, int64_t ldx, OUT* HT,
-                        int64_t ldh, hipStream_t st) {
+                        int64_t ldh, int row0, int row1, hipStream_t st) {
   const size_t lds = (size_t)d.n_tab_rows * kTile * sizeof(T);
   if (lds > (size_t)kMaxLds) {
     sbi::set_error("sbi_newton_basis: the tile's factor table exceeds LDS");
@@ -279,15 +311,17 @@ static int basis_launch(const sbi_newton_desc& d, const double* x, int64_t P, in
     attr = true;
   }
   const int chunk = 2048;
-  dim3 grid((unsigned)((P + kTile - 1) / kTile), (unsigned)((d.basis_n + chunk - 1) / chunk));
+  if (row1 <= row0) return SBI_OK;
+  dim3 grid((unsigned)((P + kTile - 1) / kTile), (unsigned)((row1 - row0 + chunk - 1) / chunk));
   hipLaunchKernelGGL((newton_basis_kernel<T, OUT>), grid, dim3(kTile * kBWaves), lds, st, d, x, P,
-                     ldx, HT, ldh, chunk);
+                     ldx, HT, ldh, chunk, row0, row1);
   SBI_HIP_CHECK(hipGetLastError());
   return SBI_OK;
 }
 
-extern "C" int sbi_newton_basis(const sbi_newton_plan* plan, const double* x, int64_t P,
-                                int64_t ldx, void* HT, int64_t ldh, int32_t dtype, void* stream) {
+extern "C" int sbi_newton_basis_rows(const sbi_newton_plan* plan, const double* x, int64_t P,
+                                     int64_t ldx, int32_t row0, int32_t row1, void* HT,
+                                     int64_t ldh, int32_t dtype, void* stream) {
   const sbi_newton_desc* dp = nullptr;
   if (sbi_newton_plan_desc(plan, &dp) != SBI_OK || !x || !HT || P < 0 || ldh < P) {
     sbi::set_error("sbi_newton_basis: invalid argument");
@@ -298,16 +332,48 @@ extern "C" int sbi_newton_basis(const sbi_newton_plan* plan, const double* x, in
     sbi::set_error("sbi_newton_basis: plan has no basis tables");
     return SBI_ERR_INVALID;
   }
-  if (P == 0) return SBI_OK;
+  if (row0 < 0 || row1 > d.basis_n || row0 > row1) {
+    sbi::set_error("sbi_newton_basis: row range outside [0, basis_n]");
+    return SBI_ERR_INVALID;
+  }
+  if (P == 0 || row0 == row1) return SBI_OK;
   hipStream_t st = (hipStream_t)stream;
   switch (dtype) {
-    case 0: return basis_launch<double, double>(d, x, P, ldx, (double*)HT, ldh, st);
-    case 1: return basis_launch<float, float>(d, x, P, ldx, (float*)HT, ldh, st);
-    case 2: return basis_launch<float, __hip_bfloat16>(d, x, P, ldx, (__hip_bfloat16*)HT, ldh, st);
+    case 0: return basis_launch<double, double>(d, x, P, ldx, (double*)HT, ldh, row0, row1, st);
+    case 1: return basis_launch<float, float>(d, x, P, ldx, (float*)HT, ldh, row0, row1, st);
+    case 2:
+      return basis_launch<float, __hip_bfloat16>(d, x, P, ldx, (__hip_bfloat16*)HT, ldh, row0,
+                                                 row1, st);
     default:
       sbi::set_error("sbi_newton_basis: dtype must be 0 (f64), 1 (f32) or 2 (bf16)");
       return SBI_ERR_INVALID;
   }
+}
+
+extern "C" int sbi_newton_basis(const sbi_newton_plan* plan, const double* x, int64_t P,
+                                int64_t ldx, void* HT, int64_t ldh, int32_t dtype, void* stream) {
+  const sbi_newton_desc* dp = nullptr;
+  if (sbi_newton_plan_desc(plan, &dp) != SBI_OK) {
+    sbi::set_error("sbi_newton_basis: invalid argument");
+    return SBI_ERR_INVALID;
+  }
+  return sbi_newton_basis_rows(plan, x, P, ldx, 0, dp->basis_n, HT, ldh, dtype, stream);
+}
+
+extern "C" int sbi_newton_rows_f64(const sbi_newton_plan* plan, const double* x, int64_t P,
+                                   int64_t ldx, const int32_t* fac, int32_t n_rows, double* HT,
+                                   int64_t ldh, void* stream) {
+  const sbi_newton_desc* dp = nullptr;
+  if (sbi_newton_plan_desc(plan, &dp) != SBI_OK || !x || (n_rows > 0 && (!fac || !HT)) ||
+      P < 0 || ldh < P || n_rows < 0 || ldx < dp->d_in) {
+    sbi::set_error("sbi_newton_rows_f64: invalid argument");
+    return SBI_ERR_INVALID;
+  }
+  if (P == 0 || n_rows == 0) return SBI_OK;
+  hipLaunchKernelGGL(newton_rows_f64_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, *dp, x, P, ldx, fac, n_rows, HT, ldh);
+  SBI_HIP_CHECK(hipGetLastError());
+  return SBI_OK;
 }
 
 template <typename T>

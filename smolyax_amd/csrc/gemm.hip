@@ -60,11 +60,16 @@ __device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Optional FP64 head (FP32 operands only): rows k < K64 of A64 (K64 x P) and B64 (K64 x n) are
+// contracted first with the FP64 MFMA, straight into the FP64 accumulators.  Lane li supplies the
+// A row of point 4*(li & 3) + (li >> 2) of its 16-point block, which puts the FP64 MFMA's result
+// (row g + 4r in register r of lane group g) into the FP32 layout (row 4g + r).
 template <typename T, int FN>
 __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
     const T* __restrict__ A, int64_t lda, const T* __restrict__ B, int64_t ldb,
     const double* __restrict__ offset, double* __restrict__ Y, int64_t ldy, int64_t P, int64_t n,
-    int64_t K) {
+    int64_t K, const double* __restrict__ A64, int64_t lda64, const double* __restrict__ B64,
+    int64_t ldb64, int64_t K64) {
   using AccT = typename Acc<T>::type;
   constexpr bool kF32 = sizeof(T) == 4;
   const int lane = threadIdx.x & 63;
@@ -90,6 +95,46 @@ __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc64[fm][fn][r] = 0.0;
     }
+
+  if constexpr (kF32) {
+    if (K64 > 0) {
+      int64_t ph[2];
+#pragma unroll
+      for (int fm = 0; fm < 2; ++fm)
+        ph[fm] = min(m_base + fm * 16 + 4 * (li & 3) + (li >> 2), P - 1);
+      v4d h[2][FN];
+#pragma unroll
+      for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn) h[fm][fn] = v4d{0, 0, 0, 0};
+      for (int64_t k0 = 0; k0 < K64; k0 += 4) {
+        const int64_t k = k0 + g;
+        const bool in = k < K64;
+        const int64_t kc = in ? k : K64 - 1;
+        double ha[2], hb[FN];
+#pragma unroll
+        for (int fm = 0; fm < 2; ++fm) {
+          const double v = A64[kc * lda64 + ph[fm]];
+          ha[fm] = in ? v : 0.0;
+        }
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn) {
+          const double v = B64[kc * ldb64 + jcol[fn]];
+          hb[fn] = in ? v : 0.0;
+        }
+#pragma unroll
+        for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+          for (int fn = 0; fn < FN; ++fn) h[fm][fn] = mfma(ha[fm], hb[fn], h[fm][fn]);
+      }
+#pragma unroll
+      for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc64[fm][fn][r] = h[fm][fn][r];
+    }
+  }
 
   const int64_t K16 = K & ~(int64_t)(kChunk - 1);
   T a[2][4], b[FN][4];
@@ -187,18 +232,19 @@ __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
 
 template <typename T>
 int contract_launch(const T* A, int64_t lda, const T* B, int64_t ldb, const double* offset,
-                    double* Y, int64_t ldy, int64_t P, int64_t n, int64_t K, hipStream_t st) {
+                    double* Y, int64_t ldy, int64_t P, int64_t n, int64_t K, const double* A64,
+                    int64_t lda64, const double* B64, int64_t ldb64, int64_t K64, hipStream_t st) {
   const dim3 block(64 * kWaves);
   const unsigned gm = (unsigned)((P + kBlockM - 1) / kBlockM);
   if (n <= 32) {
     hipLaunchKernelGGL((contract_kernel<T, 1>), dim3(gm, (unsigned)((n + 31) / 32)), block, 0, st,
-                       A, lda, B, ldb, offset, Y, ldy, P, n, K);
+                       A, lda, B, ldb, offset, Y, ldy, P, n, K, A64, lda64, B64, ldb64, K64);
   } else if (n <= 64) {
     hipLaunchKernelGGL((contract_kernel<T, 2>), dim3(gm, (unsigned)((n + 63) / 64)), block, 0, st,
-                       A, lda, B, ldb, offset, Y, ldy, P, n, K);
+                       A, lda, B, ldb, offset, Y, ldy, P, n, K, A64, lda64, B64, ldb64, K64);
   } else {
     hipLaunchKernelGGL((contract_kernel<T, 4>), dim3(gm, (unsigned)((n + 127) / 128)), block, 0,
-                       st, A, lda, B, ldb, offset, Y, ldy, P, n, K);
+                       st, A, lda, B, ldb, offset, Y, ldy, P, n, K, A64, lda64, B64, ldb64, K64);
   }
   SBI_HIP_CHECK(hipGetLastError());
   return SBI_OK;
@@ -219,10 +265,27 @@ extern "C" int sbi_contract(const void* A, int64_t lda, const void* B, int64_t l
   hipStream_t st = (hipStream_t)stream;
   if (dtype == 0)
     return contract_launch<double>((const double*)A, lda, (const double*)B, ldb, offset, Y, ldy,
-                                   P, n, K, st);
+                                   P, n, K, nullptr, 0, nullptr, 0, 0, st);
   if (dtype == 1)
     return contract_launch<float>((const float*)A, lda, (const float*)B, ldb, offset, Y, ldy, P,
-                                  n, K, st);
+                                  n, K, nullptr, 0, nullptr, 0, 0, st);
   sbi::set_error("sbi_contract: dtype must be 0 (f64) or 1 (f32)");
   return SBI_ERR_INVALID;
+}
+
+extern "C" int sbi_contract_split(const double* A64, int64_t lda64, const double* B64,
+                                  int64_t ldb64, int64_t K64, const float* A32, int64_t lda32,
+                                  const float* B32, int64_t ldb32, int64_t K32,
+                                  const double* offset, double* Y, int64_t ldy, int64_t P,
+                                  int64_t n, void* stream) {
+  if (P < 0 || n < 0 || K64 < 0 || K32 < 0 || (P > 0 && n > 0 && (!Y || ldy < n)) ||
+      (P > 0 && n > 0 && K64 > 0 && (!A64 || !B64 || lda64 < P || ldb64 < n)) ||
+      (P > 0 && n > 0 && K32 > 0 && (!A32 || !B32 || lda32 < P || ldb32 < n)) ||
+      P > (int64_t)65535 * kBlockM || n > (int64_t)65535 * 32) {
+    sbi::set_error("sbi_contract_split: invalid argument");
+    return SBI_ERR_INVALID;
+  }
+  if (P == 0 || n == 0) return SBI_OK;
+  return contract_launch<float>(A32, lda32, B32, ldb32, offset, Y, ldy, P, n, K32, A64, lda64,
+                                B64, ldb64, K64, (hipStream_t)stream);
 }

@@ -210,8 +210,11 @@ class SmolyakBarycentricInterpolator:
         }
         if NL.node_sq is not None:  # staged kernel tables (else the unstaged kernel runs)
             t["node_sq"] = _dev_tensor(NL.node_sq, i32, dev)
-        # surpluses in the order of the basis rows (nf.nus), for the GEMM-shaped paths
+        # surpluses in nf.nus order (adjoint), and in the basis-row order of the GEMM paths (the
+        # FP64 head rows first)
         self._S = {"f64": _dev_tensor(self._nform.s, f64, dev)}
+        self._Sg = {"f64": _dev_tensor(self._nform.s[NL.basis_order], f64, dev)}
+        self._head_fac = _dev_tensor(NL.head_fac, i32, dev)
         self._ndev = t
         d = N.NewtonDesc()
         d.d_in, d.d_out = self._d_in, self._d_out
@@ -343,12 +346,15 @@ class SmolyakBarycentricInterpolator:
             self._contract(WT, F, self._dev["offset"], y[a:b])
         return y
 
-    def _surpluses(self, kind):
-        S = self._S.get(kind)
+    def _surpluses(self, kind, gemm=False):
+        """Surpluses in dtype `kind` ("f64", "f32", "bf16"): nf.nus row order, or with gemm=True
+        the basis-row order of the GEMM paths."""
+        store = self._Sg if gemm else self._S
+        S = store.get(kind)
         if S is None:
             dt = {"f32": torch.float32, "bf16": torch.bfloat16}[kind]
-            S = self._S["f64"].to(dt)
-            self._S[kind] = S
+            S = store["f64"].to(dt)
+            store[kind] = S
         return S
 
     def _eval_newton_gemm(self, x, precision):
@@ -359,17 +365,34 @@ class SmolyakBarycentricInterpolator:
         Nn = self._nlayout.basis_n
         kind, code, dt = {"newton-fp64": ("f64", 0, torch.float64), "mixed": ("f32", 1, torch.float32),
                           "bf16": ("bf16", 2, torch.bfloat16)}[precision]
-        S = self._surpluses(kind)
+        S = self._surpluses(kind, gemm=True)
         y = torch.empty((P, self._d_out), dtype=torch.float64, device=self._device)
         pc = self._point_chunk(Nn, torch.finfo(dt).bits // 8, P)
+        nh = self._nlayout.n_head if kind == "f32" else 0
+        if nh:
+            S64 = self._surpluses("f64", gemm=True)[:nh]
+            S32 = S[nh:]
         for a in range(0, P, pc):
             b = min(P, a + pc)
             xc = x[a:b]
-            HT = torch.empty((Nn, b - a), dtype=dt, device=self._device)
-            N.check(N.fn["sbi_newton_basis"](self._nplan, xc.data_ptr(), b - a, xc.stride(0),
-                                             HT.data_ptr(), b - a, code, self._stream()), "basis")
+            HT = torch.empty((Nn - nh, b - a), dtype=dt, device=self._device)
+            if Nn > nh:
+                N.check(N.fn["sbi_newton_basis_rows"](self._nplan, xc.data_ptr(), b - a,
+                                                      xc.stride(0), nh, Nn, HT.data_ptr(), b - a,
+                                                      code, self._stream()), "basis")
             if kind == "bf16":
                 y[a:b] = torch.mm(HT.t(), S, out_dtype=torch.float32)
+            elif nh:  # FP64 head rows, FP32 remainder
+                H64 = torch.empty((nh, b - a), dtype=torch.float64, device=self._device)
+                N.check(N.fn["sbi_newton_rows_f64"](self._nplan, xc.data_ptr(), b - a,
+                                                    xc.stride(0), self._head_fac.data_ptr(), nh,
+                                                    H64.data_ptr(), b - a, self._stream()), "rows")
+                yc = y[a:b]
+                N.check(N.fn["sbi_contract_split"](
+                    H64.data_ptr(), H64.stride(0), S64.data_ptr(), S64.stride(0), nh,
+                    HT.data_ptr() if Nn > nh else None, b - a,
+                    S32.data_ptr() if Nn > nh else None, self._d_out, Nn - nh, None,
+                    yc.data_ptr(), yc.stride(0), b - a, self._d_out, self._stream()), "contract")
             else:
                 self._contract(HT, S, None, y[a:b])
         return y

@@ -267,7 +267,52 @@ def build_newton_layout(nf: NewtonForm, n_waves: int = N_WAVES) -> NewtonLayout:
     assert L.n_internal + 1 < 65536 and r < 65536
     L.int_rec = np.array(irec or [0], dtype=np.int32)
     build_basis_tables(nf, L)
+    # basis rows for the GEMM paths: the heaviest rows first (FP64 head of the mixed contraction)
+    L.basis_order, L.n_head, L.head_fac = head_order(nf, L)
+    desc = L.basis_desc.view(np.uint16).reshape(L.basis_n, 8)
+    L.basis_desc = np.ascontiguousarray(desc[L.basis_order]).reshape(-1).view(np.uint8)
     return L
+
+
+HEAD_ROWS = 64  # rows of the Newton basis that the mixed GEMM path contracts in FP64
+
+
+def head_order(nf: NewtonForm, L: NewtonLayout, n_head: int = HEAD_ROWS):
+    """Row order of the basis matrix on the GEMM paths, and the factors of the head rows.
+
+    The n_head rows with the largest max_x |h_nu(x)| * ||s_nu||_2 (max over each dim's domain, on
+    a 513-point grid) come first, the others follow in nus order.  These few rows (the constant and
+    the low-degree terms of the first dims) carry most of the result; the mixed contraction keeps
+    them in FP64 so that their FP32 rounding does not dominate its error.
+    Returns (order into nf.nus, n_head, int32 [16 * n_head] (dim, deg) pairs, dim -1 = none).
+    """
+    nus = nf.nus
+    n_head = min(n_head, len(nus))
+    grid = np.linspace(-1.0, 1.0, 513)
+    peak = {}
+
+    def hmax(j, k):
+        v = peak.get((j, k))
+        if v is None:
+            xs = np.asarray(nf.node_gen[j].scale(grid), dtype=np.float64)
+            h = np.full(grid.shape, L.dim_c[j, k])
+            for r in range(k):
+                h = h * (xs - L.dim_xi[j, r])
+            v = peak[(j, k)] = float(np.max(np.abs(h)))
+        return v
+
+    snorm = np.linalg.norm(np.asarray(nf.s, dtype=np.float64).reshape(len(nus), -1), axis=1)
+    w = np.array([np.prod([hmax(j, k) for j, k in nu]) if nu else 1.0 for nu in nus]) * snorm
+    head = np.argsort(-w, kind="stable")[:n_head]
+    is_head = np.zeros(len(nus), dtype=bool)
+    is_head[head] = True
+    order = np.concatenate([head, np.nonzero(~is_head)[0]]).astype(np.int64)
+    fac = np.zeros((max(n_head, 1), 8, 2), dtype=np.int32)
+    fac[..., 0] = -1
+    for r, i in enumerate(head):
+        for q, (j, k) in enumerate(nus[i]):
+            fac[r, q] = (j, k)
+    return order, n_head, fac.reshape(-1)
 
 
 def build_basis_tables(nf: NewtonForm, L: NewtonLayout) -> None:

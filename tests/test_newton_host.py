@@ -74,3 +74,35 @@ def test_newton_layout_limits_raise_capacity_error():
     with pytest.raises(newton.CapacityError):
         newton.build_basis_tables(nf, L)
     assert issubclass(newton.CapacityError, ValueError)
+
+
+def test_gemm_row_order_puts_heavy_rows_first():
+    """The GEMM paths' basis rows are a permutation of nf.nus with the head rows (largest
+    max |h| * ||s||) first; basis descriptors and head factor tables follow that order."""
+    from smolyax_amd import newton
+    from smolyax_amd import nodes as SN
+    from smolyax_amd import indices as SI
+
+    d = 12
+    k = np.log((2 + np.arange(d)) / np.log(2))
+    t = SI.find_approximate_threshold(k, 400, True)
+    nf = newton.NewtonForm(k, t, SN.Leja(dim=d), 3)
+    rng = np.random.default_rng(0)
+    nf.s = rng.standard_normal((len(nf.nus), 3)) / (1.0 + np.arange(len(nf.nus)))[:, None]
+    L = newton.build_newton_layout(nf)
+    order = L.basis_order
+    assert sorted(order.tolist()) == list(range(len(nf.nus)))
+    assert L.n_head == min(newton.HEAD_ROWS, len(nf.nus))
+    # descriptor row r decodes to the factors of nus[order[r]]
+    desc = L.basis_desc.view(np.uint16).reshape(L.basis_n, 8)
+    tab_of = {}
+    for i, j in enumerate(L.tab_dim[: L.n_tab_dims]):
+        for kk in range(1, int(L.tab_kmax[i]) + 1):
+            tab_of[int(L.tab_row[i]) + kk - 1] = (int(j), kk)
+    for r in range(L.basis_n):
+        got = sorted(tab_of[int(v)] for v in desc[r] if v != 0xFFFF)
+        assert got == sorted(nf.nus[order[r]])
+    fac = L.head_fac.reshape(-1, 8, 2)
+    for r in range(L.n_head):
+        got = sorted((int(j), int(kk)) for j, kk in fac[r] if j >= 0)
+        assert got == sorted(nf.nus[order[r]])
