@@ -91,3 +91,51 @@ def test_contract_rejects_bad_arguments(gpu):
     assert f(A.data_ptr(), 4, B.data_ptr(), 8, None, Y.data_ptr(), 8, 8, 8, 4, 1, st) == N.SBI_ERR_INVALID
     assert f(A.data_ptr(), 8, B.data_ptr(), 8, None, Y.data_ptr(), 8, 8, 8, 4, 5, st) == N.SBI_ERR_INVALID
     assert f(A.data_ptr(), 8, B.data_ptr(), 8, None, Y.data_ptr(), 8, 8, 8, 4, 1, st) == N.SBI_OK
+
+
+def contract_split(A64, B64, A32, B32, offset=None):
+    K64, P = A64.shape
+    K32 = A32.shape[0]
+    n = B64.shape[1]
+    Y = torch.full((P, n), float("nan"), dtype=torch.float64, device=A64.device)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rc = N.fn["sbi_contract_split"](A64.data_ptr(), A64.stride(0), B64.data_ptr(), B64.stride(0),
+                                    K64, A32.data_ptr() if K32 else None, P,
+                                    B32.data_ptr() if K32 else None, n, K32,
+                                    None if offset is None else offset.data_ptr(), Y.data_ptr(), n,
+                                    P, n, st)
+    N.check(rc, "contract_split")
+    return Y
+
+
+@pytest.mark.parametrize("P,n,K64,K32", [(1, 1, 1, 0), (70, 33, 64, 0), (130, 100, 64, 500),
+                                         (257, 129, 13, 1000), (64, 16, 64, 17)])
+def test_contract_split_matches_fp64_product(gpu, P, n, K64, K32):
+    """FP64 head rows on the FP64 MFMA (bound 1e-13 when there is no FP32 part: this also checks
+    the lane permutation that maps the FP64 result into the FP32 layout), FP32 rows as sbi_contract
+    (bound 2e-6 of sum |a||b|)."""
+    g = torch.Generator(device="cuda").manual_seed(P + 7 * n + 31 * K64 + K32)
+    A64 = torch.randn((K64, P), dtype=torch.float64, device="cuda", generator=g)
+    B64 = torch.randn((K64, n), dtype=torch.float64, device="cuda", generator=g)
+    A32 = torch.randn((max(K32, 1), P), dtype=torch.float64, device="cuda", generator=g).float()[:K32]
+    B32 = torch.randn((max(K32, 1), n), dtype=torch.float64, device="cuda", generator=g).float()[:K32]
+    off = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    Y = contract_split(A64, B64, A32.contiguous(), B32.contiguous(), off)
+    ref = off + A64.t() @ B64 + A32.double().t() @ B32.double()
+    s64 = A64.abs().t() @ B64.abs()
+    s32 = A32.double().abs().t() @ B32.double().abs()
+    assert torch.isfinite(Y).all()
+    assert ((Y - ref).abs() <= 1e-13 * s64 + 2e-6 * s32 + 1e-300).all()
+
+
+def test_contract_split_rows_independent_of_batch(gpu):
+    g = torch.Generator(device="cuda").manual_seed(3)
+    K64, K32, P, n = 64, 700, 500, 90
+    A64 = torch.randn((K64, P), dtype=torch.float64, device="cuda", generator=g)
+    B64 = torch.randn((K64, n), dtype=torch.float64, device="cuda", generator=g)
+    A32 = torch.randn((K32, P), dtype=torch.float32, device="cuda", generator=g)
+    B32 = torch.randn((K32, n), dtype=torch.float32, device="cuda", generator=g)
+    Y = contract_split(A64, B64, A32, B32)
+    for lo, hi in [(0, 1), (3, 4), (100, 211), (499, 500)]:
+        Ys = contract_split(A64[:, lo:hi].contiguous(), B64, A32[:, lo:hi].contiguous(), B32)
+        assert torch.equal(Ys, Y[lo:hi]), (lo, hi)
