@@ -186,22 +186,41 @@ def bench_serving(torch, W, n_requests=600, qps=None, seed=0):
         torch.cuda.synchronize()
         _log(f"serve: first evaluation of {n} done")
     setup_s = time.perf_counter() - ts
-    # capacity: all requests submitted at once (closed loop)
-    srv = serving.Server(ips, precision="mixed", n_streams=4)
-    reqs, inputs = serving.run_poisson(srv, names, qps=1e9, n_requests=n_requests, seed=seed,
-                                       time_limit_s=120.0, progress=_log)
-    cap = srv.stats(reqs)
-    _log(f"serve: capacity run done, {cap['requests'] / cap['span_s']:.0f} requests/s")
-    load_qps = qps if qps is not None else 0.7 * cap["requests"] / cap["span_s"]
-    srv = serving.Server(ips, precision="mixed", n_streams=4)
-    reqs, _ = serving.run_poisson(srv, names, qps=load_qps, n_requests=n_requests, seed=seed,
-                                  inputs=inputs, time_limit_s=120.0, progress=_log)
-    st = srv.stats(reqs)
-    _log(f"serve: loaded run done, p50 {st['p50_ms']:.3f} ms p99 {st['p99_ms']:.3f} ms")
+
+    def run(graphs, inputs=None, load_qps=None):
+        srv = serving.Server(ips, precision="mixed", n_streams=4, graphs=graphs)
+        if graphs is not False:
+            t0 = time.perf_counter()
+            srv.prewarm()
+            _log(f"serve: {sum(len(sl.graphs) for sl in srv.slots)} graphs captured in "
+                 f"{time.perf_counter() - t0:.1f} s")
+        # capacity: all requests submitted at once (closed loop), then Poisson arrivals at 70 %
+        reqs, inputs = serving.run_poisson(srv, names, qps=1e9, n_requests=n_requests, seed=seed,
+                                           inputs=inputs, time_limit_s=120.0, progress=_log)
+        cap = srv.stats(reqs)
+        _log(f"serve (graphs={graphs}): capacity {cap['requests'] / cap['span_s']:.0f} requests/s")
+        q = load_qps if load_qps is not None else 0.7 * cap["requests"] / cap["span_s"]
+        srv.completed = []
+        reqs, _ = serving.run_poisson(srv, names, qps=q, n_requests=n_requests, seed=seed,
+                                      inputs=inputs, time_limit_s=120.0, progress=_log)
+        st = srv.stats(reqs)
+        _log(f"serve (graphs={graphs}): p50 {st['p50_ms']:.3f} ms p99 {st['p99_ms']:.3f} ms")
+        return cap, st, q, inputs
+
+    cap_e, st_e, q_e, inputs = run(False)
+    q = qps if qps is not None else q_e
+    cap_g, st_g, _, _ = run(True, inputs=inputs, load_qps=q)
+    cap, st, load_qps, _ = run("auto", inputs=inputs, load_qps=q)
     return {"qps": load_qps, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
             "mean_ms": st["mean_ms"], "point_evals_per_s": st["point_evals_per_s"],
             "requests": st["requests"], "capacity_point_evals_per_s": cap["point_evals_per_s"],
-            "capacity_qps": cap["requests"] / cap["span_s"], "precision": "mixed (FP32)",
+            "capacity_qps": cap["requests"] / cap["span_s"],
+            "launches": "HIP graphs under a backlog, direct launches otherwise",
+            "eager": {"p50_ms": st_e["p50_ms"], "p99_ms": st_e["p99_ms"],
+                      "capacity_qps": cap_e["requests"] / cap_e["span_s"]},
+            "graphs": {"p50_ms": st_g["p50_ms"], "p99_ms": st_g["p99_ms"],
+                       "capacity_qps": cap_g["requests"] / cap_g["span_s"]},
+            "precision": "mixed (FP32)",
             "interpolants": len(names), "setup_s": setup_s,
             "workload": "32 GH interpolants, d_in logU[8,500], nodes logU[1e2,2e4], d_out "
                         "logU[1,1024], batch logU[1,4096], Poisson arrivals at 70% of capacity"}

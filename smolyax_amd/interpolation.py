@@ -51,12 +51,15 @@ class SmolyakBarycentricInterpolator:
           "fp64"        barycentric combination technique in FP64 (the reference's algorithm);
           "mixed"       hierarchical Newton form with FP32 surpluses and products and FP64 node
                         differences; d_out == 1: FP64 accumulation across groups; larger d_out:
-                        FP32 GEMM on MFMA (nested nodes);
-          "newton-fp64" hierarchical Newton form in FP64 (nested nodes);
+                        the heaviest rows in FP64, the rest FP32 on MFMA with FP64 accumulation;
+          "newton-fp64" hierarchical Newton form in FP64;
           "bf16"        hierarchical Newton form, BF16 basis and surpluses with FP32 accumulation
-                        on MFMA (nested nodes);
+                        on MFMA;
           "bary-fp32"   barycentric form with FP64 node differences, FP32 entry weights and FP32
-                        contraction on MFMA (any nodes).  For non-nested nodes "mixed" means this.
+                        contraction on MFMA.
+        The Newton form is built on the interpolant's own nodes when they are nested (Leja) and on
+        a nested proxy grid otherwise (Gauss-Hermite: the same polynomial, see _newton_proxy);
+        when the index set exceeds its layout limits "mixed" falls back to "bary-fp32".
         """
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {PRECISIONS}")
@@ -109,15 +112,53 @@ class SmolyakBarycentricInterpolator:
                 self._nform, self._nlayout = nf, nl
             except newton.CapacityError as e:
                 self._newton_off = str(e)
-        else:
-            self._newton_off = "the Newton (hierarchical) form needs nested nodes (Leja)"
+        self._upload()
+        if not self._is_nested:  # Newton form on a nested proxy grid (needs the FP64 plan)
+            try:
+                self._nform = self._newton_proxy()
+                self._nlayout = newton.build_newton_layout(self._nform)
+                self._upload_newton()
+            except newton.CapacityError as e:
+                self._nform = self._nlayout = None
+                self._newton_off = str(e)
         if self._newton_off and self._precision in ("newton-fp64", "bf16"):
             raise ValueError(f"precision {self._precision!r}: {self._newton_off}")
-        self._upload()
         if self._n_inputs is not None:
             x0 = torch.rand((self._n_inputs, self._d_in), dtype=torch.float64, device=self._device)
             self(x0)
         return f_evals
+
+    def _newton_proxy(self):
+        """Newton form of a non-nested (Gauss-Hermite) interpolant.
+
+        The Smolyak interpolant is a polynomial in P_Lambda = span{x^nu : nu in Lambda} for any
+        node family, and the sparse grid of a nested sequence is unisolvent for P_Lambda.  So the
+        interpolant is evaluated (FP64 barycentric path) at the |Lambda| grid points of Leja
+        sequences spanning each dim's outermost node, and hierarchised: the same polynomial with
+        |Lambda| terms instead of K_raw, without the combination technique's cancellation.
+        """
+        from . import nodes as _nodes
+
+        d = self._d_in
+        maxdeg = np.zeros(d, dtype=np.int64)
+        for nu in indices.indexset(self._k, self._t):
+            for j, v in nu:
+                maxdeg[j] = max(maxdeg[j], v)
+        dom = np.zeros((d, 2))
+        for j, g in enumerate(self._node_gen):
+            pts = np.asarray(g(int(maxdeg[j])), dtype=np.float64)
+            lo, hi = float(pts.min()), float(pts.max())
+            if hi - lo <= 1e-12 * max(1.0, abs(hi)):  # one node: a unit interval around it
+                lo, hi = lo - 1.0, hi + 1.0
+            dom[j] = (lo, hi)
+        nf = newton.NewtonForm(self._k, self._t, _nodes.Leja(domains=dom), self._d_out)
+        zero = np.array([g(0)[0] for g in nf.node_gen], dtype=np.float64)
+        X = np.stack([nf.grid_point(nu, zero) for nu in nf.nus])
+        keep = self._n_inputs
+        Y = self.evaluate(torch.as_tensor(X, device=self._device), precision="fp64")
+        self._n_inputs = keep
+        nf.set_values(Y.cpu().numpy())
+        return nf
 
     def _upload(self):
         if not torch.cuda.is_available():

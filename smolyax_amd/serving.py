@@ -13,6 +13,18 @@ precision runs on the library GEMM instead, whose kernels may rely on all workgr
 being resident (stream-K): two such launches overlapping on different streams can deadlock, so
 that precision is served on one stream.
 
+With `graphs="auto"` the server replays graphs only while requests are queued (at least
+`n_streams` pending): a captured launch costs the host less, so it raises throughput under a
+backlog, but on this stack a replay takes longer to complete than the same few kernels launched
+directly, so a short queue is served eagerly for latency.
+
+With `graphs=True` each (interpolant, batch bucket, stream) evaluation is captured once in a
+HIP graph (torch.cuda.CUDAGraph) on static input/output buffers: a batch is copied into the
+static input of the smallest power-of-two bucket that holds it, the graph is replayed on the
+stream, and the batch's rows of the static output are copied out.  The captured work is the basis
+kernel and the contraction (no memset nodes); rows past the batch hold earlier inputs and are
+ignored, and every point is computed independently, so results equal eager evaluation bit for bit.
+
 Latency = completion (event observed) - arrival; throughput = point evaluations per second.
 """
 
@@ -33,6 +45,7 @@ class Request:
     y: torch.Tensor = None
     done: float = None
     batch_points: int = 0
+    taken: bool = False
 
 
 @dataclass
@@ -42,24 +55,84 @@ class _Slot:
     reqs: list = field(default_factory=list)
     reserved: int = 0
     keep: list = field(default_factory=list)
+    pool: object = None                          # graph memory pool of this stream
+    graphs: dict = field(default_factory=dict)   # (name, bucket) -> (graph, x_static, y_static)
+
+
+MIN_BUCKET = 64
 
 
 class Server:
     def __init__(self, interpolants, precision="mixed", n_streams=4, hbm_budget_bytes=8 << 30,
-                 max_batch_points=8192):
+                 max_batch_points=8192, graphs=False):
         if precision == "bf16" and n_streams > 1:
             raise ValueError("precision 'bf16' uses the library GEMM: serve it on one stream")
+        if graphs not in (False, True, "auto"):
+            raise ValueError("graphs must be False, True or 'auto'")
+        self.graph_mode = graphs
+        self.graphs = graphs is not False
         self.ips = dict(interpolants)
         self.precision = precision
         self.max_batch_points = int(max_batch_points)
         self.budget = int(hbm_budget_bytes)
         self.reserved = 0
+        # arrival order (entries already batched are skipped lazily) and per-interpolant queues
         self.pending = collections.deque()
+        self.by_name = {name: collections.deque() for name in self.ips}
+        self.n_pending = 0
+        self._bpp = {name: self.footprint(name, 1) for name in self.ips}
         self.slots = [_Slot(torch.cuda.Stream()) for _ in range(n_streams)]
+        if self.graphs:
+            for sl in self.slots:
+                sl.pool = torch.cuda.graph_pool_handle()
         self.completed = []
         self._next = 0
 
     # ------------------------------------------------------------------ footprint / admission
+    @staticmethod
+    def bucket(n_points):
+        """Captured batch size for n_points: the smallest of MIN_BUCKET * 2^(i/2), rounded up to
+        a multiple of 32, that holds them (at most ~1.41 x padding, ~1.2 x on average)."""
+        i = 0
+        while True:
+            b = -(-int(MIN_BUCKET * 2.0 ** (i / 2)) // 32) * 32
+            if b >= n_points:
+                return b
+            i += 1
+
+    def _graph(self, slot, name, bucket):
+        """The captured evaluation of `name` on `bucket` points on this slot's stream."""
+        key = (name, bucket)
+        entry = slot.graphs.get(key)
+        if entry is None:
+            ip = self.ips[name]
+            xs = torch.zeros((bucket, ip.d_in), dtype=torch.float64, device="cuda")
+            with torch.cuda.stream(slot.stream):
+                ip.evaluate(xs, precision=self.precision)  # warm-up outside the capture
+            slot.stream.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, pool=slot.pool, stream=slot.stream):
+                ys = ip.evaluate(xs, precision=self.precision)
+            entry = slot.graphs[key] = (g, xs, ys)
+        return entry
+
+    def prewarm(self, buckets=None):
+        """Capture every (interpolant, bucket, stream) graph ahead of serving."""
+        if not self.graphs:
+            return
+        if buckets is None:
+            buckets, b = [], 1
+            while b <= self.max_batch_points:
+                b = self.bucket(b)
+                buckets.append(b)
+                b += 1
+        for sl in self.slots:
+            for name in self.ips:
+                for b in sorted(set(buckets)):
+                    if b <= self.bucket(self.max_batch_points):
+                        self._graph(sl, name, b)
+        torch.cuda.synchronize()
+
     def footprint(self, name, n_points):
         """Device bytes a batch needs while in flight: basis matrix + output + input copy."""
         ip = self.ips[name]
@@ -76,6 +149,8 @@ class Server:
             raise ValueError("request larger than the HBM budget of the server")
         self._next += 1
         self.pending.append(r)
+        self.by_name[name].append(r)
+        self.n_pending += 1
         return r
 
     def _poll(self):
@@ -92,34 +167,56 @@ class Server:
         return freed
 
     def _admit(self):
+        """Oldest pending request picks the interpolant; its queue (arrival order) fills the batch
+        up to max_batch_points and the free HBM budget.  O(batch) per launch."""
         launched = 0
         for s in self.slots:
-            if s.event is not None or not self.pending:
+            if s.event is not None:
                 continue
+            while self.pending and self.pending[0].taken:
+                self.pending.popleft()
+            if not self.pending:
+                break
             head = self.pending[0]
+            q = self.by_name[head.name]
+            bpp = self._bpp[head.name]
             batch, pts = [], 0
-            for r in list(self.pending):
-                if r.name != head.name:
-                    continue
-                n = r.x.shape[0]
+            while q:
+                n = q[0].x.shape[0]
                 if batch and pts + n > self.max_batch_points:
                     break
-                if self.reserved + self.footprint(head.name, pts + n) > self.budget:
+                if self.reserved + bpp * (pts + n) > self.budget:
                     break
+                r = q.popleft()
+                r.taken = True
                 batch.append(r)
                 pts += n
             if not batch:
                 break  # wait for memory to free up
-            for r in batch:
-                self.pending.remove(r)
-            need = self.footprint(head.name, pts)
+            self.n_pending -= len(batch)
+            need = bpp * pts
             self.reserved += need
             ip = self.ips[head.name]
-            with torch.cuda.stream(s.stream):
-                xs = torch.cat([r.x for r in batch]) if len(batch) > 1 else batch[0].x
-                y = ip.evaluate(xs, precision=self.precision)
-                ev = torch.cuda.Event()
-                ev.record(s.stream)
+            use_graph = self.graphs and (self.graph_mode is True or
+                                         self.n_pending >= len(self.slots))
+            if use_graph:
+                g, x_st, y_st = self._graph(s, head.name, self.bucket(pts))
+                with torch.cuda.stream(s.stream):
+                    if len(batch) > 1:  # one copy kernel for the whole batch
+                        torch.cat([r.x for r in batch], out=x_st[:pts])
+                    else:
+                        x_st[:pts].copy_(batch[0].x)
+                    g.replay()
+                    y = y_st[:pts].clone()
+                    xs = x_st
+                    ev = torch.cuda.Event()
+                    ev.record(s.stream)
+            else:
+                with torch.cuda.stream(s.stream):
+                    xs = torch.cat([r.x for r in batch]) if len(batch) > 1 else batch[0].x
+                    y = ip.evaluate(xs, precision=self.precision)
+                    ev = torch.cuda.Event()
+                    ev.record(s.stream)
             a = 0
             for r in batch:
                 n = r.x.shape[0]
@@ -134,7 +231,7 @@ class Server:
         """One scheduler iteration: observe completions, admit work.  Returns True if busy."""
         self._poll()
         self._admit()
-        return bool(self.pending) or any(s.event is not None for s in self.slots)
+        return self.n_pending > 0 or any(s.event is not None for s in self.slots)
 
     def drain(self):
         while self.step():
@@ -177,7 +274,7 @@ def run_poisson(server, names, qps, n_requests, batch_lo=1, batch_hi=4096, seed=
         if time_limit_s is not None and now - t0 > time_limit_s:
             raise RuntimeError(
                 f"serving run exceeded {time_limit_s} s: {i} of {n_requests} submitted, "
-                f"{len(server.completed)} completed, {len(server.pending)} pending, "
+                f"{len(server.completed)} completed, {server.n_pending} pending, "
                 f"{sum(s.event is not None for s in server.slots)} batches in flight")
         if progress is not None and now - last_report > 10.0:
             progress(f"serve: {i} submitted, {len(server.completed)} completed")

@@ -103,7 +103,8 @@ def test_autograd_function(gpu):
     assert rel(x.grad.cpu().numpy(), g_ref) < 1e-10
 
 
-def test_vjp_non_nested_uses_jacobian(gpu):
+def test_vjp_non_nested_matches_oracle(gpu):
+    """Gauss-Hermite nodes: the adjoint of the Newton form on the proxy grid (same polynomial)."""
     ref, ip, rng = make(4, 3, 60, seed=6, kind="gh")
     x = rng.standard_normal((20, 4)) / np.sqrt(2)
     v = rng.standard_normal((20, 3))

@@ -48,20 +48,27 @@ def test_newton_paths(gpu, d, m, dom):
     assert e_mixed < 1e-6, e_mixed
 
 
-def test_newton_rejects_non_nested(gpu):
-    """The Newton-form precisions need nested nodes; "mixed" on non-nested nodes is the FP32
-    barycentric contraction."""
+def test_newton_form_of_non_nested_interpolant(gpu):
+    """Gauss-Hermite interpolants get a Newton form on a nested proxy grid (the same polynomial):
+    its FP64 evaluation matches the barycentric FP64 path, the mixed paths are FP32-accurate."""
     from smolyax_amd import nodes as SN
     from smolyax_amd.interpolation import SmolyakBarycentricInterpolator
 
-    for prec in ("newton-fp64", "bf16"):
-        with pytest.raises(ValueError):
-            SmolyakBarycentricInterpolator(d_out=1, node_gen=SN.GaussHermite(dim=2), k=[1.0, 1.5],
-                                           t=3.0, f=lambda x: x[0], precision=prec)
-    ip = SmolyakBarycentricInterpolator(d_out=1, node_gen=SN.GaussHermite(dim=2), k=[1.0, 1.5],
-                                        t=3.0, f=lambda x: x[0], precision="mixed")
-    x = torch.randn((50, 2), dtype=torch.float64, device="cuda")
-    assert float((ip(x)[:, 0] - x[:, 0]).abs().max()) < 1e-5
+    for d, d_out, m in [(2, 1, 30), (6, 1, 300), (5, 20, 400)]:
+        k = np.log((2 + np.arange(d)) / np.log(2))
+        t = OI.find_approximate_threshold(k, m, False)
+        rng = np.random.default_rng(d)
+        a = rng.standard_normal((d_out, d)) / (1.0 + np.arange(d))
+        f = lambda x: np.sin(a @ x + 0.3)
+        ip = SmolyakBarycentricInterpolator(d_out=d_out, node_gen=SN.GaussHermite(dim=d), k=k, t=t,
+                                            f=f, precision="mixed")
+        assert ip._nplan is not None
+        x = torch.randn((300, d), dtype=torch.float64, device="cuda")
+        ref = ip.evaluate(x, "fp64")
+        scale = float(ref.abs().max())
+        assert float((ip.evaluate(x, "newton-fp64") - ref).abs().max()) <= 1e-9 * scale
+        assert float((ip.evaluate(x, "mixed") - ref).abs().max()) <= 1e-5 * scale
+        assert float((ip(x) - ref).abs().max()) <= 1e-5 * scale
 
 
 @pytest.mark.parametrize("d", [1, 6, 40])

@@ -71,3 +71,26 @@ def test_library_gemm_precision_needs_one_stream(gpu):
 
     with pytest.raises(ValueError):
         serving.Server({}, precision="bf16", n_streams=4)
+
+
+def test_graph_replays_equal_eager(gpu):
+    """Captured batches (HIP graphs, replayed many times with other work and synchronisations in
+    between) give every request the same bits as evaluating it alone without graphs."""
+    from smolyax_amd import serving
+    from smolyax_amd import workloads as W
+
+    ips = W.build_serving_interpolants(small_specs()[:4])
+    names = sorted(ips)
+    srv = serving.Server(ips, precision="mixed", n_streams=4, max_batch_points=1024, graphs=True)
+    srv.prewarm(buckets=[64, 128, 256, 512, 1024])
+    auto = serving.Server(ips, precision="mixed", n_streams=4, max_batch_points=1024,
+                          graphs="auto")
+    for rnd, server in enumerate([srv, srv, auto]):
+        reqs, _ = serving.run_poisson(server, names, qps=30000.0, n_requests=60, batch_hi=400,
+                                      seed=10 + rnd)
+        torch.cuda.synchronize()
+        assert len(reqs) == 60
+        for r in reqs:
+            alone = srv.ips[r.name].evaluate(r.x, precision="mixed")
+            assert torch.equal(r.y, alone), (rnd, r.rid, r.name, r.batch_points)
+    assert any(len(sl.graphs) > 0 for sl in srv.slots)
