@@ -146,6 +146,7 @@ def bench_gemm_config(args, torch, W, cfg):
             accuracy["vjp-fp64"] = {"vs_oracle": errs(outputs["vjp-fp64"][:no], gr)}
             accuracy["vjp-mixed"]["vs_oracle"] = errs(outputs["vjp-mixed"][:no], gr)
             accuracy["oracle_points"] = no
+    rows = bench_gradient_integral(ip, x, torch)
     L, NL = ip.layout, ip._nlayout
     flops_newton = 2.0 * d_out * NL.basis_n * P
     flops_bary = 2.0 * d_out * L.k_raw * P
@@ -165,9 +166,73 @@ def bench_gemm_config(args, torch, W, cfg):
         "config": {"workload": args.config, "d_in": d, "d_out": d_out, "points": P,
                    "nodes": int(ip.n_f_evals), "k_raw": int(L.k_raw), "t": t},
         "roofline": roofline, "paths": results, "path_tflops": extra, "accuracy": accuracy,
+        "gradient_integral": rows,
         "cpu_baseline": None, "setup_s": setup_s, **out_extra,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_gradient_integral(ip, x, torch, steps=10):
+    """gradient (dense Jacobian, FP64) and integral (quadrature sum over F) through the C ABI.
+
+    The Jacobian runs on at most 1024 of the points, fewer when J would pass 256 MB. Each
+    (point, dim j, entry of a block holding j) costs one FMA per output, so its roofline is FP64
+    FLOP/s: 2 * d_out * sum_b n_b * nent_b per point. The integral reads F
+    (K_raw * d_out * 8 bytes) once: HBM roofline."""
+    import ctypes
+
+    import numpy as np
+
+    from smolyax_amd import _native as N
+
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    L = ip.layout
+    d_in, d_out = ip.d_in, ip.d_out
+    per_point = 8 * d_out * d_in
+    Pg = max(64, min(int(x.shape[0]), 1024, (256 << 20) // per_point) // 64 * 64)
+    Pg = min(Pg, int(x.shape[0]))
+    xg = x[:Pg]
+    J = torch.empty((Pg, d_out, d_in), dtype=torch.float64, device="cuda")
+    nent = np.diff(L.blk_foff).astype(np.float64)
+    flop_pt = 2.0 * d_out * float(np.dot(L.blk_n.astype(np.float64), nent))
+
+    def jac():
+        N.check(N.fn["sbi_bary_jacobian_f64"](ip._plan, xg.data_ptr(), Pg, xg.stride(0),
+                                              J.data_ptr(), st), "jacobian")
+
+    ws = int(N.fn["sbi_bary_integral_workspace"](ip._plan))
+    part = torch.empty(ws, dtype=torch.float64, device="cuda")
+    Q = torch.empty(d_out, dtype=torch.float64, device="cuda")
+
+    def integ():
+        N.check(N.fn["sbi_bary_integral_f64"](ip._plan, Q.data_ptr(), part.data_ptr(), st),
+                "integral")
+
+    out = {}
+    try:
+        t0 = time.perf_counter()
+        jac()
+        torch.cuda.synchronize()
+        once = time.perf_counter() - t0
+        js = max(1, min(steps, int(5.0 / max(once, 1e-6))))
+        _log(f"[bench] jacobian: {Pg} points, first call {once * 1e3:.1f} ms, {js} timed calls")
+        _, jm, _ = timed_loop(jac, js, 1, torch)
+        tf = flop_pt * Pg / (jm * 1e-3) / 1e12
+        out["gradient"] = {"kernel": "bary_jacobian_kernel", "points": Pg, "kernel_ms": jm,
+                           "point_evals_per_s": Pg / (jm * 1e-3),
+                           "roofline": {"bound": "fp64", "achieved": tf, "peak": PEAK_FP64_TFLOPS,
+                                        "unit": "TFLOP/s", "frac": tf / PEAK_FP64_TFLOPS,
+                                        "flop_per_point": flop_pt}}
+    except Exception as e:  # a documented limit of the Jacobian kernel (dims / degree per block)
+        out["gradient"] = {"error": str(e)}
+    _log("[bench] integral")
+    _, im, _ = timed_loop(integ, steps, 2, torch)
+    ib = 8.0 * L.k_raw * d_out
+    out["integral"] = {"kernel": "bary_integral_partial_kernel + final", "kernel_ms": im,
+                       "roofline": {"bound": "hbm", "achieved": ib / (im * 1e-3) / 1e9,
+                                    "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                    "frac": ib / (im * 1e-3) / 1e9 / PEAK_HBM_GBS}}
+    return out
 
 
 def _log(msg):
@@ -328,6 +393,8 @@ def main():
                  "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                  "frac": flops / t_b / 1e12 / PEAK_FP64_TFLOPS, "avg_launch_ms": results["fp64"]["kernel_ms"]}
 
+    rows = bench_gradient_integral(ip, x, torch)
+
     cpu = None
     if not args.no_cpu:
         import torch as _t
@@ -371,6 +438,7 @@ def main():
         "paths": results,
         "speedup_mixed_vs_fp64": results["mixed"]["value"] / results["fp64"]["value"],
         "accuracy": accuracy,
+        "gradient_integral": rows,
         "serve": serve,
         "setup_s": setup_s,
     }

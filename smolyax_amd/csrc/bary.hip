@@ -21,6 +21,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -349,11 +350,22 @@ __device__ __forceinline__ void basis_and_deriv_lds(double x, const double* __re
   }
 }
 
+constexpr int kJO = 8;  // outputs accumulated in registers per pass of the jacobian kernel
+
+// One workgroup = one dim j for 64 points (one per lane).  For each pass over kJO outputs, every
+// block containing j contributes sum_e d(prod_i Lhat_i)/dx_j [e] * F_b[e, o]: the factors' bases
+// (and the derivative of factor j) are computed per lane into LDS, entries are walked with an
+// odometer (last factor innermost, the others' product hoisted), F is read through scalar loads
+// (the same for all lanes) and the sums stay in registers until one write per output.
 __global__ __launch_bounds__(64) void bary_jacobian_kernel(sbi_bary_desc d,
                                                            const double* __restrict__ x, int64_t P,
                                                            int64_t ldx, double* __restrict__ J) {
-  __shared__ double Ls[kJMaxN * kJDm * kTile];
-  __shared__ double dLs[kJDm * kTile];
+  // LDS (sized by the plan): max_n factor rows of dmax1 basis values per lane, then the
+  // derivative row of factor j
+  extern __shared__ double jl_raw[];
+  const int dm = d.dmax1;
+  double* Ls = jl_raw;
+  double* dLs = jl_raw + (int64_t)d.max_n * dm * kTile;
   const int lane = threadIdx.x;
   const int j = blockIdx.y;
   const int64_t p_raw = (int64_t)blockIdx.x * kTile + lane;
@@ -361,40 +373,73 @@ __global__ __launch_bounds__(64) void bary_jacobian_kernel(sbi_bary_desc d,
   const int64_t p = valid ? p_raw : (P - 1);
   const double* __restrict__ xp = x + p * ldx;
   const int64_t dout = d.d_out;
+  cptr<int32_t> dim_blk = C(d.dim_blk), dbi = C(d.dim_blk_idx), dbp = C(d.dim_blk_pos),
+                blk_fac = C(d.blk_fac), blast = C(d.blk_last_slot), fac_slot = C(d.fac_slot),
+                slot_deg = C(d.slot_deg), slot_dim = C(d.slot_dim);
+  cptr<int64_t> foff = C(d.blk_foff);
+  cptr<double> F = C(d.F);
+  const int q0 = dim_blk[j], q1 = dim_blk[j + 1];
 
-  if (valid)
-    for (int64_t o = 0; o < dout; ++o) J[(p * dout + o) * d.d_in + j] = 0.0;
-
-  for (int q = d.dim_blk[j]; q < d.dim_blk[j + 1]; ++q) {
-    const int b = d.dim_blk_idx[q];
-    const int pos = d.dim_blk_pos[q];
-    const int f0 = d.blk_fac[b];
-    const int npre = d.blk_fac[b + 1] - f0;
-    const int ls = d.blk_last_slot[b];
-    const int nf = npre + (ls >= 0 ? 1 : 0);
-    int64_t nent = 1;
-    for (int i = 0; i < nf; ++i) {
-      const int s = (i < npre) ? d.fac_slot[f0 + i] : ls;
-      const int deg = d.slot_deg[s];
-      nent *= (deg + 1);
-      basis_and_deriv_lds(xp[d.slot_dim[s]], d.slot_nodes + (int64_t)s * d.dmax1,
-                          d.slot_w + (int64_t)s * d.dmax1, deg, i == pos,
-                          Ls + i * kJDm * kTile, dLs, lane);
-    }
-    // entries in C order over factors (last factor fastest)
-    const double* __restrict__ Fb = d.F + d.blk_foff[b] * dout;
-    for (int64_t e = 0; e < nent; ++e) {
-      int64_t rem = e;
-      double wgt = 1.0;
-      for (int i = nf - 1; i >= 0; --i) {
-        const int s = (i < npre) ? d.fac_slot[f0 + i] : ls;
-        const int di1 = d.slot_deg[s] + 1;
-        const int mi = (int)(rem % di1);
-        rem /= di1;
-        wgt *= (i == pos) ? dLs[mi * kTile + lane] : Ls[(i * kJDm + mi) * kTile + lane];
+  for (int64_t o0 = (int64_t)blockIdx.z * kJO; o0 < dout; o0 += (int64_t)gridDim.z * kJO) {
+    const int no = (int)min((int64_t)kJO, dout - o0);
+    double acc[kJO];
+#pragma unroll
+    for (int oi = 0; oi < kJO; ++oi) acc[oi] = 0.0;
+    for (int q = q0; q < q1; ++q) {
+      const int b = dbi[q], pos = dbp[q];
+      const int f0 = blk_fac[b], npre = blk_fac[b + 1] - f0, ls = blast[b];
+      const int nf = npre + (ls >= 0 ? 1 : 0);
+      int deg1[kJMaxN], idx[kJMaxN];
+#pragma unroll
+      for (int i = 0; i < kJMaxN; ++i) {
+        deg1[i] = 1;
+        idx[i] = 0;
+        if (i < nf) {
+          const int s = (i < npre) ? fac_slot[f0 + i] : ls;
+          const int dg = slot_deg[s];
+          deg1[i] = dg + 1;
+          basis_and_deriv_lds(xp[slot_dim[s]], d.slot_nodes + (int64_t)s * d.dmax1,
+                              d.slot_w + (int64_t)s * d.dmax1, dg, i == pos,
+                              Ls + i * dm * kTile, dLs, lane);
+        }
       }
-      if (valid)
-        for (int64_t o = 0; o < dout; ++o) J[(p * dout + o) * d.d_in + j] += wgt * Fb[e * dout + o];
+      const double* lrow = (nf - 1 == pos) ? dLs : Ls + (nf - 1) * dm * kTile;
+      int nl = 1;
+#pragma unroll
+      for (int i = 0; i < kJMaxN; ++i)
+        if (i == nf - 1) nl = deg1[i];
+      int64_t e = foff[b];
+      while (true) {
+        double pre = 1.0;
+#pragma unroll
+        for (int i = 0; i < kJMaxN - 1; ++i)
+          if (i < nf - 1)
+            pre *= (i == pos) ? dLs[idx[i] * kTile + lane] : Ls[(i * dm + idx[i]) * kTile + lane];
+        for (int m = 0; m < nl; ++m, ++e) {
+          const double wgt = pre * lrow[m * kTile + lane];
+          cptr<double> Fe = F + e * dout + o0;
+#pragma unroll
+          for (int oi = 0; oi < kJO; ++oi)
+            if (oi < no) acc[oi] = fma(wgt, Fe[oi], acc[oi]);
+        }
+        bool carry = true;  // odometer over the factors before the last, last-but-one fastest
+#pragma unroll
+        for (int i = kJMaxN - 2; i >= 0; --i) {
+          if (carry && i < nf - 1) {
+            if (++idx[i] < deg1[i]) {
+              carry = false;
+            } else {
+              idx[i] = 0;
+            }
+          }
+        }
+        if (carry) break;
+      }
+    }
+    if (valid) {
+#pragma unroll
+      for (int oi = 0; oi < kJO; ++oi)
+        if (oi < no) J[(p * dout + o0 + oi) * d.d_in + j] = acc[oi];
     }
   }
 }
@@ -555,12 +600,28 @@ extern "C" int sbi_bary_jacobian_f64(const sbi_plan* plan, const double* x, int6
   }
   if (P == 0) return SBI_OK;
   const sbi_bary_desc& d = plan->bary;
-  dim3 grid((unsigned)((P + kTile - 1) / kTile), (unsigned)d.d_in);
+  // passes of kJO outputs are spread over grid.z until the launch has ~4096 workgroups
+  const int64_t tiles = (P + kTile - 1) / kTile;
+  const int64_t passes = (d.d_out + kJO - 1) / kJO;
+  const int64_t base = tiles * d.d_in;
+  const int64_t gz = std::max<int64_t>(1, std::min<int64_t>(passes, (4096 + base - 1) / base));
+  if (tiles > 0x7fffffff || d.d_in > 65535) {
+    sbi::set_error("sbi_bary_jacobian_f64: too many points or input dims for one launch");
+    return SBI_ERR_INVALID;
+  }
+  dim3 grid((unsigned)tiles, (unsigned)d.d_in, (unsigned)gz);
   if (d.max_n > kJMaxN || d.dmax1 > kJDm) {
     sbi::set_error("sbi_bary_jacobian_f64: supports at most 8 active dims and degree 15");
     return SBI_ERR_UNSUPPORTED;
   }
-  hipLaunchKernelGGL(bary_jacobian_kernel, grid, dim3(64), 0, (hipStream_t)stream, d, x, P, ldx,
+  const size_t lds = (size_t)(d.max_n + 1) * d.dmax1 * kTile * sizeof(double);
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)bary_jacobian_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr = true;
+  }
+  hipLaunchKernelGGL(bary_jacobian_kernel, grid, dim3(64), lds, (hipStream_t)stream, d, x, P, ldx,
                      J);
   SBI_HIP_CHECK(hipGetLastError());
   return SBI_OK;

@@ -110,3 +110,20 @@ def test_vjp_non_nested_matches_oracle(gpu):
     v = rng.standard_normal((20, 3))
     g_ref = np.einsum("pm,pmj->pj", v, ref.gradient(x))
     assert rel(ip.vjp(x, v).cpu().numpy(), g_ref) < 1e-10
+
+
+def test_jacobian_many_outputs_matches_adjoint(gpu):
+    """Dense Jacobian at a size the oracle does not reach (700 points, d_out 600: several point
+    tiles and several passes of outputs per workgroup) against the FP64 adjoint, which shares no
+    kernel with it: v^T J == vjp(x, v) to 1e-10 relative."""
+    d, d_out = 20, 600
+    ref, ip, rng = make(d, d_out, 400, seed=8)
+    x = rng.uniform(-1, 1, (700, d))
+    v = rng.standard_normal((700, d_out))
+    J = ip.gradient(x)
+    assert tuple(J.shape) == (700, d_out, d)
+    vt = torch.as_tensor(v, device="cuda")
+    g = torch.bmm(vt.unsqueeze(1), J).squeeze(1).cpu().numpy()
+    assert rel(g, ip.vjp(x, v, "fp64").cpu().numpy()) < 1e-10
+    # the first points against the oracle's Jacobian
+    assert rel(J[:3].cpu().numpy(), ref.gradient(x[:3])) < 1e-11
