@@ -175,10 +175,12 @@ def bench_gemm_config(args, torch, W, cfg):
 def bench_gradient_integral(ip, x, torch, steps=10):
     """gradient (dense Jacobian, FP64) and integral (quadrature sum over F) through the C ABI.
 
-    The Jacobian runs on at most 1024 of the points, fewer when J would pass 256 MB. Each
-    (point, dim j, entry of a block holding j) costs one FMA per output, so its roofline is FP64
-    FLOP/s: 2 * d_out * sum_b n_b * nent_b per point. The integral reads F
-    (K_raw * d_out * 8 bytes) once: HBM roofline."""
+    The Jacobian runs the path gradient() takes (the Newton form's sbi_newton_jacobian_f64 when
+    the plan has one, else the barycentric kernel) on at most 4096 of the points, fewer when J
+    would pass 1 GiB. Algorithmic work per point: J bytes 8 * d_out * d_in, and 2 * d_out FLOP
+    per derivative-basis term (Newton: one term per (node, factor) pair; barycentric:
+    sum_b n_b * nent_b); the roofline is the bound with the longer ideal time. The integral reads
+    F (K_raw * d_out * 8 bytes) once: HBM roofline."""
     import ctypes
 
     import numpy as np
@@ -189,19 +191,32 @@ def bench_gradient_integral(ip, x, torch, steps=10):
     L = ip.layout
     d_in, d_out = ip.d_in, ip.d_out
     per_point = 8 * d_out * d_in
-    Pg = max(64, min(int(x.shape[0]), 1024, (256 << 20) // per_point) // 64 * 64)
+    Pg = max(64, min(int(x.shape[0]), 4096, (1 << 30) // per_point) // 64 * 64)
     Pg = min(Pg, int(x.shape[0]))
     xg = x[:Pg]
     J = torch.empty((Pg, d_out, d_in), dtype=torch.float64, device="cuda")
-    nent = np.diff(L.blk_foff).astype(np.float64)
-    flop_pt = 2.0 * d_out * float(np.dot(L.blk_n.astype(np.float64), nent))
+    bary_terms = float(np.dot(L.blk_n.astype(np.float64), np.diff(L.blk_foff).astype(np.float64)))
+    if ip._nplan is not None:
+        kernel = "sbi_newton_jacobian_f64"
+        terms = float(ip._nlayout.n_vjp_ent)
+        S = ip._S["f64"]
+        nb = int(N.fn["sbi_newton_jacobian_workspace"](ip._nplan, Pg, d_out))
+        ws = torch.empty(max(1, (nb + 7) // 8), dtype=torch.float64, device="cuda")
 
-    def jac():
-        N.check(N.fn["sbi_bary_jacobian_f64"](ip._plan, xg.data_ptr(), Pg, xg.stride(0),
-                                              J.data_ptr(), st), "jacobian")
+        def jac():
+            N.check(N.fn["sbi_newton_jacobian_f64"](ip._nplan, xg.data_ptr(), Pg, xg.stride(0),
+                                                    S.data_ptr(), S.stride(0), d_out, J.data_ptr(),
+                                                    ws.data_ptr(), nb, st), "jacobian")
+    else:
+        kernel = "sbi_bary_jacobian_f64"
+        terms = bary_terms
 
-    ws = int(N.fn["sbi_bary_integral_workspace"](ip._plan))
-    part = torch.empty(ws, dtype=torch.float64, device="cuda")
+        def jac():
+            N.check(N.fn["sbi_bary_jacobian_f64"](ip._plan, xg.data_ptr(), Pg, xg.stride(0),
+                                                  J.data_ptr(), st), "jacobian")
+
+    ws_i = int(N.fn["sbi_bary_integral_workspace"](ip._plan))
+    part = torch.empty(ws_i, dtype=torch.float64, device="cuda")
     Q = torch.empty(d_out, dtype=torch.float64, device="cuda")
 
     def integ():
@@ -215,20 +230,30 @@ def bench_gradient_integral(ip, x, torch, steps=10):
         torch.cuda.synchronize()
         once = time.perf_counter() - t0
         js = max(1, min(steps, int(5.0 / max(once, 1e-6))))
-        _log(f"[bench] jacobian: {Pg} points, first call {once * 1e3:.1f} ms, {js} timed calls")
+        _log(f"[bench] jacobian ({kernel}): {Pg} points, first call {once * 1e3:.1f} ms, "
+             f"{js} timed calls")
         _, jm, _ = timed_loop(jac, js, 1, torch)
-        tf = flop_pt * Pg / (jm * 1e-3) / 1e12
-        out["gradient"] = {"kernel": "bary_jacobian_kernel", "points": Pg, "kernel_ms": jm,
-                           "point_evals_per_s": Pg / (jm * 1e-3),
-                           "roofline": {"bound": "fp64", "achieved": tf, "peak": PEAK_FP64_TFLOPS,
-                                        "unit": "TFLOP/s", "frac": tf / PEAK_FP64_TFLOPS,
-                                        "flop_per_point": flop_pt}}
-    except Exception as e:  # a documented limit of the Jacobian kernel (dims / degree per block)
+        flops = 2.0 * d_out * terms * Pg
+        jbytes = 8.0 * Pg * d_out * d_in + 8.0 * Pg * d_in
+        t_s = jm * 1e-3
+        if jbytes / (PEAK_HBM_GBS * 1e9) >= flops / (PEAK_FP64_TFLOPS * 1e12):
+            roof = {"bound": "hbm", "achieved": jbytes / t_s / 1e9, "peak": PEAK_HBM_GBS,
+                    "unit": "GB/s"}
+        else:
+            roof = {"bound": "fp64", "achieved": flops / t_s / 1e12, "peak": PEAK_FP64_TFLOPS,
+                    "unit": "TFLOP/s"}
+        roof["frac"] = roof["achieved"] / roof["peak"]
+        roof.update({"flop_per_point": 2.0 * d_out * terms, "bytes_per_point": jbytes / Pg})
+        out["gradient"] = {"kernel": kernel, "points": Pg, "ms": jm,
+                           "point_evals_per_s": Pg / t_s, "roofline": roof,
+                           "barycentric_equivalent_tflops": 2.0 * d_out * bary_terms * Pg / t_s
+                           / 1e12}
+    except Exception as e:  # a documented limit of the Jacobian kernels
         out["gradient"] = {"error": str(e)}
     _log("[bench] integral")
     _, im, _ = timed_loop(integ, steps, 2, torch)
     ib = 8.0 * L.k_raw * d_out
-    out["integral"] = {"kernel": "bary_integral_partial_kernel + final", "kernel_ms": im,
+    out["integral"] = {"kernel": "bary_integral_partial_kernel + final", "ms": im,
                        "roofline": {"bound": "hbm", "achieved": ib / (im * 1e-3) / 1e9,
                                     "peak": PEAK_HBM_GBS, "unit": "GB/s",
                                     "frac": ib / (im * 1e-3) / 1e9 / PEAK_HBM_GBS}}

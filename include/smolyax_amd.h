@@ -192,7 +192,7 @@ typedef struct sbi_newton_desc {
   int32_t basis_n;            /* rows of H^T: all nodes of Lambda including the root */
   int32_t n_tab_dims;         /* dims with a factor in some node */
   int32_t n_tab_rows;         /* rows of the per-tile factor table: sum over dims of max deg */
-  int32_t _pad3;
+  int32_t n_vjp_ent;          /* adjoint entries: one per (node nu, factor (j, k) of nu) */
   const void* basis_desc;     /* [basis_n] 16 bytes: 8 x u16 factor-table rows (0xffff: none) */
   const int32_t* tab_dim;     /* [n_tab_dims] */
   const int32_t* tab_row;     /* [n_tab_dims] first table row of the dim (row of k is row+k-1) */
@@ -200,6 +200,12 @@ typedef struct sbi_newton_desc {
   const int32_t* vjp_slot_off;/* [n_tab_rows + 1] CSR over factor rows of the adjoint entries */
   const void* vjp_ent;        /* 16 bytes per entry: int32 node row of G^T, 6 x u16 table rows
                                  of the node's other factors (0xffff: none) */
+  /* dense Jacobian (optional) */
+  const int32_t* vjp_dim_off; /* [d_in + 1] adjoint entries of each input dim (entries are stored
+                                 by dim; empty range: the dim has no factor) */
+  const int32_t* sing_off;    /* [d_in + 1] CSR over dims of sing_x */
+  const double* sing_x;       /* coordinates of dim j at which dy/dx_j is NaN: the nodes of every
+                                 (dim j, degree) factor of the barycentric blocks */
 } sbi_newton_desc;
 
 typedef struct sbi_newton_plan sbi_newton_plan;
@@ -236,6 +242,20 @@ int sbi_newton_rows_f64(const sbi_newton_plan* plan, const double* x, int64_t P,
  * (dtype 0: FP64, 1: FP32).  Deterministic (no atomics). */
 int sbi_newton_vjp(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
                    const void* GT, int64_t ldg, int32_t dtype, double* gx, void* stream);
+/* Bytes of device workspace sbi_newton_jacobian_f64 needs for P points and n outputs: the factor
+ * table when it exceeds LDS, and for n > 8 the derivative basis D^T (adjoint entries x P). */
+int64_t sbi_newton_jacobian_workspace(const sbi_newton_plan* plan, int64_t P, int64_t n);
+/* Dense Jacobian of the Newton form, J[P, n, d_in] (FP64, contiguous), the reference's gradient
+ * (interpolation.py:306-345, barycentric.py:126-229) of the same polynomial:
+ *   J[p, o, j] = sum over nodes nu with a factor (j, k):  S[nu, o] h'_k(x_j) prod_{other (j',k')} h_k'(x_j')
+ * S: surpluses [basis_n, lds], rows in the node order of the adjoint (the rows of G^T of
+ * sbi_newton_vjp).  n <= 8: one pass, sums in registers;
+ * n > 8: D^T is written to the workspace and contracted on the FP64 matrix cores, one group of rows
+ * per input dim.  Then J[p, :, j] = NaN where x_j is one of sing_x of dim j (the reference's result
+ * at a node coordinate).  Deterministic. */
+int sbi_newton_jacobian_f64(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
+                            const double* S, int64_t lds, int64_t n, double* J, void* workspace,
+                            int64_t ws_bytes, void* stream);
 /* Barycentric entry weights W^T[k_raw, ldw] (FP64) at x[P, ldx]: y = offset + W F, the
  * reference's per-block contraction (barycentric.py:69-123) as one dense GEMM (large d_out). */
 int sbi_bary_basis_f64(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx, double* WT,

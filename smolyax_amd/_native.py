@@ -54,9 +54,10 @@ class NewtonDesc(ctypes.Structure):
         ("n_nodes_pad", c_i32), ("n_hs_dims", c_i32), ("n_hs_rows", c_i32), ("_pad", c_i32),
         ("node_sq", c_vp), ("grp_rec", c_vp), ("grp_j", c_vp),
         ("hs_dim", c_vp), ("hs_kmax", c_vp), ("hs_row", c_vp), ("int_rec", c_vp),
-        ("basis_n", c_i32), ("n_tab_dims", c_i32), ("n_tab_rows", c_i32), ("_pad3", c_i32),
+        ("basis_n", c_i32), ("n_tab_dims", c_i32), ("n_tab_rows", c_i32), ("n_vjp_ent", c_i32),
         ("basis_desc", c_vp), ("tab_dim", c_vp), ("tab_row", c_vp), ("tab_kmax", c_vp),
         ("vjp_slot_off", c_vp), ("vjp_ent", c_vp),
+        ("vjp_dim_off", c_vp), ("sing_off", c_vp), ("sing_x", c_vp),
     ]
 
 
@@ -90,6 +91,9 @@ SIGNATURES = {
     "sbi_newton_eval_f64": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp]),
     "sbi_newton_basis": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_i32, c_vp]),
     "sbi_newton_vjp": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_i32, c_vp, c_vp]),
+    "sbi_newton_jacobian_workspace": (c_i64, [c_vp, c_i64, c_i64]),
+    "sbi_newton_jacobian_f64": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, c_vp, c_vp,
+                                        c_i64, c_vp]),
     "sbi_bary_basis_f64": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp]),
     "sbi_bary_basis_f32": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp]),
     "sbi_newton_mixed_staged": (c_i32, [c_vp]),

@@ -320,33 +320,31 @@ __global__ __launch_bounds__(kTile* kWaves) void bary_eval_kernel(
 constexpr int kJMaxN = 8;    // max active dims per block supported by the jacobian kernel
 constexpr int kJDm = 16;     // max (deg + 1) per factor in the jacobian kernel
 
-// Basis (first form) and, for the differentiated factor, its derivative
-//   dL_mu = w_mu sum_{k != mu} prod_{nu != mu, k} (x - xi_nu)
-// into LDS rows [f][m][lane].  The derivative is NaN if x is a node of the slot, which is the
-// reference's result there (barycentric.py:152-154 turns a 0 denominator into NaN).
+// Basis (first form) L_mu = w_mu prod_{nu != mu} (x - xi_nu) and, for the differentiated factor,
+// its derivative by the product rule (running pair (p, p') over the factors, O(deg^2) per point)
+// into LDS rows [f][m][lane].  At x == xi_mu the basis is one-hot; the derivative is NaN where
+// (x - xi_nu)^2 == 0 for some nu <= deg, the reference's singular mask (barycentric.py:152-154).
 __device__ __forceinline__ void basis_and_deriv_lds(double x, const double* __restrict__ xi,
                                                     const double* __restrict__ w, int deg,
                                                     bool want_d, double* __restrict__ Lrow,
                                                     double* __restrict__ dLrow, int lane) {
   int hit = -1;
-  for (int m = 0; m <= deg; ++m)
-    if (x - xi[m] == 0.0) hit = m;
+  bool sing = false;
   for (int m = 0; m <= deg; ++m) {
-    double v = w[m];
-    for (int q = 0; q <= deg; ++q)
-      if (q != m) v *= (x - xi[q]);
-    Lrow[m * kTile + lane] = (hit >= 0) ? (m == hit ? 1.0 : 0.0) : v;
-    if (want_d) {
-      double dv = 0.0;
-      for (int k = 0; k <= deg; ++k) {
-        if (k == m) continue;
-        double pr = w[m];
-        for (int q = 0; q <= deg; ++q)
-          if (q != m && q != k) pr *= (x - xi[q]);
-        dv += pr;
-      }
-      dLrow[m * kTile + lane] = (hit >= 0) ? NAN : dv;
+    const double dx = x - xi[m];
+    if (dx == 0.0) hit = m;
+    sing |= (dx * dx == 0.0);
+  }
+  for (int m = 0; m <= deg; ++m) {
+    double v = w[m], dv = 0.0;
+    for (int q = 0; q <= deg; ++q) {
+      if (q == m) continue;
+      const double a = x - xi[q];
+      dv = dv * a + v;
+      v *= a;
     }
+    Lrow[m * kTile + lane] = (hit >= 0) ? (m == hit ? 1.0 : 0.0) : v;
+    if (want_d) dLrow[m * kTile + lane] = sing ? NAN : dv;
   }
 }
 

@@ -21,11 +21,14 @@
 #include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <string>
 
 #include "../../include/smolyax_amd.h"
 #include "common.h"
+#include "gemm.h"
 
 namespace {
 
@@ -218,6 +221,222 @@ __global__ __launch_bounds__(kTile* kBWaves) void newton_vjp_kernel(sbi_newton_d
     }
     if (valid) gx[p * d.d_in + j] = gj;
   }
+}
+
+// ------------------------------------------------------------------------------------------
+// Dense Jacobian of the Newton form.  Adjoint entry e = (node nu, factor (j, k) of nu) has the
+// derivative-basis value D[p, e] = h'_k(x_j) prod_{other factors (j',k') of nu} h_k'(x_j'), and
+//   J[p, o, j] = sum over the entries e of dim j of D[p, e] S[nu_e, o].
+// The factor table of the tile sits in LDS when it fits, else in a global table written first by
+// newton_htab_kernel ([row][ldt], points contiguous).
+// ------------------------------------------------------------------------------------------
+template <bool kG>
+struct HTab {
+  typename lds_t<double>::type* lds;
+  const double* __restrict__ g;
+  int64_t ldt;
+  __device__ __forceinline__ double at(int r, int lane, int64_t p) const {
+    if constexpr (kG) {
+      return g[(int64_t)r * ldt + p];
+    } else {
+      return lds[r * kTile + lane];
+    }
+  }
+};
+
+__global__ __launch_bounds__(256) void newton_htab_kernel(sbi_newton_desc d,
+                                                         const double* __restrict__ x, int64_t P,
+                                                         int64_t ldx, double* __restrict__ tab,
+                                                         int64_t ldt) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= P) return;
+  cptr<int32_t> tdim = C(d.tab_dim), trow = C(d.tab_row), tk = C(d.tab_kmax);
+  cptr<double> dim_xi = C(d.dim_xi), dim_c = C(d.dim_c);
+  for (int i = blockIdx.y; i < d.n_tab_dims; i += gridDim.y) {
+    const int j = tdim[i], row = trow[i], km = tk[i];
+    const double xj = x[p * ldx + j];
+    cptr<double> xi = dim_xi + (int64_t)j * d.kmax1;
+    cptr<double> c = dim_c + (int64_t)j * d.kmax1;
+    double pr = 1.0;
+    for (int k = 1; k <= km; ++k) {
+      pr *= (xj - xi[k - 1]);
+      tab[(int64_t)(row + k - 1) * ldt + p] = pr * c[k];
+    }
+  }
+}
+
+// One pass over n <= NO outputs, sums in registers.  Waves take table dims round-robin (grid.y
+// workgroups per tile, 4 waves each); (p, j) is written once.
+template <int NO, bool kG>
+__global__ __launch_bounds__(kTile* kBWaves) void newton_jac_kernel(
+    sbi_newton_desc d, const double* __restrict__ x, int64_t P, int64_t ldx,
+    const double* __restrict__ S, int64_t lds, int n, const double* __restrict__ gtab,
+    int64_t ldt, double* __restrict__ J) {
+  extern __shared__ double lds_raw[];
+  typedef typename lds_t<double>::type LT;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t p_raw = (int64_t)blockIdx.x * kTile + lane;
+  const bool valid = p_raw < P;
+  const int64_t p = valid ? p_raw : (P - 1);
+  const double* __restrict__ xp = x + p * ldx;
+  HTab<kG> H{(LT*)lds_raw, gtab, ldt};
+  if constexpr (!kG) {
+    build_h_table<double>(d, xp, H.lds, nullptr, wave, lane, kBWaves);
+    __syncthreads();
+  }
+  cptr<int32_t> tdim = C(d.tab_dim), trow = C(d.tab_row), tk = C(d.tab_kmax),
+                slot_off = C(d.vjp_slot_off);
+  cptr<double> dim_xi = C(d.dim_xi), dim_c = C(d.dim_c);
+  const uint4* __restrict__ ent = reinterpret_cast<const uint4*>(d.vjp_ent);
+  const int nw = (int)gridDim.y * kBWaves;
+  for (int i = (int)blockIdx.y * kBWaves + wave; i < d.n_tab_dims; i += nw) {
+    const int j = tdim[i], row = trow[i], km = tk[i];
+    const double xj = xp[j];
+    cptr<double> xi = dim_xi + (int64_t)j * d.kmax1;
+    cptr<double> cj = dim_c + (int64_t)j * d.kmax1;
+    double g[NO];
+#pragma unroll
+    for (int o = 0; o < NO; ++o) g[o] = 0.0;
+    double pr = 1.0, dpr = 0.0;  // prod_{r<k} (x - xi_r) and its derivative
+    for (int k = 1; k <= km; ++k) {
+      const double dx = xj - xi[k - 1];
+      dpr = dpr * dx + pr;
+      pr *= dx;
+      const int s = row + k - 1;
+      double A[NO];
+#pragma unroll
+      for (int o = 0; o < NO; ++o) A[o] = 0.0;
+      const int e0 = slot_off[s], e1 = slot_off[s + 1];
+      for (int base = e0; base < e1; base += 64) {
+        const int mine = base + lane;
+        const uint4 ev = (mine < e1) ? ent[mine]
+                                     : make_uint4(0u, 0xffffffffu, 0xffffffffu, 0xffffffffu);
+        const int cnt = min(64, e1 - base);
+        for (int q = 0; q < cnt; ++q) {
+          const int node = __builtin_amdgcn_readlane((int)ev.x, q);
+          const uint32_t ws[3] = {(uint32_t)__builtin_amdgcn_readlane((int)ev.y, q),
+                                  (uint32_t)__builtin_amdgcn_readlane((int)ev.z, q),
+                                  (uint32_t)__builtin_amdgcn_readlane((int)ev.w, q)};
+          double v = 1.0;
+#pragma unroll
+          for (int f = 0; f < 6; ++f) {
+            const uint32_t r = (ws[f >> 1] >> ((f & 1) * 16)) & 0xffff;
+            if (r != 0xffff) v *= H.at((int)r, lane, p);
+          }
+          cptr<double> Sn = C(S) + (int64_t)node * lds;
+#pragma unroll
+          for (int o = 0; o < NO; ++o)
+            if (o < n) A[o] = fma(v, Sn[o], A[o]);
+        }
+      }
+      const double dh = dpr * cj[k];
+#pragma unroll
+      for (int o = 0; o < NO; ++o) g[o] = fma(dh, A[o], g[o]);
+    }
+    if (valid) {
+#pragma unroll
+      for (int o = 0; o < NO; ++o)
+        if (o < n) J[(p * n + o) * d.d_in + j] = g[o];
+    }
+  }
+}
+
+// D^T[e, p] for all adjoint entries, in groups of 64 entries per wave (the groups are spread over
+// grid.y workgroups per tile).  Entries are stored by factor row; the derivative h'_k(x_j) of the
+// current row is recomputed when the row changes.
+template <bool kG>
+__global__ __launch_bounds__(kTile* kBWaves) void newton_jac_basis_kernel(
+    sbi_newton_desc d, const double* __restrict__ x, int64_t P, int64_t ldx,
+    const double* __restrict__ gtab, int64_t ldt, double* __restrict__ DT, int64_t ldd) {
+  extern __shared__ double lds_raw[];
+  typedef typename lds_t<double>::type LT;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t p_raw = (int64_t)blockIdx.x * kTile + lane;
+  const bool valid = p_raw < P;
+  const int64_t p = valid ? p_raw : (P - 1);
+  const double* __restrict__ xp = x + p * ldx;
+  HTab<kG> H{(LT*)lds_raw, gtab, ldt};
+  if constexpr (!kG) {
+    build_h_table<double>(d, xp, H.lds, nullptr, wave, lane, kBWaves);
+    __syncthreads();
+  }
+  cptr<int32_t> tdim = C(d.tab_dim), trow = C(d.tab_row), slot_off = C(d.vjp_slot_off);
+  cptr<double> dim_xi = C(d.dim_xi), dim_c = C(d.dim_c);
+  const uint4* __restrict__ ent = reinterpret_cast<const uint4*>(d.vjp_ent);
+  const int n_ent = d.n_vjp_ent;
+  const int ngrp = (n_ent + 63) / 64;
+  const int nw = (int)gridDim.y * kBWaves;
+  for (int gi = (int)blockIdx.y * kBWaves + wave; gi < ngrp; gi += nw) {
+    const int base = gi * 64;
+    // factor row of entry `base`: the last s with slot_off[s] <= base
+    int lo = 0, hi = d.n_tab_rows;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (slot_off[mid] <= base) lo = mid;
+      else hi = mid;
+    }
+    int s = lo, s_done = -1;
+    double dh = 0.0;
+    const int mine = base + lane;
+    const uint4 ev = (mine < n_ent) ? ent[mine]
+                                    : make_uint4(0u, 0xffffffffu, 0xffffffffu, 0xffffffffu);
+    const int cnt = min(64, n_ent - base);
+    for (int q = 0; q < cnt; ++q) {
+      const int e = base + q;
+      while (s + 1 < d.n_tab_rows && slot_off[s + 1] <= e) ++s;
+      if (s != s_done) {  // h'_k(x_j) of row s = (j, k): the dim is the last i with trow[i] <= s
+        int a = 0, b = d.n_tab_dims;
+        while (b - a > 1) {
+          const int mid = (a + b) >> 1;
+          if (trow[mid] <= s) a = mid;
+          else b = mid;
+        }
+        const int j = tdim[a], k = s - trow[a] + 1;
+        const double xj = xp[j];
+        cptr<double> xi = dim_xi + (int64_t)j * d.kmax1;
+        double pr = 1.0, dpr = 0.0;
+        for (int r = 0; r < k; ++r) {
+          const double dx = xj - xi[r];
+          dpr = dpr * dx + pr;
+          pr *= dx;
+        }
+        dh = dpr * dim_c[(int64_t)j * d.kmax1 + k];
+        s_done = s;
+      }
+      const uint32_t ws[3] = {(uint32_t)__builtin_amdgcn_readlane((int)ev.y, q),
+                              (uint32_t)__builtin_amdgcn_readlane((int)ev.z, q),
+                              (uint32_t)__builtin_amdgcn_readlane((int)ev.w, q)};
+      double v = dh;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        const uint32_t r = (ws[f >> 1] >> ((f & 1) * 16)) & 0xffff;
+        if (r != 0xffff) v *= H.at((int)r, lane, p);
+      }
+      if (valid) DT[(int64_t)e * ldd + p] = v;
+    }
+  }
+}
+
+// J[p, :, j] = NaN where x_j is a node coordinate of dim j (the reference's singular mask
+// (x - xi)^2 == 0 of barycentric.py:152-154); one thread per (p, j).
+__global__ __launch_bounds__(256) void jac_mark_nodes_kernel(sbi_newton_desc d,
+                                                            const double* __restrict__ x,
+                                                            int64_t P, int64_t ldx, int64_t n,
+                                                            double* __restrict__ J) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= P * d.d_in) return;
+  const int64_t p = t / d.d_in;
+  const int64_t j = t - p * d.d_in;
+  const double xv = x[p * ldx + j];
+  bool hit = false;
+  for (int i = d.sing_off[j]; i < d.sing_off[j + 1]; ++i) {
+    const double dv = xv - d.sing_x[i];
+    hit |= (dv * dv == 0.0);
+  }
+  if (hit)
+    for (int64_t o = 0; o < n; ++o) J[(p * n + o) * d.d_in + j] = NAN;
 }
 
 // Zero the gradient of dims that no factor touches (written separately so every entry of gx is
@@ -453,4 +672,139 @@ extern "C" int sbi_bary_basis_f64(const sbi_plan* plan, const double* x, int64_t
 extern "C" int sbi_bary_basis_f32(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx,
                                   float* WT, int64_t ldw, void* stream) {
   return bary_basis_launch<float>(plan, x, P, ldx, WT, ldw, stream);
+}
+
+// ------------------------------------------------------------------------------------------
+// Dense Jacobian of the Newton form
+// ------------------------------------------------------------------------------------------
+constexpr int kJacDirectMax = 8;  // outputs handled in one register pass
+
+static bool jac_global_table(const sbi_newton_desc& d) {
+  return (size_t)d.n_tab_rows * kTile * sizeof(double) > (size_t)kMaxLds;
+}
+
+extern "C" int64_t sbi_newton_jacobian_workspace(const sbi_newton_plan* plan, int64_t P,
+                                                 int64_t n) {
+  const sbi_newton_desc* dp = nullptr;
+  if (sbi_newton_plan_desc(plan, &dp) != SBI_OK || P < 0 || n < 0) return -1;
+  int64_t bytes = 0;
+  if (jac_global_table(*dp)) bytes += (int64_t)dp->n_tab_rows * P * (int64_t)sizeof(double);
+  if (n > kJacDirectMax) bytes += (int64_t)dp->n_vjp_ent * P * (int64_t)sizeof(double);
+  return bytes;
+}
+
+template <int NO, bool kG>
+static void jac_direct_launch(const sbi_newton_desc& d, const double* x, int64_t P, int64_t ldx,
+                              const double* S, int64_t lds, int n, const double* gtab,
+                              double* J, hipStream_t st) {
+  const size_t lds_bytes = kG ? 0 : (size_t)d.n_tab_rows * kTile * sizeof(double);
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)newton_jac_kernel<NO, kG>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
+    attr = true;
+  }
+  const int64_t tiles = (P + kTile - 1) / kTile;
+  // workgroups per tile: enough for ~2048 workgroups, at most one wave per table dim
+  const int64_t ny = std::max<int64_t>(
+      1, std::min<int64_t>((d.n_tab_dims + kBWaves - 1) / kBWaves, (2048 + tiles - 1) / tiles));
+  hipLaunchKernelGGL((newton_jac_kernel<NO, kG>), dim3((unsigned)tiles, (unsigned)ny),
+                     dim3(kTile * kBWaves), lds_bytes, st, d, x, P, ldx, S, lds, n, gtab, P, J);
+}
+
+template <bool kG>
+static void jac_basis_launch(const sbi_newton_desc& d, const double* x, int64_t P, int64_t ldx,
+                             const double* gtab, double* DT, hipStream_t st) {
+  const size_t lds_bytes = kG ? 0 : (size_t)d.n_tab_rows * kTile * sizeof(double);
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)newton_jac_basis_kernel<kG>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
+    attr = true;
+  }
+  const int64_t tiles = (P + kTile - 1) / kTile;
+  const int64_t ngrp = (d.n_vjp_ent + 63) / 64;
+  const int64_t ny = std::max<int64_t>(
+      1, std::min<int64_t>((ngrp + kBWaves - 1) / kBWaves, (4096 + tiles - 1) / tiles));
+  hipLaunchKernelGGL(newton_jac_basis_kernel<kG>, dim3((unsigned)tiles, (unsigned)ny),
+                     dim3(kTile * kBWaves), lds_bytes, st, d, x, P, ldx, gtab, P, DT, P);
+}
+
+extern "C" int sbi_newton_jacobian_f64(const sbi_newton_plan* plan, const double* x, int64_t P,
+                                       int64_t ldx, const double* S, int64_t lds, int64_t n,
+                                       double* J, void* workspace, int64_t ws_bytes,
+                                       void* stream) {
+  const sbi_newton_desc* dp = nullptr;
+  if (sbi_newton_plan_desc(plan, &dp) != SBI_OK || !x || !S || !J || P < 0 || n < 1 || lds < n ||
+      ws_bytes < 0) {
+    sbi::set_error("sbi_newton_jacobian_f64: invalid argument");
+    return SBI_ERR_INVALID;
+  }
+  const sbi_newton_desc& d = *dp;
+  if (!d.vjp_ent || !d.vjp_slot_off || !d.tab_dim || !d.vjp_dim_off || !d.sing_off ||
+      !d.sing_x || ldx < d.d_in) {
+    sbi::set_error("sbi_newton_jacobian_f64: plan has no Jacobian tables");
+    return SBI_ERR_INVALID;
+  }
+  if (P == 0) return SBI_OK;
+  if (d.d_in > 65535 || P > (int64_t)65535 * kTile) {
+    sbi::set_error("sbi_newton_jacobian_f64: at most 65535 input dims and 65535*64 points");
+    return SBI_ERR_INVALID;
+  }
+  const int64_t need = sbi_newton_jacobian_workspace(plan, P, n);
+  if (ws_bytes < need || (need > 0 && !workspace)) {
+    sbi::set_error("sbi_newton_jacobian_f64: workspace smaller than "
+                   "sbi_newton_jacobian_workspace()");
+    return SBI_ERR_INVALID;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const bool g = jac_global_table(d);
+  double* gtab = g ? (double*)workspace : nullptr;
+  double* DT = (double*)workspace + (g ? (int64_t)d.n_tab_rows * P : 0);
+  if (g && d.n_tab_dims > 0) {
+    const int64_t ny = std::min<int64_t>(d.n_tab_dims, 64);
+    hipLaunchKernelGGL(newton_htab_kernel, dim3((unsigned)((P + 255) / 256), (unsigned)ny),
+                       dim3(256), 0, st, d, x, P, ldx, gtab, P);
+    SBI_HIP_CHECK(hipGetLastError());
+  }
+  if (n <= kJacDirectMax) {
+    if (d.n_tab_dims < d.d_in) {  // dims without a factor keep 0
+      const int64_t tot = P * n * d.d_in;
+      hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                         J, tot);
+      SBI_HIP_CHECK(hipGetLastError());
+    }
+    const int ni = (int)n;
+    if (d.n_tab_dims > 0) {
+      if (g) {
+        if (n == 1) jac_direct_launch<1, true>(d, x, P, ldx, S, lds, ni, gtab, J, st);
+        else if (n <= 2) jac_direct_launch<2, true>(d, x, P, ldx, S, lds, ni, gtab, J, st);
+        else if (n <= 4) jac_direct_launch<4, true>(d, x, P, ldx, S, lds, ni, gtab, J, st);
+        else jac_direct_launch<8, true>(d, x, P, ldx, S, lds, ni, gtab, J, st);
+      } else {
+        if (n == 1) jac_direct_launch<1, false>(d, x, P, ldx, S, lds, ni, gtab, J, st);
+        else if (n <= 2) jac_direct_launch<2, false>(d, x, P, ldx, S, lds, ni, gtab, J, st);
+        else if (n <= 4) jac_direct_launch<4, false>(d, x, P, ldx, S, lds, ni, gtab, J, st);
+        else jac_direct_launch<8, false>(d, x, P, ldx, S, lds, ni, gtab, J, st);
+      }
+      SBI_HIP_CHECK(hipGetLastError());
+    }
+  } else {
+    if (d.n_vjp_ent > 0) {
+      if (g) jac_basis_launch<true>(d, x, P, ldx, gtab, DT, st);
+      else jac_basis_launch<false>(d, x, P, ldx, gtab, DT, st);
+      SBI_HIP_CHECK(hipGetLastError());
+    }
+    // one group of D^T rows per input dim (dims without entries get zeros), B rows gathered by
+    // the entries' node index (first int32 of each 16-byte entry)
+    const int rc = sbi::contract_grouped_f64(DT, P, S, lds, (const int32_t*)d.vjp_ent, 4,
+                                             d.vjp_dim_off, (int)d.d_in, J, n * d.d_in, d.d_in,
+                                             1, P, n, st);
+    if (rc != SBI_OK) return rc;
+  }
+  const int64_t tot = P * d.d_in;
+  hipLaunchKernelGGL(jac_mark_nodes_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                     d, x, P, ldx, n, J);
+  SBI_HIP_CHECK(hipGetLastError());
+  return SBI_OK;
 }

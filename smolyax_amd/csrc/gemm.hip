@@ -31,6 +31,7 @@
 
 #include "../../include/smolyax_amd.h"
 #include "common.h"
+#include "gemm.h"
 
 namespace {
 
@@ -64,12 +65,41 @@ __device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
 // contracted first with the FP64 MFMA, straight into the FP64 accumulators.  Lane li supplies the
 // A row of point 4*(li & 3) + (li >> 2) of its 16-point block, which puts the FP64 MFMA's result
 // (row g + 4r in register r of lane group g) into the FP32 layout (row 4g + r).
-template <typename T, int FN>
+//
+// Grouped form (kGrp): launch slice z = blockIdx.z contracts rows [zoff[z], zoff[z+1]) of A with
+// the B rows brow[k * brs] (a gather), into Y + z * yzs with column stride ycs.
+struct GroupArgs {
+  const int32_t* zoff;
+  const int32_t* brow;
+  int64_t brs;
+  int64_t ycs;
+  int64_t yzs;
+};
+
+template <typename T, int FN, bool kGrp = false>
 __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
     const T* __restrict__ A, int64_t lda, const T* __restrict__ B, int64_t ldb,
     const double* __restrict__ offset, double* __restrict__ Y, int64_t ldy, int64_t P, int64_t n,
     int64_t K, const double* __restrict__ A64, int64_t lda64, const double* __restrict__ B64,
-    int64_t ldb64, int64_t K64) {
+    int64_t ldb64, int64_t K64, GroupArgs ga) {
+  const int32_t* __restrict__ brow = nullptr;
+  int64_t ycs = 1;
+  if constexpr (kGrp) {
+    const int z = blockIdx.z;
+    const int64_t k0 = ga.zoff[z];
+    K = ga.zoff[z + 1] - k0;
+    A += k0 * lda;
+    brow = ga.brow + k0 * ga.brs;
+    Y += (int64_t)z * ga.yzs;
+    ycs = ga.ycs;
+  }
+  auto brow_of = [&](int64_t k) -> int64_t {
+    if constexpr (kGrp) {
+      return brow[k * ga.brs];
+    } else {
+      return k;
+    }
+  };
   using AccT = typename Acc<T>::type;
   constexpr bool kF32 = sizeof(T) == 4;
   const int lane = threadIdx.x & 63;
@@ -143,7 +173,7 @@ __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
     for (int s = 0; s < 4; ++s) {
       const int64_t k = k0 + 4 * g + s;
       const T* __restrict__ ar = A + k * lda;
-      const T* __restrict__ br = B + k * ldb;
+      const T* __restrict__ br = B + brow_of(k) * ldb;
 #pragma unroll
       for (int fm = 0; fm < 2; ++fm) a[fm][s] = ar[pcol[fm]];
 #pragma unroll
@@ -204,7 +234,7 @@ __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
       }
 #pragma unroll
       for (int fn = 0; fn < FN; ++fn) {
-        const T v = B[kc * ldb + jcol[fn]];
+        const T v = B[brow_of(kc) * ldb + jcol[fn]];
         b[fn][s] = in ? v : T(0);
       }
     }
@@ -225,7 +255,7 @@ __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
         // 16x16x4 holds row g + 4r (column = lane % 16 in both)
         const int64_t row = m_base + fm * 16 + (kF32 ? 4 * g + r : g + 4 * r);
         const double v = kF32 ? acc64[fm][fn][r] : (double)acc[fm][fn][r];
-        if (row < P) Y[row * ldy + col] = off + v;
+        if (row < P) Y[row * ldy + col * ycs] = off + v;
       }
     }
 }
@@ -238,19 +268,53 @@ int contract_launch(const T* A, int64_t lda, const T* B, int64_t ldb, const doub
   const unsigned gm = (unsigned)((P + kBlockM - 1) / kBlockM);
   if (n <= 32) {
     hipLaunchKernelGGL((contract_kernel<T, 1>), dim3(gm, (unsigned)((n + 31) / 32)), block, 0, st,
-                       A, lda, B, ldb, offset, Y, ldy, P, n, K, A64, lda64, B64, ldb64, K64);
+                       A, lda, B, ldb, offset, Y, ldy, P, n, K, A64, lda64, B64, ldb64, K64, GroupArgs{});
   } else if (n <= 64) {
     hipLaunchKernelGGL((contract_kernel<T, 2>), dim3(gm, (unsigned)((n + 63) / 64)), block, 0, st,
-                       A, lda, B, ldb, offset, Y, ldy, P, n, K, A64, lda64, B64, ldb64, K64);
+                       A, lda, B, ldb, offset, Y, ldy, P, n, K, A64, lda64, B64, ldb64, K64, GroupArgs{});
   } else {
     hipLaunchKernelGGL((contract_kernel<T, 4>), dim3(gm, (unsigned)((n + 127) / 128)), block, 0,
-                       st, A, lda, B, ldb, offset, Y, ldy, P, n, K, A64, lda64, B64, ldb64, K64);
+                       st, A, lda, B, ldb, offset, Y, ldy, P, n, K, A64, lda64, B64, ldb64, K64, GroupArgs{});
   }
   SBI_HIP_CHECK(hipGetLastError());
   return SBI_OK;
 }
 
 }  // namespace
+
+namespace sbi {
+
+int contract_grouped_f64(const double* A, int64_t lda, const double* B, int64_t ldb,
+                         const int32_t* brow, int64_t brs, const int32_t* zoff, int nz, double* Y,
+                         int64_t ldy, int64_t ycs, int64_t yzs, int64_t P, int64_t n,
+                         hipStream_t st) {
+  if (P <= 0 || n <= 0 || nz <= 0) return SBI_OK;
+  if (P > (int64_t)65535 * kBlockM || n > (int64_t)65535 * 32 || nz > 65535) {
+    set_error("contract_grouped_f64: launch too large");
+    return SBI_ERR_INVALID;
+  }
+  const dim3 block(64 * kWaves);
+  const unsigned gm = (unsigned)((P + kBlockM - 1) / kBlockM);
+  const GroupArgs ga{zoff, brow, brs, ycs, yzs};
+  if (n <= 32) {
+    hipLaunchKernelGGL((contract_kernel<double, 1, true>), dim3(gm, (unsigned)((n + 31) / 32), nz),
+                       block, 0, st, A, lda, B, ldb, nullptr, Y, ldy, P, n, (int64_t)0, nullptr,
+                       (int64_t)0, nullptr, (int64_t)0, (int64_t)0, ga);
+  } else if (n <= 64) {
+    hipLaunchKernelGGL((contract_kernel<double, 2, true>), dim3(gm, (unsigned)((n + 63) / 64), nz),
+                       block, 0, st, A, lda, B, ldb, nullptr, Y, ldy, P, n, (int64_t)0, nullptr,
+                       (int64_t)0, nullptr, (int64_t)0, (int64_t)0, ga);
+  } else {
+    hipLaunchKernelGGL((contract_kernel<double, 4, true>),
+                       dim3(gm, (unsigned)((n + 127) / 128), nz), block, 0, st, A, lda, B, ldb,
+                       nullptr, Y, ldy, P, n, (int64_t)0, nullptr, (int64_t)0, nullptr, (int64_t)0,
+                       (int64_t)0, ga);
+  }
+  SBI_HIP_CHECK(hipGetLastError());
+  return SBI_OK;
+}
+
+}  // namespace sbi
 
 extern "C" int sbi_contract(const void* A, int64_t lda, const void* B, int64_t ldb,
                             const double* offset, double* Y, int64_t ldy, int64_t P, int64_t n,

@@ -248,7 +248,10 @@ class SmolyakBarycentricInterpolator:
             "tab_kmax": _dev_tensor(NL.tab_kmax, i32, dev),
             "vjp_slot_off": _dev_tensor(NL.vjp_slot_off, i32, dev),
             "vjp_ent": torch.as_tensor(NL.vjp_ent.copy()).to(dev),
+            "vjp_dim_off": _dev_tensor(NL.vjp_dim_off, i32, dev),
         }
+        t["sing_off"], t["sing_x"] = (_dev_tensor(a, i32 if a.dtype == np.int32 else f64, dev)
+                                      for a in self._singular_coordinates())
         if NL.node_sq is not None:  # staged kernel tables (else the unstaged kernel runs)
             t["node_sq"] = _dev_tensor(NL.node_sq, i32, dev)
         # surpluses in nf.nus order (adjoint), and in the basis-row order of the GEMM paths (the
@@ -263,6 +266,7 @@ class SmolyakBarycentricInterpolator:
         d.n_split, d.n_levels, d.kmax1 = newton.N_WAVES, NL.n_levels, NL.kmax1
         d.n_nodes_pad, d.n_hs_dims, d.n_hs_rows = NL.n_nodes_pad, NL.n_hs_dims, NL.n_hs_rows
         d.basis_n, d.n_tab_dims, d.n_tab_rows = NL.basis_n, NL.n_tab_dims, NL.n_tab_rows
+        d.n_vjp_ent = NL.n_vjp_ent
         for name, ten in t.items():
             setattr(d, name, ten.data_ptr())
         self._ndesc = d
@@ -277,6 +281,20 @@ class SmolyakBarycentricInterpolator:
             return
         N.check(rc, "newton_plan")
         self._nplan = plan
+
+    def _singular_coordinates(self):
+        """CSR over input dims of the coordinates at which the reference's gradient is NaN: the
+        nodes of every (dim, degree) factor of the barycentric blocks (barycentric.py:152-154
+        masks (x - xi_mu)^2 == 0 for mu <= nu_i)."""
+        L = self._layout
+        per_dim = [[] for _ in range(self._d_in)]
+        for s, (dim, deg) in enumerate(L.slot_key):
+            per_dim[dim].extend(L.slot_nodes[s, : deg + 1].tolist())
+        vals = [np.unique(np.asarray(v, dtype=np.float64)) for v in per_dim]
+        off = np.zeros(self._d_in + 1, dtype=np.int32)
+        off[1:] = np.cumsum([len(v) for v in vals])
+        x = np.concatenate(vals) if off[-1] else np.zeros(1)
+        return off, x.astype(np.float64)
 
     def __del__(self):
         for attr, fn in (("_plan", "sbi_plan_destroy"), ("_nplan", "sbi_newton_plan_destroy")):
@@ -475,9 +493,29 @@ class SmolyakBarycentricInterpolator:
         J = torch.empty((P, self._d_out, self._d_in), dtype=torch.float64, device=self._device)
         if P == 0:
             return J
-        N.check(N.fn["sbi_bary_jacobian_f64"](self._plan, x.data_ptr(), P, x.stride(0), J.data_ptr(),
-                                              self._stream()), "jacobian")
+        if self._nplan is not None:
+            self._jacobian_newton(x, J)
+        else:
+            N.check(N.fn["sbi_bary_jacobian_f64"](self._plan, x.data_ptr(), P, x.stride(0),
+                                                  J.data_ptr(), self._stream()), "jacobian")
         return J
+
+    def _jacobian_newton(self, x, J):
+        """J of the Newton form (same polynomial, 6-8x fewer terms than the barycentric blocks):
+        sbi_newton_jacobian_f64 in point chunks whose workspace stays within memory_limit."""
+        P, n = x.shape[0], self._d_out
+        ws1 = int(N.fn["sbi_newton_jacobian_workspace"](self._nplan, 64, n))
+        chunk = self._point_chunk(max(1, ws1 // 64), 1, P) if ws1 > 0 else P
+        S = self._S["f64"]
+        for p0 in range(0, P, chunk):
+            p1 = min(P, p0 + chunk)
+            xc = x[p0:p1]
+            nb = int(N.fn["sbi_newton_jacobian_workspace"](self._nplan, p1 - p0, n))
+            ws = torch.empty(max(1, (nb + 7) // 8), dtype=torch.float64, device=self._device)
+            N.check(N.fn["sbi_newton_jacobian_f64"](self._nplan, xc.data_ptr(), p1 - p0,
+                                                    xc.stride(0), S.data_ptr(), S.stride(0), n,
+                                                    J[p0:p1].data_ptr(), ws.data_ptr(), nb,
+                                                    self._stream()), "jacobian")
 
     def integral(self) -> torch.Tensor:
         """interpolation.py:347-390: Q[d_out]; synchronises like the reference's block_until_ready."""

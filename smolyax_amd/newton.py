@@ -352,6 +352,12 @@ def build_basis_tables(nf: NewtonForm, L: NewtonLayout) -> None:
             ent.append(e)
         off.append(len(ent))
     L.vjp_slot_off = np.array(off, dtype=np.int32)
+    L.n_vjp_ent = len(ent)
+    # entries are stored by factor row, rows by dim: CSR of the entries over input dims
+    dim_off = np.zeros(nf.d_in + 1, dtype=np.int64)
+    for j in tab_dims:
+        dim_off[j + 1] = off[row0[j] + int(nf.maxdeg[j])] - off[row0[j]]
+    L.vjp_dim_off = np.cumsum(dim_off).astype(np.int32)
     L.vjp_ent = (np.stack(ent) if ent else np.full((1, 8), 0xFFFF, dtype=np.uint16)).reshape(-1).view(np.uint8)
 
 

@@ -127,6 +127,55 @@ def test_gradient_nan_at_nodes(gpu):
     assert rel(J[ok], Jr[ok]) <= 1e-12
 
 
+@pytest.mark.parametrize("d_out", [1, 12])
+def test_gradient_global_factor_table(gpu, d_out):
+    """400 input dims: the Newton factor table (369 rows) exceeds LDS and is kept in global
+    memory; both the register path (d_out <= 8) and the matrix-core path; NaN at a node."""
+    d = 400
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((d_out, d)) / (np.arange(d) + 1.0)
+    f = lambda x: np.sin(A @ x + 0.4)
+    ref, ip, ong = build_pair(d, d_out, 1000, f)
+    x = sample(ong, 5, seed=2)
+    x[1, 0] = ong[0](3)[2]
+    J = ip.gradient(x).cpu().numpy()
+    Jr = ref.gradient(x)
+    assert np.array_equal(np.isnan(J), np.isnan(Jr)) and np.isnan(J[1, :, 0]).all()
+    ok = ~np.isnan(Jr)
+    assert rel(J[ok], Jr[ok]) <= 1e-11, rel(J[ok], Jr[ok])
+
+
+def test_bary_jacobian_kernel(gpu):
+    """The barycentric Jacobian kernel (sbi_bary_jacobian_f64, used when the Newton form is
+    unavailable) through the C ABI: two output passes, two point tiles, NaN at a node."""
+    import ctypes
+
+    import torch
+
+    from smolyax_amd import _native as N
+
+    d, d_out = 6, 11
+    rng = np.random.default_rng(9)
+    A = rng.standard_normal((d_out, d))
+    f = lambda x: np.tanh(A @ x / d)
+    ref, ip, ong = build_pair(d, d_out, 200, f)
+    # coordinates kept 0.01 away from the end nodes +-1: at 2.7e-6 from a node the reference's
+    # formula (b / sum b with |b| ~ 1 / |x - xi|) loses 4.8e-10 relative against exact rational
+    # arithmetic while the first form used here stays within 4e-14
+    x = np.clip(sample(ong, 90, seed=5), -0.99, 0.99)
+    x[7, 2] = ong[2](4)[1]
+    xt = torch.as_tensor(x, device="cuda")
+    J = torch.empty((90, d_out, d), dtype=torch.float64, device="cuda")
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    N.check(N.fn["sbi_bary_jacobian_f64"](ip._plan, xt.data_ptr(), 90, xt.stride(0), J.data_ptr(),
+                                          st), "jacobian")
+    J = J.cpu().numpy()
+    Jr = ref.gradient(x)
+    assert np.array_equal(np.isnan(J), np.isnan(Jr)) and np.isnan(J[7, :, 2]).all()
+    ok = ~np.isnan(Jr)
+    assert rel(J[ok], Jr[ok]) <= 1e-11
+
+
 @pytest.mark.parametrize("d,d_out,m,kind,custom", CASES)
 def test_integral_parity(gpu, d, d_out, m, kind, custom):
     rng = np.random.default_rng(d + 5)

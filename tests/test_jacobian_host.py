@@ -1,0 +1,110 @@
+"""Dense Jacobian of the Newton form on the host, no GPU: a numpy model of what
+sbi_newton_jacobian_f64 computes from the plan's tables (factor rows, adjoint entries, their CSR
+over input dims) reproduces the oracle's gradient, and the singular-coordinate table gives the
+oracle's NaN pattern at node coordinates.
+
+Tolerance: 1e-11 relative to max |J| (FP64 in both; the Newton and barycentric forms differ only
+in rounding)."""
+
+import types
+
+import numpy as np
+import pytest
+
+from oracle import indices as OI
+from oracle import nodes as ON
+from oracle import smolyak as OS
+
+
+def jacobian_model(L, S, x):
+    """J[p, o, j] = sum over the adjoint entries e of dim j of D[p, e] S[node_e, o],
+    D[p, e] = h'_k(x_j) prod_{other factor rows r of e} h_r(x)."""
+    P, d = x.shape
+    R = L.n_tab_rows
+    h, dh = np.zeros((R, P)), np.zeros((R, P))
+    for i in range(L.n_tab_dims):
+        j, row = L.tab_dim[i], L.tab_row[i]
+        pr, dpr = np.ones(P), np.zeros(P)
+        for k in range(1, L.tab_kmax[i] + 1):
+            dx = x[:, j] - L.dim_xi[j, k - 1]
+            dpr = dpr * dx + pr
+            pr = pr * dx
+            h[row + k - 1] = pr * L.dim_c[j, k]
+            dh[row + k - 1] = dpr * L.dim_c[j, k]
+    ent = L.vjp_ent.view(np.uint16).reshape(-1, 8)[: L.n_vjp_ent]
+    node = ent[:, :2].copy().view(np.int32)[:, 0]
+    rows = np.repeat(np.arange(R), np.diff(L.vjp_slot_off))
+    D = dh[rows].copy()
+    for f in range(6):
+        r = ent[:, 2 + f].astype(np.int64)
+        has = r != 0xFFFF
+        D[has] *= h[r[has]]
+    J = np.zeros((P, S.shape[1], d))
+    for j in range(d):
+        e0, e1 = L.vjp_dim_off[j], L.vjp_dim_off[j + 1]
+        J[:, :, j] = D[e0:e1].T @ S[node[e0:e1]]
+    return J
+
+
+def _pair(d, d_out, m, seed):
+    from smolyax_amd import newton as NW
+    from smolyax_amd import nodes as SN
+
+    rng = np.random.default_rng(seed)
+    k = np.log((2 + np.arange(d)) / np.log(2))
+    ong, sng = ON.leja(dim=d), SN.Leja(dim=d)
+    t = OI.find_approximate_threshold(k, m, True)
+    A = rng.standard_normal((d_out, d)) / (np.arange(d) + 1.0)
+    f = lambda x: np.sin(np.asarray(x) @ A.T + 0.3)
+    ref = OS.Interpolant(d_out=d_out, node_gen=ong, k=k, t=t, f=f)
+    nf = NW.NewtonForm(k, t, sng, d_out)
+    zero = np.array([g(0)[0] for g in sng])
+    X = np.stack([nf.grid_point(nu, zero) for nu in nf.nus])
+    nf.set_values(f(X).reshape(len(X), d_out))
+    return ref, nf, NW.build_newton_layout(nf), rng, k, t, sng
+
+
+@pytest.mark.parametrize("d,d_out,m", [(1, 1, 9), (4, 3, 60), (8, 2, 300), (12, 1, 500)])
+def test_newton_jacobian_tables_match_oracle(d, d_out, m):
+    ref, nf, L, rng, _, _, _ = _pair(d, d_out, m, seed=d)
+    x = rng.uniform(-1, 1, (40, d))
+    S = np.asarray(nf.s, dtype=np.float64).reshape(len(nf.nus), d_out)
+    J = jacobian_model(L, S, x)
+    Jr = ref.gradient(x)
+    assert rel_ok(J, Jr, 1e-11)
+    # the CSR over dims covers every entry once, in order
+    assert L.vjp_dim_off[0] == 0 and L.vjp_dim_off[-1] == L.n_vjp_ent
+    assert np.all(np.diff(L.vjp_dim_off) >= 0)
+    assert L.n_vjp_ent == sum(len(nu) for nu in nf.nus)
+
+
+def rel_ok(a, b, tol):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b))) <= tol
+
+
+def test_singular_coordinates_give_the_oracle_nan_pattern():
+    """x_j set to nodes of dim j of several degrees (inside and beyond the largest degree any
+    block uses in dim j): NaN in J[p, :, j] exactly where x_j is in the table."""
+    from smolyax_amd import packing
+    from smolyax_amd.interpolation import SmolyakBarycentricInterpolator as SBI
+
+    d, d_out = 5, 2
+    ref, nf, _, rng, k, t, sng = _pair(d, d_out, 80, seed=3)
+    L = packing.build_layout(packing.nonzero_blocks(k, t), sng, d, d_out)
+    off, vals = SBI._singular_coordinates(types.SimpleNamespace(_layout=L, _d_in=d))
+    assert off.shape == (d + 1,) and off[-1] == len(vals) or (off[-1] == 0 and len(vals) == 1)
+    seq = [sng[j](12) for j in range(d)]
+    x = rng.uniform(-0.9, 0.9, (36, d))
+    for p in range(36):
+        j, mu = p % d, p // d
+        x[p, j] = seq[j][mu]
+    Jr = ref.gradient(x)
+    seen = set()
+    for p in range(36):
+        for j in range(d):
+            in_table = bool(np.any((x[p, j] - vals[off[j]:off[j + 1]]) ** 2 == 0.0))
+            assert np.isnan(Jr[p, :, j]).all() == in_table, (p, j)
+            assert np.isnan(Jr[p, :, j]).any() == in_table, (p, j)
+            if j == p % d:
+                seen.add(in_table)
+    assert seen == {True, False}  # node coordinates inside and beyond the table both occur
