@@ -206,6 +206,14 @@ typedef struct sbi_newton_desc {
   const int32_t* sing_off;    /* [d_in + 1] CSR over dims of sing_x */
   const double* sing_x;       /* coordinates of dim j at which dy/dx_j is NaN: the nodes of every
                                  (dim j, degree) factor of the barycentric blocks */
+  int32_t n_jac_units;        /* work units of the register pass (d_out <= 8) */
+  int32_t n_jac_partials;     /* units whose dim is split over several units */
+  int32_t n_jac_red;          /* dims split over several units */
+  int32_t _pad4;
+  const int32_t* jac_unit;    /* [4 * n_jac_units] {table dim index, first entry, end entry,
+                                 partial slot or -1 (the unit covers its dim)}; units of one dim
+                                 are consecutive */
+  const int32_t* jac_red;     /* [3 * n_jac_red] {dim, first partial slot, slot count} */
 } sbi_newton_desc;
 
 typedef struct sbi_newton_plan sbi_newton_plan;
@@ -243,15 +251,17 @@ int sbi_newton_rows_f64(const sbi_newton_plan* plan, const double* x, int64_t P,
 int sbi_newton_vjp(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
                    const void* GT, int64_t ldg, int32_t dtype, double* gx, void* stream);
 /* Bytes of device workspace sbi_newton_jacobian_f64 needs for P points and n outputs: the factor
- * table when it exceeds LDS, and for n > 8 the derivative basis D^T (adjoint entries x P). */
+ * table when it exceeds LDS; for n <= 8 the partial sums of split dims; for n > 8 the derivative
+ * basis D^T (adjoint entries x P). */
 int64_t sbi_newton_jacobian_workspace(const sbi_newton_plan* plan, int64_t P, int64_t n);
 /* Dense Jacobian of the Newton form, J[P, n, d_in] (FP64, contiguous), the reference's gradient
  * (interpolation.py:306-345, barycentric.py:126-229) of the same polynomial:
  *   J[p, o, j] = sum over nodes nu with a factor (j, k):  S[nu, o] h'_k(x_j) prod_{other (j',k')} h_k'(x_j')
  * S: surpluses [basis_n, lds], rows in the node order of the adjoint (the rows of G^T of
- * sbi_newton_vjp).  n <= 8: one pass, sums in registers;
- * n > 8: D^T is written to the workspace and contracted on the FP64 matrix cores, one group of rows
- * per input dim.  Then J[p, :, j] = NaN where x_j is one of sing_x of dim j (the reference's result
+ * sbi_newton_vjp).  n <= 8: one pass over work units of at most 128 entries, sums in registers
+ * (a dim split over several units adds their partial sums in unit order);
+ * n > 8: D^T is written to the workspace and contracted on the FP64 matrix cores with the
+ * entries' surplus rows, one group of rows per input dim, eight dims per workgroup.  Then J[p, :, j] = NaN where x_j is one of sing_x of dim j (the reference's result
  * at a node coordinate).  Deterministic. */
 int sbi_newton_jacobian_f64(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
                             const double* S, int64_t lds, int64_t n, double* J, void* workspace,

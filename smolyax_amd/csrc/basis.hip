@@ -265,13 +265,39 @@ __global__ __launch_bounds__(256) void newton_htab_kernel(sbi_newton_desc d,
   }
 }
 
-// One pass over n <= NO outputs, sums in registers.  Waves take table dims round-robin (grid.y
-// workgroups per tile, 4 waves each); (p, j) is written once.
+// Factor row of adjoint entry e (the last s < n_rows with off[s] <= e): a branchless binary
+// search, one entry per lane, so that a wave resolves 64 entries with ~log2(n_rows) loads.
+__device__ __forceinline__ int row_of_entry(const int32_t* __restrict__ off, int n_rows, int e) {
+  int lo = 0;
+  for (int step = 1 << (31 - __builtin_clz(max(n_rows, 1))); step > 0; step >>= 1) {
+    const int c = lo + step;
+    if (c < n_rows && off[c] <= e) lo = c;
+  }
+  return lo;
+}
+
+// h'_k(x_j) of factor row s = (j, k) of table dim i (row s = trow[i] + k - 1).
+__device__ __forceinline__ double row_derivative(const sbi_newton_desc& d, const double* xp, int j,
+                                                 int k) {
+  cptr<double> xi = C(d.dim_xi) + (int64_t)j * d.kmax1;
+  const double xj = xp[j];
+  double pr = 1.0, dpr = 0.0;
+  for (int r = 0; r < k; ++r) {
+    const double dx = xj - xi[r];
+    dpr = dpr * dx + pr;
+    pr *= dx;
+  }
+  return dpr * C(d.dim_c)[(int64_t)j * d.kmax1 + k];
+}
+
+// One pass over n <= NO outputs, sums in registers.  Waves take work units (runs of at most 128
+// adjoint entries of one dim) round-robin over grid.y workgroups per tile; a unit that covers
+// its dim writes J, the others write partial sums that jac_reduce_kernel adds in unit order.
 template <int NO, bool kG>
 __global__ __launch_bounds__(kTile* kBWaves) void newton_jac_kernel(
     sbi_newton_desc d, const double* __restrict__ x, int64_t P, int64_t ldx,
     const double* __restrict__ S, int64_t lds, int n, const double* __restrict__ gtab,
-    int64_t ldt, double* __restrict__ J) {
+    int64_t ldt, double* __restrict__ J, double* __restrict__ part) {
   extern __shared__ double lds_raw[];
   typedef typename lds_t<double>::type LT;
   const int lane = threadIdx.x & 63;
@@ -285,68 +311,84 @@ __global__ __launch_bounds__(kTile* kBWaves) void newton_jac_kernel(
     build_h_table<double>(d, xp, H.lds, nullptr, wave, lane, kBWaves);
     __syncthreads();
   }
-  cptr<int32_t> tdim = C(d.tab_dim), trow = C(d.tab_row), tk = C(d.tab_kmax),
-                slot_off = C(d.vjp_slot_off);
-  cptr<double> dim_xi = C(d.dim_xi), dim_c = C(d.dim_c);
+  cptr<int32_t> tdim = C(d.tab_dim), trow = C(d.tab_row),
+                unit = C(d.jac_unit);
   const uint4* __restrict__ ent = reinterpret_cast<const uint4*>(d.vjp_ent);
   const int nw = (int)gridDim.y * kBWaves;
-  for (int i = (int)blockIdx.y * kBWaves + wave; i < d.n_tab_dims; i += nw) {
-    const int j = tdim[i], row = trow[i], km = tk[i];
-    const double xj = xp[j];
-    cptr<double> xi = dim_xi + (int64_t)j * d.kmax1;
-    cptr<double> cj = dim_c + (int64_t)j * d.kmax1;
+  for (int u = (int)blockIdx.y * kBWaves + wave; u < d.n_jac_units; u += nw) {
+    const int ti = unit[4 * u], e0 = unit[4 * u + 1], e1 = unit[4 * u + 2],
+              slot = unit[4 * u + 3];
+    const int j = tdim[ti], row0 = trow[ti];
+    int s_done = -1;
+    double dh = 0.0;
     double g[NO];
 #pragma unroll
     for (int o = 0; o < NO; ++o) g[o] = 0.0;
-    double pr = 1.0, dpr = 0.0;  // prod_{r<k} (x - xi_r) and its derivative
-    for (int k = 1; k <= km; ++k) {
-      const double dx = xj - xi[k - 1];
-      dpr = dpr * dx + pr;
-      pr *= dx;
-      const int s = row + k - 1;
-      double A[NO];
-#pragma unroll
-      for (int o = 0; o < NO; ++o) A[o] = 0.0;
-      const int e0 = slot_off[s], e1 = slot_off[s + 1];
-      for (int base = e0; base < e1; base += 64) {
-        const int mine = base + lane;
-        const uint4 ev = (mine < e1) ? ent[mine]
-                                     : make_uint4(0u, 0xffffffffu, 0xffffffffu, 0xffffffffu);
-        const int cnt = min(64, e1 - base);
-        for (int q = 0; q < cnt; ++q) {
-          const int node = __builtin_amdgcn_readlane((int)ev.x, q);
-          const uint32_t ws[3] = {(uint32_t)__builtin_amdgcn_readlane((int)ev.y, q),
-                                  (uint32_t)__builtin_amdgcn_readlane((int)ev.z, q),
-                                  (uint32_t)__builtin_amdgcn_readlane((int)ev.w, q)};
-          double v = 1.0;
-#pragma unroll
-          for (int f = 0; f < 6; ++f) {
-            const uint32_t r = (ws[f >> 1] >> ((f & 1) * 16)) & 0xffff;
-            if (r != 0xffff) v *= H.at((int)r, lane, p);
-          }
-          cptr<double> Sn = C(S) + (int64_t)node * lds;
-#pragma unroll
-          for (int o = 0; o < NO; ++o)
-            if (o < n) A[o] = fma(v, Sn[o], A[o]);
+    for (int base = e0; base < e1; base += 64) {
+      const int mine = base + lane;
+      const uint4 ev = (mine < e1) ? ent[mine]
+                                   : make_uint4(0u, 0xffffffffu, 0xffffffffu, 0xffffffffu);
+      const int rl = row_of_entry(d.vjp_slot_off, d.n_tab_rows, min(mine, e1 - 1));
+      const int cnt = min(64, e1 - base);
+      for (int q = 0; q < cnt; ++q) {
+        const int s = __builtin_amdgcn_readlane(rl, q);
+        if (s != s_done) {  // next factor row of the dim
+          dh = row_derivative(d, xp, j, s - row0 + 1);
+          s_done = s;
         }
-      }
-      const double dh = dpr * cj[k];
+        const int node = __builtin_amdgcn_readlane((int)ev.x, q);
+        const uint32_t ws[3] = {(uint32_t)__builtin_amdgcn_readlane((int)ev.y, q),
+                                (uint32_t)__builtin_amdgcn_readlane((int)ev.z, q),
+                                (uint32_t)__builtin_amdgcn_readlane((int)ev.w, q)};
+        double v = dh;
 #pragma unroll
-      for (int o = 0; o < NO; ++o) g[o] = fma(dh, A[o], g[o]);
+        for (int f = 0; f < 6; ++f) {
+          const uint32_t r = (ws[f >> 1] >> ((f & 1) * 16)) & 0xffff;
+          if (r != 0xffff) v *= H.at((int)r, lane, p);
+        }
+        cptr<double> Sn = C(S) + (int64_t)node * lds;
+#pragma unroll
+        for (int o = 0; o < NO; ++o)
+          if (o < n) g[o] = fma(v, Sn[o], g[o]);
+      }
     }
     if (valid) {
+      if (slot < 0) {
 #pragma unroll
-      for (int o = 0; o < NO; ++o)
-        if (o < n) J[(p * n + o) * d.d_in + j] = g[o];
+        for (int o = 0; o < NO; ++o)
+          if (o < n) J[(p * n + o) * d.d_in + j] = g[o];
+      } else {
+#pragma unroll
+        for (int o = 0; o < NO; ++o)
+          if (o < n) part[((int64_t)slot * P + p) * n + o] = g[o];
+      }
     }
   }
 }
 
-// D^T[e, p] for all adjoint entries, in groups of 64 entries per wave (the groups are spread over
-// grid.y workgroups per tile).  Entries are stored by factor row; the derivative h'_k(x_j) of the
-// current row is recomputed when the row changes.
+// J[p, o, j] = sum of the partial sums of dim j's units, in unit order.
+__global__ __launch_bounds__(256) void jac_reduce_kernel(sbi_newton_desc d, int64_t P, int64_t n,
+                                                        const double* __restrict__ part,
+                                                        double* __restrict__ J) {
+  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= P) return;
+  cptr<int32_t> red = C(d.jac_red);
+  const int r = blockIdx.y;
+  const int j = red[3 * r], s0 = red[3 * r + 1], ns = red[3 * r + 2];
+  for (int64_t o = 0; o < n; ++o) {
+    double acc = 0.0;
+    for (int u = 0; u < ns; ++u) acc += part[((int64_t)(s0 + u) * P + p) * n + o];
+    J[(p * n + o) * d.d_in + j] = acc;
+  }
+}
+
+// D^T[e, p] for all adjoint entries, in groups of 64 entries per wave; the workgroups of a tile
+// (grid.y of them, kDTWaves waves each) share out the groups.  The factor row of each entry and
+// its (dim, deg) are found lane-parallel per group; h'_k(x_j) is recomputed when the row changes.
+constexpr int kDTWaves = 8;
+
 template <bool kG>
-__global__ __launch_bounds__(kTile* kBWaves) void newton_jac_basis_kernel(
+__global__ __launch_bounds__(kTile* kDTWaves) void newton_jac_basis_kernel(
     sbi_newton_desc d, const double* __restrict__ x, int64_t P, int64_t ldx,
     const double* __restrict__ gtab, int64_t ldt, double* __restrict__ DT, int64_t ldd) {
   extern __shared__ double lds_raw[];
@@ -359,50 +401,30 @@ __global__ __launch_bounds__(kTile* kBWaves) void newton_jac_basis_kernel(
   const double* __restrict__ xp = x + p * ldx;
   HTab<kG> H{(LT*)lds_raw, gtab, ldt};
   if constexpr (!kG) {
-    build_h_table<double>(d, xp, H.lds, nullptr, wave, lane, kBWaves);
+    build_h_table<double>(d, xp, H.lds, nullptr, wave, lane, kDTWaves);
     __syncthreads();
   }
-  cptr<int32_t> tdim = C(d.tab_dim), trow = C(d.tab_row), slot_off = C(d.vjp_slot_off);
-  cptr<double> dim_xi = C(d.dim_xi), dim_c = C(d.dim_c);
+  cptr<int32_t> tdim = C(d.tab_dim);
   const uint4* __restrict__ ent = reinterpret_cast<const uint4*>(d.vjp_ent);
   const int n_ent = d.n_vjp_ent;
   const int ngrp = (n_ent + 63) / 64;
-  const int nw = (int)gridDim.y * kBWaves;
-  for (int gi = (int)blockIdx.y * kBWaves + wave; gi < ngrp; gi += nw) {
+  const int nw = (int)gridDim.y * kDTWaves;
+  for (int gi = (int)blockIdx.y * kDTWaves + wave; gi < ngrp; gi += nw) {
     const int base = gi * 64;
-    // factor row of entry `base`: the last s with slot_off[s] <= base
-    int lo = 0, hi = d.n_tab_rows;
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (slot_off[mid] <= base) lo = mid;
-      else hi = mid;
-    }
-    int s = lo, s_done = -1;
-    double dh = 0.0;
-    const int mine = base + lane;
-    const uint4 ev = (mine < n_ent) ? ent[mine]
-                                    : make_uint4(0u, 0xffffffffu, 0xffffffffu, 0xffffffffu);
+    const int mine = min(base + lane, n_ent - 1);
+    const uint4 ev = ent[mine];
+    const int rl = row_of_entry(d.vjp_slot_off, d.n_tab_rows, mine);
+    const int il = row_of_entry(d.tab_row, d.n_tab_dims, rl);  // table dim of the row
+    const int kl = rl - d.tab_row[il] + 1;
     const int cnt = min(64, n_ent - base);
+    int s_done = -1;
+    double dh = 0.0;
     for (int q = 0; q < cnt; ++q) {
       const int e = base + q;
-      while (s + 1 < d.n_tab_rows && slot_off[s + 1] <= e) ++s;
-      if (s != s_done) {  // h'_k(x_j) of row s = (j, k): the dim is the last i with trow[i] <= s
-        int a = 0, b = d.n_tab_dims;
-        while (b - a > 1) {
-          const int mid = (a + b) >> 1;
-          if (trow[mid] <= s) a = mid;
-          else b = mid;
-        }
-        const int j = tdim[a], k = s - trow[a] + 1;
-        const double xj = xp[j];
-        cptr<double> xi = dim_xi + (int64_t)j * d.kmax1;
-        double pr = 1.0, dpr = 0.0;
-        for (int r = 0; r < k; ++r) {
-          const double dx = xj - xi[r];
-          dpr = dpr * dx + pr;
-          pr *= dx;
-        }
-        dh = dpr * dim_c[(int64_t)j * d.kmax1 + k];
+      const int s = __builtin_amdgcn_readlane(rl, q);
+      if (s != s_done) {
+        const int i = __builtin_amdgcn_readlane(il, q), k = __builtin_amdgcn_readlane(kl, q);
+        dh = row_derivative(d, xp, tdim[i], k);
         s_done = s;
       }
       const uint32_t ws[3] = {(uint32_t)__builtin_amdgcn_readlane((int)ev.y, q),
@@ -690,13 +712,14 @@ extern "C" int64_t sbi_newton_jacobian_workspace(const sbi_newton_plan* plan, in
   int64_t bytes = 0;
   if (jac_global_table(*dp)) bytes += (int64_t)dp->n_tab_rows * P * (int64_t)sizeof(double);
   if (n > kJacDirectMax) bytes += (int64_t)dp->n_vjp_ent * P * (int64_t)sizeof(double);
+  else bytes += (int64_t)dp->n_jac_partials * P * n * (int64_t)sizeof(double);
   return bytes;
 }
 
 template <int NO, bool kG>
 static void jac_direct_launch(const sbi_newton_desc& d, const double* x, int64_t P, int64_t ldx,
                               const double* S, int64_t lds, int n, const double* gtab,
-                              double* J, hipStream_t st) {
+                              double* J, double* part, hipStream_t st) {
   const size_t lds_bytes = kG ? 0 : (size_t)d.n_tab_rows * kTile * sizeof(double);
   static bool attr = false;
   if (!attr) {
@@ -705,11 +728,12 @@ static void jac_direct_launch(const sbi_newton_desc& d, const double* x, int64_t
     attr = true;
   }
   const int64_t tiles = (P + kTile - 1) / kTile;
-  // workgroups per tile: enough for ~2048 workgroups, at most one wave per table dim
+  // workgroups per tile: enough for ~4096 workgroups, at most one wave per work unit
   const int64_t ny = std::max<int64_t>(
-      1, std::min<int64_t>((d.n_tab_dims + kBWaves - 1) / kBWaves, (2048 + tiles - 1) / tiles));
+      1, std::min<int64_t>((d.n_jac_units + kBWaves - 1) / kBWaves, (4096 + tiles - 1) / tiles));
   hipLaunchKernelGGL((newton_jac_kernel<NO, kG>), dim3((unsigned)tiles, (unsigned)ny),
-                     dim3(kTile * kBWaves), lds_bytes, st, d, x, P, ldx, S, lds, n, gtab, P, J);
+                     dim3(kTile * kBWaves), lds_bytes, st, d, x, P, ldx, S, lds, n, gtab, P, J,
+                     part);
 }
 
 template <bool kG>
@@ -724,10 +748,11 @@ static void jac_basis_launch(const sbi_newton_desc& d, const double* x, int64_t 
   }
   const int64_t tiles = (P + kTile - 1) / kTile;
   const int64_t ngrp = (d.n_vjp_ent + 63) / 64;
+  // the factor table is built once per workgroup: ~1024 workgroups, each wave looping over groups
   const int64_t ny = std::max<int64_t>(
-      1, std::min<int64_t>((ngrp + kBWaves - 1) / kBWaves, (4096 + tiles - 1) / tiles));
+      1, std::min<int64_t>((ngrp + kDTWaves - 1) / kDTWaves, (1024 + tiles - 1) / tiles));
   hipLaunchKernelGGL(newton_jac_basis_kernel<kG>, dim3((unsigned)tiles, (unsigned)ny),
-                     dim3(kTile * kBWaves), lds_bytes, st, d, x, P, ldx, gtab, P, DT, P);
+                     dim3(kTile * kDTWaves), lds_bytes, st, d, x, P, ldx, gtab, P, DT, P);
 }
 
 extern "C" int sbi_newton_jacobian_f64(const sbi_newton_plan* plan, const double* x, int64_t P,
@@ -742,7 +767,7 @@ extern "C" int sbi_newton_jacobian_f64(const sbi_newton_plan* plan, const double
   }
   const sbi_newton_desc& d = *dp;
   if (!d.vjp_ent || !d.vjp_slot_off || !d.tab_dim || !d.vjp_dim_off || !d.sing_off ||
-      !d.sing_x || ldx < d.d_in) {
+      !d.sing_x || !d.jac_unit || !d.jac_red || ldx < d.d_in) {
     sbi::set_error("sbi_newton_jacobian_f64: plan has no Jacobian tables");
     return SBI_ERR_INVALID;
   }
@@ -775,18 +800,24 @@ extern "C" int sbi_newton_jacobian_f64(const sbi_newton_plan* plan, const double
       SBI_HIP_CHECK(hipGetLastError());
     }
     const int ni = (int)n;
-    if (d.n_tab_dims > 0) {
+    double* part = DT;  // partial sums of split dims: [n_jac_partials][P][n]
+    if (d.n_jac_units > 0) {
       if (g) {
-        if (n == 1) jac_direct_launch<1, true>(d, x, P, ldx, S, lds, ni, gtab, J, st);
-        else if (n <= 2) jac_direct_launch<2, true>(d, x, P, ldx, S, lds, ni, gtab, J, st);
-        else if (n <= 4) jac_direct_launch<4, true>(d, x, P, ldx, S, lds, ni, gtab, J, st);
-        else jac_direct_launch<8, true>(d, x, P, ldx, S, lds, ni, gtab, J, st);
+        if (n == 1) jac_direct_launch<1, true>(d, x, P, ldx, S, lds, ni, gtab, J, part, st);
+        else if (n <= 2) jac_direct_launch<2, true>(d, x, P, ldx, S, lds, ni, gtab, J, part, st);
+        else if (n <= 4) jac_direct_launch<4, true>(d, x, P, ldx, S, lds, ni, gtab, J, part, st);
+        else jac_direct_launch<8, true>(d, x, P, ldx, S, lds, ni, gtab, J, part, st);
       } else {
-        if (n == 1) jac_direct_launch<1, false>(d, x, P, ldx, S, lds, ni, gtab, J, st);
-        else if (n <= 2) jac_direct_launch<2, false>(d, x, P, ldx, S, lds, ni, gtab, J, st);
-        else if (n <= 4) jac_direct_launch<4, false>(d, x, P, ldx, S, lds, ni, gtab, J, st);
-        else jac_direct_launch<8, false>(d, x, P, ldx, S, lds, ni, gtab, J, st);
+        if (n == 1) jac_direct_launch<1, false>(d, x, P, ldx, S, lds, ni, gtab, J, part, st);
+        else if (n <= 2) jac_direct_launch<2, false>(d, x, P, ldx, S, lds, ni, gtab, J, part, st);
+        else if (n <= 4) jac_direct_launch<4, false>(d, x, P, ldx, S, lds, ni, gtab, J, part, st);
+        else jac_direct_launch<8, false>(d, x, P, ldx, S, lds, ni, gtab, J, part, st);
       }
+      SBI_HIP_CHECK(hipGetLastError());
+    }
+    if (d.n_jac_red > 0) {
+      hipLaunchKernelGGL(jac_reduce_kernel, dim3((unsigned)((P + 255) / 256), (unsigned)d.n_jac_red),
+                         dim3(256), 0, st, d, P, n, part, J);
       SBI_HIP_CHECK(hipGetLastError());
     }
   } else {
@@ -795,11 +826,10 @@ extern "C" int sbi_newton_jacobian_f64(const sbi_newton_plan* plan, const double
       else jac_basis_launch<false>(d, x, P, ldx, gtab, DT, st);
       SBI_HIP_CHECK(hipGetLastError());
     }
-    // one group of D^T rows per input dim (dims without entries get zeros), B rows gathered by
-    // the entries' node index (first int32 of each 16-byte entry)
-    const int rc = sbi::contract_grouped_f64(DT, P, S, lds, (const int32_t*)d.vjp_ent, 4,
-                                             d.vjp_dim_off, (int)d.d_in, J, n * d.d_in, d.d_in,
-                                             1, P, n, st);
+    // one group of D^T rows per input dim (dims without entries get zeros); the surplus row of
+    // an entry is its node, the first int32 of its 16-byte record
+    const int rc = sbi::contract_jacobian_f64(DT, P, S, lds, (const int32_t*)d.vjp_ent, 4,
+                                              d.vjp_dim_off, (int)d.d_in, J, P, n, st);
     if (rc != SBI_OK) return rc;
   }
   const int64_t tot = P * d.d_in;

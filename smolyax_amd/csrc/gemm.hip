@@ -65,41 +65,12 @@ __device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
 // contracted first with the FP64 MFMA, straight into the FP64 accumulators.  Lane li supplies the
 // A row of point 4*(li & 3) + (li >> 2) of its 16-point block, which puts the FP64 MFMA's result
 // (row g + 4r in register r of lane group g) into the FP32 layout (row 4g + r).
-//
-// Grouped form (kGrp): launch slice z = blockIdx.z contracts rows [zoff[z], zoff[z+1]) of A with
-// the B rows brow[k * brs] (a gather), into Y + z * yzs with column stride ycs.
-struct GroupArgs {
-  const int32_t* zoff;
-  const int32_t* brow;
-  int64_t brs;
-  int64_t ycs;
-  int64_t yzs;
-};
-
-template <typename T, int FN, bool kGrp = false>
+template <typename T, int FN>
 __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
     const T* __restrict__ A, int64_t lda, const T* __restrict__ B, int64_t ldb,
     const double* __restrict__ offset, double* __restrict__ Y, int64_t ldy, int64_t P, int64_t n,
     int64_t K, const double* __restrict__ A64, int64_t lda64, const double* __restrict__ B64,
-    int64_t ldb64, int64_t K64, GroupArgs ga) {
-  const int32_t* __restrict__ brow = nullptr;
-  int64_t ycs = 1;
-  if constexpr (kGrp) {
-    const int z = blockIdx.z;
-    const int64_t k0 = ga.zoff[z];
-    K = ga.zoff[z + 1] - k0;
-    A += k0 * lda;
-    brow = ga.brow + k0 * ga.brs;
-    Y += (int64_t)z * ga.yzs;
-    ycs = ga.ycs;
-  }
-  auto brow_of = [&](int64_t k) -> int64_t {
-    if constexpr (kGrp) {
-      return brow[k * ga.brs];
-    } else {
-      return k;
-    }
-  };
+    int64_t ldb64, int64_t K64) {
   using AccT = typename Acc<T>::type;
   constexpr bool kF32 = sizeof(T) == 4;
   const int lane = threadIdx.x & 63;
@@ -173,7 +144,7 @@ __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
     for (int s = 0; s < 4; ++s) {
       const int64_t k = k0 + 4 * g + s;
       const T* __restrict__ ar = A + k * lda;
-      const T* __restrict__ br = B + brow_of(k) * ldb;
+      const T* __restrict__ br = B + k * ldb;
 #pragma unroll
       for (int fm = 0; fm < 2; ++fm) a[fm][s] = ar[pcol[fm]];
 #pragma unroll
@@ -234,7 +205,7 @@ __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
       }
 #pragma unroll
       for (int fn = 0; fn < FN; ++fn) {
-        const T v = B[brow_of(kc) * ldb + jcol[fn]];
+        const T v = B[kc * ldb + jcol[fn]];
         b[fn][s] = in ? v : T(0);
       }
     }
@@ -255,7 +226,7 @@ __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
         // 16x16x4 holds row g + 4r (column = lane % 16 in both)
         const int64_t row = m_base + fm * 16 + (kF32 ? 4 * g + r : g + 4 * r);
         const double v = kF32 ? acc64[fm][fn][r] : (double)acc[fm][fn][r];
-        if (row < P) Y[row * ldy + col * ycs] = off + v;
+        if (row < P) Y[row * ldy + col] = off + v;
       }
     }
 }
@@ -268,48 +239,155 @@ int contract_launch(const T* A, int64_t lda, const T* B, int64_t ldb, const doub
   const unsigned gm = (unsigned)((P + kBlockM - 1) / kBlockM);
   if (n <= 32) {
     hipLaunchKernelGGL((contract_kernel<T, 1>), dim3(gm, (unsigned)((n + 31) / 32)), block, 0, st,
-                       A, lda, B, ldb, offset, Y, ldy, P, n, K, A64, lda64, B64, ldb64, K64, GroupArgs{});
+                       A, lda, B, ldb, offset, Y, ldy, P, n, K, A64, lda64, B64, ldb64, K64);
   } else if (n <= 64) {
     hipLaunchKernelGGL((contract_kernel<T, 2>), dim3(gm, (unsigned)((n + 63) / 64)), block, 0, st,
-                       A, lda, B, ldb, offset, Y, ldy, P, n, K, A64, lda64, B64, ldb64, K64, GroupArgs{});
+                       A, lda, B, ldb, offset, Y, ldy, P, n, K, A64, lda64, B64, ldb64, K64);
   } else {
     hipLaunchKernelGGL((contract_kernel<T, 4>), dim3(gm, (unsigned)((n + 127) / 128)), block, 0,
-                       st, A, lda, B, ldb, offset, Y, ldy, P, n, K, A64, lda64, B64, ldb64, K64, GroupArgs{});
+                       st, A, lda, B, ldb, offset, Y, ldy, P, n, K, A64, lda64, B64, ldb64, K64);
   }
   SBI_HIP_CHECK(hipGetLastError());
   return SBI_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// Jacobian contraction: for each input dim z, J[p, c, z] = sum_{k in [zoff[z], zoff[z+1])}
+// A[k, p] B[brow[k * brs], c] (B rows gathered per row of A).  A workgroup owns a
+// 64-point x 32-output tile for kJG consecutive dims, streams their rows [zoff[z0], zoff[z0+kJG])
+// as one pipelined sequence of 16-row chunks and keeps kJG accumulator tiles in registers: a
+// chunk that spans a dim boundary is applied once per dim it touches, with the other rows masked
+// to zero.  Each lane finally writes kJG consecutive doubles J[p, c, z0 .. z0 + kJG) (one 64-byte
+// run) instead of kJG scattered words.
+// Workgroup order: the n-tiles of one (point tile, dim group) go to the same XCD (consecutive
+// workgroups of one XCD), so its L2 serves their shared A rows; consecutive (point tile, dim
+// group) pairs go to different XCDs, which spreads the heavy low dims.
+// ------------------------------------------------------------------------------------------
+constexpr int kJG = 8;
+
+__global__ __launch_bounds__(64 * kWaves) void jac_contract_kernel(
+    const double* __restrict__ A, int64_t lda, const double* __restrict__ B, int64_t ldb,
+    const int32_t* __restrict__ brow, int64_t brs, const int32_t* __restrict__ zoff, int nz,
+    double* __restrict__ J, int64_t P, int64_t n, int nt, int ntp, int npairs) {
+  const int L = blockIdx.x;
+  const int xcd = L & 7, idx = L >> 3;
+  const int tn = idx % nt;
+  const int pair = (idx / nt) * 8 + xcd;
+  if (pair >= npairs) return;
+  const int tp = pair % ntp, zg = pair / ntp;
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int li = lane & 15, g = lane >> 4;
+  const int64_t m_base = (int64_t)tp * kBlockM + wm * 32;
+  const int64_t n_base = (int64_t)tn * 32 + wn * 16;
+  int64_t pcol[2];
+#pragma unroll
+  for (int fm = 0; fm < 2; ++fm) pcol[fm] = min(m_base + fm * 16 + li, P - 1);
+  const int64_t jcol = min(n_base + li, n - 1);
+
+  const int z0 = zg * kJG;
+  int zb[kJG + 1];  // row boundaries of the group's dims (uniform)
+#pragma unroll
+  for (int jj = 0; jj <= kJG; ++jj) zb[jj] = zoff[min(z0 + jj, nz)];
+  const int64_t R0 = zb[0], R1 = zb[kJG];
+
+  v4d acc[kJG][2];
+#pragma unroll
+  for (int jj = 0; jj < kJG; ++jj) acc[jj][0] = acc[jj][1] = v4d{0, 0, 0, 0};
+
+  // rows k0 + 4g + s of a chunk, clamped into the stream (the extra rows are masked); the B row
+  // indices are read one chunk ahead of the B rows themselves
+  double a[2][4], b[4];
+  int bi[4];
+  auto load_idx = [&](int64_t k0) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) bi[s] = brow[min(k0 + 4 * g + s, R1 - 1) * brs];
+  };
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int64_t k = min(k0 + 4 * g + s, R1 - 1);
+      const double* __restrict__ ar = A + k * lda;
+#pragma unroll
+      for (int fm = 0; fm < 2; ++fm) a[fm][s] = ar[pcol[fm]];
+      b[s] = B[(int64_t)bi[s] * ldb + jcol];
+    }
+  };
+  if (R1 > R0) {
+    load_idx(R0);
+    load(R0);
+    if (R0 + kChunk < R1) load_idx(R0 + kChunk);
+  }
+  for (int64_t k0 = R0; k0 < R1; k0 += kChunk) {
+    double ca[2][4], cb[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      ca[0][s] = a[0][s];
+      ca[1][s] = a[1][s];
+      cb[s] = b[s];
+    }
+    if (k0 + kChunk < R1) {  // next chunk in flight during this one
+      load(k0 + kChunk);
+      if (k0 + 2 * kChunk < R1) load_idx(k0 + 2 * kChunk);
+    }
+    const int64_t klast = min(k0 + kChunk, R1) - 1;
+    // dims touched by this chunk: [first row's dim, last row's dim] (uniform)
+#pragma unroll
+    for (int jj = 0; jj < kJG; ++jj) {
+      if (zb[jj + 1] <= k0 || zb[jj] > klast || zb[jj] == zb[jj + 1]) continue;
+      if (zb[jj] <= k0 && zb[jj + 1] >= k0 + kChunk) {  // all 16 rows belong to this dim
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+          for (int fm = 0; fm < 2; ++fm) acc[jj][fm] = mfma(ca[fm][s], cb[s], acc[jj][fm]);
+      } else {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          const int64_t k = k0 + 4 * g + s;
+          const bool in = k >= zb[jj] && k < zb[jj + 1];
+          const double bv = in ? cb[s] : 0.0;
+#pragma unroll
+          for (int fm = 0; fm < 2; ++fm) acc[jj][fm] = mfma(ca[fm][s], bv, acc[jj][fm]);
+        }
+      }
+    }
+  }
+
+  // FP64 16x16x4 result: row (point) g + 4r in register r, column (output) = lane % 16
+  const int64_t col = n_base + li;
+  if (col >= n) return;
+#pragma unroll
+  for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t row = m_base + fm * 16 + g + 4 * r;
+      if (row >= P) continue;
+      double* __restrict__ out = J + (row * n + col) * nz + z0;
+#pragma unroll
+      for (int jj = 0; jj < kJG; ++jj)
+        if (z0 + jj < nz) out[jj] = acc[jj][fm][r];
+    }
 }
 
 }  // namespace
 
 namespace sbi {
 
-int contract_grouped_f64(const double* A, int64_t lda, const double* B, int64_t ldb,
-                         const int32_t* brow, int64_t brs, const int32_t* zoff, int nz, double* Y,
-                         int64_t ldy, int64_t ycs, int64_t yzs, int64_t P, int64_t n,
-                         hipStream_t st) {
+int contract_jacobian_f64(const double* A, int64_t lda, const double* B, int64_t ldb,
+                          const int32_t* brow, int64_t brs, const int32_t* zoff, int nz, double* J,
+                          int64_t P, int64_t n, hipStream_t st) {
   if (P <= 0 || n <= 0 || nz <= 0) return SBI_OK;
-  if (P > (int64_t)65535 * kBlockM || n > (int64_t)65535 * 32 || nz > 65535) {
-    set_error("contract_grouped_f64: launch too large");
+  const int64_t nt = (n + 31) / 32, ntp = (P + kBlockM - 1) / kBlockM, nzg = (nz + kJG - 1) / kJG;
+  const int64_t npairs = ntp * nzg;
+  const int64_t grid = nt * ((npairs + 7) / 8) * 8;
+  if (grid > 0x7fffffff || npairs > 0x7fffffff) {
+    set_error("contract_jacobian_f64: launch too large");
     return SBI_ERR_INVALID;
   }
-  const dim3 block(64 * kWaves);
-  const unsigned gm = (unsigned)((P + kBlockM - 1) / kBlockM);
-  const GroupArgs ga{zoff, brow, brs, ycs, yzs};
-  if (n <= 32) {
-    hipLaunchKernelGGL((contract_kernel<double, 1, true>), dim3(gm, (unsigned)((n + 31) / 32), nz),
-                       block, 0, st, A, lda, B, ldb, nullptr, Y, ldy, P, n, (int64_t)0, nullptr,
-                       (int64_t)0, nullptr, (int64_t)0, (int64_t)0, ga);
-  } else if (n <= 64) {
-    hipLaunchKernelGGL((contract_kernel<double, 2, true>), dim3(gm, (unsigned)((n + 63) / 64), nz),
-                       block, 0, st, A, lda, B, ldb, nullptr, Y, ldy, P, n, (int64_t)0, nullptr,
-                       (int64_t)0, nullptr, (int64_t)0, (int64_t)0, ga);
-  } else {
-    hipLaunchKernelGGL((contract_kernel<double, 4, true>),
-                       dim3(gm, (unsigned)((n + 127) / 128), nz), block, 0, st, A, lda, B, ldb,
-                       nullptr, Y, ldy, P, n, (int64_t)0, nullptr, (int64_t)0, nullptr, (int64_t)0,
-                       (int64_t)0, ga);
-  }
+  hipLaunchKernelGGL(jac_contract_kernel, dim3((unsigned)grid), dim3(64 * kWaves), 0, st, A, lda,
+                     B, ldb, brow, brs, zoff, nz, J, P, n, (int)nt, (int)ntp, (int)npairs);
   SBI_HIP_CHECK(hipGetLastError());
   return SBI_OK;
 }

@@ -249,6 +249,8 @@ class SmolyakBarycentricInterpolator:
             "vjp_slot_off": _dev_tensor(NL.vjp_slot_off, i32, dev),
             "vjp_ent": torch.as_tensor(NL.vjp_ent.copy()).to(dev),
             "vjp_dim_off": _dev_tensor(NL.vjp_dim_off, i32, dev),
+            "jac_unit": _dev_tensor(NL.jac_unit, i32, dev),
+            "jac_red": _dev_tensor(NL.jac_red, i32, dev),
         }
         t["sing_off"], t["sing_x"] = (_dev_tensor(a, i32 if a.dtype == np.int32 else f64, dev)
                                       for a in self._singular_coordinates())
@@ -267,6 +269,7 @@ class SmolyakBarycentricInterpolator:
         d.n_nodes_pad, d.n_hs_dims, d.n_hs_rows = NL.n_nodes_pad, NL.n_hs_dims, NL.n_hs_rows
         d.basis_n, d.n_tab_dims, d.n_tab_rows = NL.basis_n, NL.n_tab_dims, NL.n_tab_rows
         d.n_vjp_ent = NL.n_vjp_ent
+        d.n_jac_units, d.n_jac_partials, d.n_jac_red = NL.n_jac_units, NL.n_jac_partials, NL.n_jac_red
         for name, ten in t.items():
             setattr(d, name, ten.data_ptr())
         self._ndesc = d

@@ -315,6 +315,9 @@ def head_order(nf: NewtonForm, L: NewtonLayout, n_head: int = HEAD_ROWS):
     return order, n_head, fac.reshape(-1)
 
 
+JAC_UNIT = 128  # adjoint entries per work unit of the Jacobian's register pass
+
+
 def build_basis_tables(nf: NewtonForm, L: NewtonLayout) -> None:
     """Descriptors of the basis matrix H^T (rows in nf.nus order) and of the adjoint."""
     tab_dims = [j for j in range(nf.d_in) if nf.maxdeg[j] > 0]
@@ -358,6 +361,22 @@ def build_basis_tables(nf: NewtonForm, L: NewtonLayout) -> None:
     for j in tab_dims:
         dim_off[j + 1] = off[row0[j] + int(nf.maxdeg[j])] - off[row0[j]]
     L.vjp_dim_off = np.cumsum(dim_off).astype(np.int32)
+    # work units of the Jacobian's register pass: runs of at most JAC_UNIT entries of one dim;
+    # a dim split into several units sums their partial results afterwards (fixed order)
+    units, red, n_part = [], [], 0
+    for i, j in enumerate(tab_dims):
+        e0, e1 = int(L.vjp_dim_off[j]), int(L.vjp_dim_off[j + 1])
+        cuts = list(range(e0, e1, JAC_UNIT)) + [e1]
+        if len(cuts) <= 2:
+            units.append((i, e0, e1, -1))
+        else:
+            red.append((j, n_part, len(cuts) - 1))
+            for a, b in zip(cuts[:-1], cuts[1:]):
+                units.append((i, a, b, n_part))
+                n_part += 1
+    L.n_jac_units, L.n_jac_partials, L.n_jac_red = len(units), n_part, len(red)
+    L.jac_unit = np.array(units or [(0, 0, 0, -1)], dtype=np.int32).reshape(-1)
+    L.jac_red = np.array(red or [(0, 0, 0)], dtype=np.int32).reshape(-1)
     L.vjp_ent = (np.stack(ent) if ent else np.full((1, 8), 0xFFFF, dtype=np.uint16)).reshape(-1).view(np.uint8)
 
 

@@ -43,6 +43,21 @@ def jacobian_model(L, S, x):
     for j in range(d):
         e0, e1 = L.vjp_dim_off[j], L.vjp_dim_off[j + 1]
         J[:, :, j] = D[e0:e1].T @ S[node[e0:e1]]
+    # the register pass's work units: a unit covering its dim writes J, split dims add their
+    # partial sums in unit order
+    Ju = np.zeros_like(J)
+    part = {}
+    for u in L.jac_unit.reshape(-1, 4)[: L.n_jac_units]:
+        i, a, b, slot = (int(v) for v in u)
+        g = D[a:b].T @ S[node[a:b]]
+        if slot < 0:
+            Ju[:, :, L.tab_dim[i]] = g
+        else:
+            part[slot] = g
+    for j, s0, ns in L.jac_red.reshape(-1, 3)[: L.n_jac_red]:
+        Ju[:, :, j] = sum(part[s0 + q] for q in range(ns))
+    assert len(part) == L.n_jac_partials
+    assert np.max(np.abs(Ju - J)) <= 1e-13 * max(1.0, np.max(np.abs(J)))
     return J
 
 
@@ -64,8 +79,11 @@ def _pair(d, d_out, m, seed):
     return ref, nf, NW.build_newton_layout(nf), rng, k, t, sng
 
 
-@pytest.mark.parametrize("d,d_out,m", [(1, 1, 9), (4, 3, 60), (8, 2, 300), (12, 1, 500)])
+@pytest.mark.parametrize("d,d_out,m", [(1, 1, 9), (4, 3, 60), (8, 2, 300), (12, 1, 500),
+                                       (30, 2, 3000)])
 def test_newton_jacobian_tables_match_oracle(d, d_out, m):
+    from smolyax_amd import newton as NW
+
     ref, nf, L, rng, _, _, _ = _pair(d, d_out, m, seed=d)
     x = rng.uniform(-1, 1, (40, d))
     S = np.asarray(nf.s, dtype=np.float64).reshape(len(nf.nus), d_out)
@@ -76,6 +94,12 @@ def test_newton_jacobian_tables_match_oracle(d, d_out, m):
     assert L.vjp_dim_off[0] == 0 and L.vjp_dim_off[-1] == L.n_vjp_ent
     assert np.all(np.diff(L.vjp_dim_off) >= 0)
     assert L.n_vjp_ent == sum(len(nu) for nu in nf.nus)
+    units = L.jac_unit.reshape(-1, 4)[: L.n_jac_units]
+    assert np.all(units[:, 2] - units[:, 1] <= NW.JAC_UNIT)
+    assert np.array_equal(np.sort(np.concatenate([np.arange(a, b) for _, a, b, _ in units])),
+                          np.arange(L.n_vjp_ent))
+    if m >= 3000:
+        assert L.n_jac_red > 0  # the heavy dims are split
 
 
 def rel_ok(a, b, tol):
