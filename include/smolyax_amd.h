@@ -116,6 +116,8 @@ typedef struct sbi_bary_desc {
   const int32_t* dim_blk;        /* [d_in + 1] CSR: blocks containing each dim (jacobian) */
   const int32_t* dim_blk_idx;    /* block ids, and the factor position in dim_blk_pos */
   const int32_t* dim_blk_pos;    /*   (0..n-1, n-1 = last factor) */
+  const double* ent_q;           /* [k_raw] quadrature weight of every entry: the product of its
+                                    factors' slot_q values (integral; optional) */
 } sbi_bary_desc;
 
 typedef struct sbi_plan sbi_plan;
@@ -138,8 +140,10 @@ int sbi_bary_eval_f64(const sbi_plan* plan, const double* x, int64_t P, int64_t 
 int sbi_bary_jacobian_f64(const sbi_plan* plan, const double* x, int64_t P, int64_t ldx,
                           double* J, void* stream);
 
-/* interpolation.py:347-390 integral: Q[d_out] = offset + sum_blocks zeta <F_nu, (x) q>.
- * `partial` is caller-provided device workspace of sbi_bary_integral_workspace(plan) doubles. */
+/* interpolation.py:347-390 integral: Q[d_out] = offset + sum_blocks zeta <F_nu, (x) q>, computed
+ * as offset + ent_q^T F (zeta is folded into F; ent_q holds each entry's product of quadrature
+ * weights).  F is read once; sums run in a fixed order.  `partial` is caller-provided device
+ * workspace of sbi_bary_integral_workspace(plan) doubles. */
 int64_t sbi_bary_integral_workspace(const sbi_plan* plan);
 int sbi_bary_integral_f64(const sbi_plan* plan, double* Q, double* partial, void* stream);
 

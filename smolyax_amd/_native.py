@@ -38,7 +38,7 @@ class BaryDesc(ctypes.Structure):
         ("fac_deg", c_vp), ("fac_slot", c_vp), ("blk_last_slot", c_vp), ("blk_foff", c_vp),
         ("F", c_vp), ("offset", c_vp), ("blk_desc", c_vp), ("seg_off", c_vp), ("seg_slot", c_vp),
         ("seg_b0", c_vp), ("seg_b1", c_vp), ("n_split", c_i32), ("n_seg", c_i32),
-        ("dim_blk", c_vp), ("dim_blk_idx", c_vp), ("dim_blk_pos", c_vp),
+        ("dim_blk", c_vp), ("dim_blk_idx", c_vp), ("dim_blk_pos", c_vp), ("ent_q", c_vp),
     ]
 
 

@@ -443,53 +443,68 @@ __global__ __launch_bounds__(64) void bary_jacobian_kernel(sbi_bary_desc d,
 }
 
 // ------------------------------------------------------------------------------------------
-// Integral: partial[c][o] = sum over blocks of chunk c of <F_b[:, o], (x)q>, then a fixed-order
-// sum over chunks.  F is streamed once, coalesced across o.
+// Integral: Q[o] = offset[o] + sum_e ent_q[e] F[e, o], a GEMV over F (streamed once).
+// A workgroup of 256 threads = kW outputs x (256 / kW) entry lanes takes one chunk of entries for
+// one tile of kW outputs; the entry lanes' sums are added in LDS in a fixed tree order, the
+// chunks' partial sums in chunk order by the final kernel: the result does not depend on timing.
+// kW follows d_out (1 .. 64) so that small d_out still uses every lane and reads F coalesced.
 // ------------------------------------------------------------------------------------------
-constexpr int kIntChunks = 512;
 constexpr int kIntThreads = 256;
+constexpr int kIntMaxChunks = 1024;
 
+template <int kW>
 __global__ __launch_bounds__(kIntThreads) void bary_integral_partial_kernel(
-    sbi_bary_desc d, double* __restrict__ partial) {
+    const double* __restrict__ F, const double* __restrict__ q, int64_t K, int64_t d_out,
+    int64_t chunk, double* __restrict__ partial) {
+  constexpr int kE = kIntThreads / kW;  // entry lanes
+  __shared__ double red[kIntThreads];
+  const int tx = threadIdx.x % kW, ty = threadIdx.x / kW;
   const int c = blockIdx.x;
-  const int64_t o = (int64_t)blockIdx.y * kIntThreads + threadIdx.x;
-  const int nb = d.n_blocks;
-  const int b0 = (int)((int64_t)nb * c / kIntChunks);
-  const int b1 = (int)((int64_t)nb * (c + 1) / kIntChunks);
+  const int64_t o = (int64_t)blockIdx.y * kW + tx;
+  const int64_t e0 = (int64_t)c * chunk, e1 = min(K, e0 + chunk);
   double acc = 0.0;
-  for (int b = b0; b < b1; ++b) {
-    const int f0 = d.blk_fac[b];
-    const int npre = d.blk_fac[b + 1] - f0;
-    const int ls = d.blk_last_slot[b];
-    const int nf = npre + (ls >= 0 ? 1 : 0);
-    int64_t nent = 1;
-    for (int i = 0; i < npre; ++i) nent *= (d.fac_deg[f0 + i] + 1);
-    if (ls >= 0) nent *= (d.slot_deg[ls] + 1);
-    const double* __restrict__ Fb = d.F + d.blk_foff[b] * d.d_out;
-    for (int64_t e = 0; e < nent; ++e) {
-      int64_t rem = e;
-      double wgt = 1.0;
-      for (int i = nf - 1; i >= 0; --i) {
-        const int s = (i < npre) ? d.fac_slot[f0 + i] : ls;
-        const int di1 = d.slot_deg[s] + 1;
-        const int mi = (int)(rem % di1);
-        rem /= di1;
-        wgt *= d.slot_q[(int64_t)s * d.dmax1 + mi];
-      }
-      if (o < d.d_out) acc = fma(wgt, Fb[e * d.d_out + o], acc);
+  if (o < d_out) {
+    int64_t e = e0 + ty;
+    for (; e + 3 * kE < e1; e += 4 * kE) {  // four independent loads in flight
+      const double f0 = F[e * d_out + o], f1 = F[(e + kE) * d_out + o];
+      const double f2 = F[(e + 2 * kE) * d_out + o], f3 = F[(e + 3 * kE) * d_out + o];
+      acc = fma(q[e], f0, acc);
+      acc = fma(q[e + kE], f1, acc);
+      acc = fma(q[e + 2 * kE], f2, acc);
+      acc = fma(q[e + 3 * kE], f3, acc);
     }
+    for (; e < e1; e += kE) acc = fma(q[e], F[e * d_out + o], acc);
   }
-  if (o < d.d_out) partial[(int64_t)c * d.d_out + o] = acc;
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = kE / 2; h > 0; h >>= 1) {  // fixed-order tree over the entry lanes
+    if (ty < h) red[threadIdx.x] += red[threadIdx.x + h * kW];
+    __syncthreads();
+  }
+  if (ty == 0 && o < d_out) partial[(int64_t)c * d_out + o] = red[tx];
 }
 
+// Q[o] = offset[o] + sum over chunks: 16 outputs x 16 chunk lanes per workgroup, lane l adding
+// chunks l, l + 16, ... in order, then a fixed tree over the lanes.
+constexpr int kFinO = 16;
+
 __global__ __launch_bounds__(kIntThreads) void bary_integral_final_kernel(
-    int64_t d_out, const double* __restrict__ offset, const double* __restrict__ partial,
+    int64_t d_out, int nc, const double* __restrict__ offset, const double* __restrict__ partial,
     double* __restrict__ Q) {
-  const int64_t o = (int64_t)blockIdx.x * kIntThreads + threadIdx.x;
-  if (o >= d_out) return;
-  double s = offset[o];
-  for (int c = 0; c < kIntChunks; ++c) s += partial[(int64_t)c * d_out + o];
-  Q[o] = s;
+  constexpr int kL = kIntThreads / kFinO;
+  __shared__ double red[kIntThreads];
+  const int tx = threadIdx.x % kFinO, ty = threadIdx.x / kFinO;
+  const int64_t o = (int64_t)blockIdx.x * kFinO + tx;
+  double s = 0.0;
+  if (o < d_out)
+    for (int c = ty; c < nc; c += kL) s += partial[(int64_t)c * d_out + o];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = kL / 2; h > 0; h >>= 1) {
+    if (ty < h) red[threadIdx.x] += red[threadIdx.x + h * kFinO];
+    __syncthreads();
+  }
+  if (ty == 0 && o < d_out) Q[o] = offset[o] + red[tx];
 }
 
 }  // namespace
@@ -627,23 +642,52 @@ extern "C" int sbi_bary_jacobian_f64(const sbi_plan* plan, const double* x, int6
 
 extern "C" int64_t sbi_bary_integral_workspace(const sbi_plan* plan) {
   if (!plan) return -1;
-  return (int64_t)kIntChunks * plan->bary.d_out;
+  return (int64_t)kIntMaxChunks * plan->bary.d_out;
+}
+
+template <int kW>
+static void integral_launch(const sbi_bary_desc& d, int nc, int64_t chunk, double* partial,
+                            hipStream_t st) {
+  dim3 g((unsigned)nc, (unsigned)((d.d_out + kW - 1) / kW));
+  hipLaunchKernelGGL(bary_integral_partial_kernel<kW>, g, dim3(kIntThreads), 0, st, d.F, d.ent_q,
+                     d.k_raw, d.d_out, chunk, partial);
 }
 
 extern "C" int sbi_bary_integral_f64(const sbi_plan* plan, double* Q, double* partial,
                                      void* stream) {
-  if (!plan || plan->kind != 1 || !Q || !partial || !plan->bary.blk_last_slot || !plan->bary.slot_q) {
-    sbi::set_error("sbi_bary_integral_f64: invalid argument");
+  if (!plan || plan->kind != 1 || !Q || !partial || !plan->bary.ent_q) {
+    sbi::set_error("sbi_bary_integral_f64: invalid argument (or the plan has no ent_q)");
     return SBI_ERR_INVALID;
   }
   const sbi_bary_desc& d = plan->bary;
   hipStream_t st = (hipStream_t)stream;
-  dim3 g1(kIntChunks, (unsigned)((d.d_out + kIntThreads - 1) / kIntThreads));
-  hipLaunchKernelGGL(bary_integral_partial_kernel, g1, dim3(kIntThreads), 0, st, d, partial);
+  // output tile width: the smallest power of two >= d_out, at most 64 (>= 4 entry lanes)
+  int w = 1;
+  while (w < d.d_out && w < 64) w <<= 1;
+  const int64_t otiles = (d.d_out + w - 1) / w;
+  // entry chunks: ~2048 workgroups in all, each chunk at least 4 entries per entry lane
+  const int64_t lanes = kIntThreads / w;
+  int64_t nc = std::max<int64_t>(1, std::min<int64_t>(kIntMaxChunks, (2048 + otiles - 1) / otiles));
+  nc = std::max<int64_t>(1, std::min<int64_t>(nc, (d.k_raw + 4 * lanes - 1) / (4 * lanes)));
+  const int64_t chunk = (d.k_raw + nc - 1) / nc;
+  nc = std::max<int64_t>(1, (d.k_raw + chunk - 1) / std::max<int64_t>(chunk, 1));
+  if (otiles > 65535) {
+    sbi::set_error("sbi_bary_integral_f64: d_out too large for one launch");
+    return SBI_ERR_INVALID;
+  }
+  switch (w) {
+    case 1: integral_launch<1>(d, (int)nc, chunk, partial, st); break;
+    case 2: integral_launch<2>(d, (int)nc, chunk, partial, st); break;
+    case 4: integral_launch<4>(d, (int)nc, chunk, partial, st); break;
+    case 8: integral_launch<8>(d, (int)nc, chunk, partial, st); break;
+    case 16: integral_launch<16>(d, (int)nc, chunk, partial, st); break;
+    case 32: integral_launch<32>(d, (int)nc, chunk, partial, st); break;
+    default: integral_launch<64>(d, (int)nc, chunk, partial, st); break;
+  }
   SBI_HIP_CHECK(hipGetLastError());
-  dim3 g2((unsigned)((d.d_out + kIntThreads - 1) / kIntThreads));
-  hipLaunchKernelGGL(bary_integral_final_kernel, g2, dim3(kIntThreads), 0, st, d.d_out, d.offset,
-                     partial, Q);
+  dim3 g2((unsigned)((d.d_out + kFinO - 1) / kFinO));
+  hipLaunchKernelGGL(bary_integral_final_kernel, g2, dim3(kIntThreads), 0, st, d.d_out, (int)nc,
+                     d.offset, partial, Q);
   SBI_HIP_CHECK(hipGetLastError());
   return SBI_OK;
 }

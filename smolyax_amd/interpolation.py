@@ -173,6 +173,8 @@ class SmolyakBarycentricInterpolator:
         t["slot_nodes"] = _dev_tensor(L.slot_nodes, f64, dev)
         t["slot_w"] = _dev_tensor(L.slot_w, f64, dev)
         t["slot_q"] = _dev_tensor(L.slot_q, f64, dev)
+        eq = packing.entry_quadrature(L)
+        t["ent_q"] = _dev_tensor(eq if eq.size else [0.0], f64, dev)
         t["prefix_slot"] = _dev_tensor(L.prefix_slot if len(L.prefix_slot) else [0], i32, dev)
         t["prefix_off"] = _dev_tensor(L.prefix_off if len(L.prefix_off) else [0], i32, dev)
         t["run_slot"] = _dev_tensor(L.run_slot if len(L.run_slot) else [0], i32, dev)

@@ -268,6 +268,22 @@ def quadrature_table(L: Layout, node_gen) -> np.ndarray:
     return q
 
 
+def entry_quadrature(L: Layout) -> np.ndarray:
+    """Quadrature weight of every entry of F in device order: the outer product of the block's
+    factor weights (prefix factors, then the streamed one), C order, as in F
+    (interpolation.py:347-390 forms the same products per group)."""
+    out = np.empty(L.k_raw, dtype=np.float64)
+    for b in range(len(L.blk_n)):
+        slots = list(L.fac_slot[L.blk_fac[b]:L.blk_fac[b + 1]])
+        if L.blk_last_slot[b] >= 0:
+            slots.append(L.blk_last_slot[b])
+        w = np.ones(1)
+        for s in slots:
+            w = np.multiply.outer(w, L.slot_q[s, : L.slot_deg[s] + 1]).reshape(-1)
+        out[L.blk_foff[b]:L.blk_foff[b + 1]] = w
+    return out
+
+
 def entry_keys(fd, fg):
     """Grid entries of one block in C order (last factor fastest): (mu tuple, sparse key)."""
     n = len(fd)

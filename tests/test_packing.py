@@ -40,6 +40,11 @@ def test_layout_matches_oracle(kind, d, d_out, m):
     y = layout_eval.eval_layout(L, F, off, x)
     yr = ref(x)
     assert np.max(np.abs(y - yr)) / np.max(np.abs(yr)) < 1e-12
+    # the integral as the device computes it: offset + ent_q^T F, with the quadrature weights
+    L.slot_q = packing.quadrature_table(L, sng)
+    Q = off + packing.entry_quadrature(L) @ F
+    Qr = ref.integral()
+    assert np.max(np.abs(Q - Qr)) / np.max(np.abs(Qr)) < 1e-12
     # the wave segments cover every block exactly once, each inside one run of its streamed slot
     covered = np.zeros(len(L.blk_n), dtype=int)
     for g in range(L.seg_off[-1]):
