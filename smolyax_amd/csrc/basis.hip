@@ -228,16 +228,17 @@ __global__ __launch_bounds__(kTile* kBWaves) void newton_vjp_kernel(sbi_newton_d
 // derivative-basis value D[p, e] = h'_k(x_j) prod_{other factors (j',k') of nu} h_k'(x_j'), and
 //   J[p, o, j] = sum over the entries e of dim j of D[p, e] S[nu_e, o].
 // The factor table of the tile sits in LDS when it fits, else in a global table written first by
-// newton_htab_kernel ([row][ldt], points contiguous).
+// newton_htab_kernel, tile by tile ([tile][row][64 points]: one tile's table is contiguous, so a
+// wave's reads stay within a few pages).
 // ------------------------------------------------------------------------------------------
 template <bool kG>
 struct HTab {
   typename lds_t<double>::type* lds;
   const double* __restrict__ g;
-  int64_t ldt;
+  int64_t ldt;  // rows per tile of the global table
   __device__ __forceinline__ double at(int r, int lane, int64_t p) const {
     if constexpr (kG) {
-      return g[(int64_t)r * ldt + p];
+      return g[((int64_t)blockIdx.x * ldt + r) * kTile + lane];
     } else {
       return lds[r * kTile + lane];
     }
@@ -260,7 +261,7 @@ __global__ __launch_bounds__(256) void newton_htab_kernel(sbi_newton_desc d,
     double pr = 1.0;
     for (int k = 1; k <= km; ++k) {
       pr *= (xj - xi[k - 1]);
-      tab[(int64_t)(row + k - 1) * ldt + p] = pr * c[k];
+      tab[((p / kTile) * ldt + row + k - 1) * kTile + (p % kTile)] = pr * c[k];
     }
   }
 }
@@ -710,7 +711,8 @@ extern "C" int64_t sbi_newton_jacobian_workspace(const sbi_newton_plan* plan, in
   const sbi_newton_desc* dp = nullptr;
   if (sbi_newton_plan_desc(plan, &dp) != SBI_OK || P < 0 || n < 0) return -1;
   int64_t bytes = 0;
-  if (jac_global_table(*dp)) bytes += (int64_t)dp->n_tab_rows * P * (int64_t)sizeof(double);
+  const int64_t P64 = (P + kTile - 1) / kTile * kTile;
+  if (jac_global_table(*dp)) bytes += (int64_t)dp->n_tab_rows * P64 * (int64_t)sizeof(double);
   if (n > kJacDirectMax) bytes += (int64_t)dp->n_vjp_ent * P * (int64_t)sizeof(double);
   else bytes += (int64_t)dp->n_jac_partials * P * n * (int64_t)sizeof(double);
   return bytes;
@@ -732,8 +734,8 @@ static void jac_direct_launch(const sbi_newton_desc& d, const double* x, int64_t
   const int64_t ny = std::max<int64_t>(
       1, std::min<int64_t>((d.n_jac_units + kBWaves - 1) / kBWaves, (4096 + tiles - 1) / tiles));
   hipLaunchKernelGGL((newton_jac_kernel<NO, kG>), dim3((unsigned)tiles, (unsigned)ny),
-                     dim3(kTile * kBWaves), lds_bytes, st, d, x, P, ldx, S, lds, n, gtab, P, J,
-                     part);
+                     dim3(kTile * kBWaves), lds_bytes, st, d, x, P, ldx, S, lds, n, gtab,
+                     (int64_t)d.n_tab_rows, J, part);
 }
 
 template <bool kG>
@@ -752,7 +754,8 @@ static void jac_basis_launch(const sbi_newton_desc& d, const double* x, int64_t 
   const int64_t ny = std::max<int64_t>(
       1, std::min<int64_t>((ngrp + kDTWaves - 1) / kDTWaves, (1024 + tiles - 1) / tiles));
   hipLaunchKernelGGL(newton_jac_basis_kernel<kG>, dim3((unsigned)tiles, (unsigned)ny),
-                     dim3(kTile * kDTWaves), lds_bytes, st, d, x, P, ldx, gtab, P, DT, P);
+                     dim3(kTile * kDTWaves), lds_bytes, st, d, x, P, ldx, gtab, (int64_t)d.n_tab_rows,
+                     DT, P);
 }
 
 extern "C" int sbi_newton_jacobian_f64(const sbi_newton_plan* plan, const double* x, int64_t P,
@@ -785,11 +788,12 @@ extern "C" int sbi_newton_jacobian_f64(const sbi_newton_plan* plan, const double
   hipStream_t st = (hipStream_t)stream;
   const bool g = jac_global_table(d);
   double* gtab = g ? (double*)workspace : nullptr;
-  double* DT = (double*)workspace + (g ? (int64_t)d.n_tab_rows * P : 0);
+  const int64_t P64 = (P + kTile - 1) / kTile * kTile;
+  double* DT = (double*)workspace + (g ? (int64_t)d.n_tab_rows * P64 : 0);
   if (g && d.n_tab_dims > 0) {
     const int64_t ny = std::min<int64_t>(d.n_tab_dims, 64);
     hipLaunchKernelGGL(newton_htab_kernel, dim3((unsigned)((P + 255) / 256), (unsigned)ny),
-                       dim3(256), 0, st, d, x, P, ldx, gtab, P);
+                       dim3(256), 0, st, d, x, P, ldx, gtab, (int64_t)d.n_tab_rows);
     SBI_HIP_CHECK(hipGetLastError());
   }
   if (n <= kJacDirectMax) {
