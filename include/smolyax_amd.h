@@ -251,9 +251,14 @@ int sbi_newton_rows_f64(const sbi_newton_plan* plan, const double* x, int64_t P,
                         const int32_t* fac, int32_t n_rows, double* HT, int64_t ldh, void* stream);
 /* Adjoint of the Newton form (the VJP of __call__, gradient path of interpolation.py:306-345):
  * gx[P, d_in] = sum_nu G[p, nu] d H[p, nu] / dx  with G^T[basis_n, ldg] = S V^T given
- * (dtype 0: FP64, 1: FP32).  Deterministic (no atomics). */
+ * (dtype 0: FP64, 1: FP32), rows in the adjoint's node order.  Work units of at most 128 adjoint
+ * entries of one dim; a dim split over several units adds their partial sums in unit order, so
+ * the result is deterministic.  workspace: sbi_newton_vjp_workspace(plan, P) bytes (partial sums,
+ * and the factor table when it exceeds LDS). */
+int64_t sbi_newton_vjp_workspace(const sbi_newton_plan* plan, int64_t P);
 int sbi_newton_vjp(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
-                   const void* GT, int64_t ldg, int32_t dtype, double* gx, void* stream);
+                   const void* GT, int64_t ldg, int32_t dtype, double* gx, void* workspace,
+                   int64_t ws_bytes, void* stream);
 /* Bytes of device workspace sbi_newton_jacobian_f64 needs for P points and n outputs: the factor
  * table when it exceeds LDS; for n <= 8 the partial sums of split dims; for n > 8 the derivative
  * basis D^T (adjoint entries x P). */

@@ -92,7 +92,9 @@ SIGNATURES = {
     "sbi_newton_eval_mixed": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp]),
     "sbi_newton_eval_f64": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp]),
     "sbi_newton_basis": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_i32, c_vp]),
-    "sbi_newton_vjp": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_i32, c_vp, c_vp]),
+    "sbi_newton_vjp_workspace": (c_i64, [c_vp, c_i64]),
+    "sbi_newton_vjp": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_i32, c_vp, c_vp, c_i64,
+                               c_vp]),
     "sbi_newton_jacobian_workspace": (c_i64, [c_vp, c_i64, c_i64]),
     "sbi_newton_jacobian_f64": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, c_vp, c_vp,
                                         c_i64, c_vp]),

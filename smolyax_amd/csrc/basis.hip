@@ -9,8 +9,9 @@
 // Adjoint (vector-Jacobian product) of the same form: with G = V S^T (a GEMM, V = dL/dy),
 //   dL/dx_{p,j} = sum_k h'^{(j)}_k(x_{p,j}) A[p, (j,k)],
 //   A[p, (j,k)] = sum_{nu containing (j,k)} G[p, nu] prod_{other factors (j',k') of nu} h_{k'}(x_{p,j'}).
-// newton_vjp_kernel evaluates it per tile from G^T, one dim per wave at a time (no atomics: each
-// (p, j) is written once).
+// newton_jac_kernel<1, kG, T> evaluates it per tile from G^T over work units of at most 128
+// adjoint entries of one dim (no atomics: each (p, j) is written once, a dim split over several
+// units adds their partial sums in unit order).
 //
 // Barycentric form (the reference's algorithm): bary_basis_kernel writes the per-entry weights
 // W^T[e, p] = prod_i Lhat_i(x_{p, s_i})[mu_i] of every block entry e (K_raw rows) so that
@@ -25,6 +26,7 @@
 #include <cmath>
 #include <cstdint>
 #include <string>
+#include <type_traits>
 
 #include "../../include/smolyax_amd.h"
 #include "common.h"
@@ -160,69 +162,6 @@ __global__ __launch_bounds__(256) void newton_rows_f64_kernel(sbi_newton_desc d,
   }
 }
 
-// dL/dx from G^T (nodes x points).  Per table dim (one wave at a time): for each of its k's the
-// list of (node, other factor rows) entries; A accumulates in FP64.
-template <typename T>
-__global__ __launch_bounds__(kTile* kBWaves) void newton_vjp_kernel(sbi_newton_desc d,
-                                                                    const double* __restrict__ x,
-                                                                    int64_t P, int64_t ldx,
-                                                                    const T* __restrict__ GT,
-                                                                    int64_t ldg,
-                                                                    double* __restrict__ gx) {
-  extern __shared__ double lds_raw[];
-  typedef typename lds_t<T>::type LT;
-  LT* tab = (LT*)lds_raw;
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t p_raw = (int64_t)blockIdx.x * kTile + lane;
-  const bool valid = p_raw < P;
-  const int64_t p = valid ? p_raw : (P - 1);
-  const double* __restrict__ xp = x + p * ldx;
-  build_h_table<T>(d, xp, tab, nullptr, wave, lane, kBWaves);
-  __syncthreads();
-
-  cptr<int32_t> tdim = C(d.tab_dim), trow = C(d.tab_row), tk = C(d.tab_kmax),
-                slot_off = C(d.vjp_slot_off);
-  cptr<double> dim_xi = C(d.dim_xi), dim_c = C(d.dim_c);
-  const uint4* __restrict__ ent = reinterpret_cast<const uint4*>(d.vjp_ent);
-  for (int i = wave; i < d.n_tab_dims; i += kBWaves) {
-    const int j = tdim[i], row = trow[i], km = tk[i];
-    const double xj = xp[j];
-    cptr<double> xi = dim_xi + (int64_t)j * d.kmax1;
-    cptr<double> cj = dim_c + (int64_t)j * d.kmax1;
-    double gj = 0.0, pr = 1.0, dpr = 0.0;  // d/dx prod_{r<k} (x - xi_r), FP64
-    for (int k = 1; k <= km; ++k) {
-      const double dx = xj - xi[k - 1];
-      dpr = dpr * dx + pr;
-      pr *= dx;
-      const int s = row + k - 1;
-      double A = 0.0;
-      const int e0 = slot_off[s], e1 = slot_off[s + 1];
-      for (int base = e0; base < e1; base += 64) {
-        const int mine = base + lane;
-        const uint4 ev = (mine < e1) ? ent[mine] : make_uint4(0u, 0xffffffffu, 0xffffffffu, 0xffffffffu);
-        const int cnt = min(64, e1 - base);
-        for (int q = 0; q < cnt; ++q) {
-          const int node = __builtin_amdgcn_readlane((int)ev.x, q);
-          const uint32_t w1 = (uint32_t)__builtin_amdgcn_readlane((int)ev.y, q);
-          const uint32_t w2 = (uint32_t)__builtin_amdgcn_readlane((int)ev.z, q);
-          const uint32_t w3 = (uint32_t)__builtin_amdgcn_readlane((int)ev.w, q);
-          const uint32_t ws[3] = {w1, w2, w3};
-          double v = (double)GT[(int64_t)node * ldg + p];
-#pragma unroll
-          for (int f = 0; f < 6; ++f) {
-            const uint32_t r = (ws[f >> 1] >> ((f & 1) * 16)) & 0xffff;
-            if (r != 0xffff) v *= (double)tab[r * kTile + lane];
-          }
-          A += v;
-        }
-      }
-      gj = fma(dpr * cj[k], A, gj);
-    }
-    if (valid) gx[p * d.d_in + j] = gj;
-  }
-}
-
 // ------------------------------------------------------------------------------------------
 // Dense Jacobian of the Newton form.  Adjoint entry e = (node nu, factor (j, k) of nu) has the
 // derivative-basis value D[p, e] = h'_k(x_j) prod_{other factors (j',k') of nu} h_k'(x_j'), and
@@ -294,11 +233,16 @@ __device__ __forceinline__ double row_derivative(const sbi_newton_desc& d, const
 // One pass over n <= NO outputs, sums in registers.  Waves take work units (runs of at most 128
 // adjoint entries of one dim) round-robin over grid.y workgroups per tile; a unit that covers
 // its dim writes J, the others write partial sums that jac_reduce_kernel adds in unit order.
-template <int NO, bool kG>
+//
+// TG = void: the coefficients are the surplus rows S[node, o] (the same for every point, scalar
+// loads).  TG = float / double: the adjoint (VJP), one output, the coefficient of point p is
+// G^T[node, p] = (S V^T)[node, p] (a per-lane load); J is then gx[P, d_in].
+template <int NO, bool kG, typename TG = void>
 __global__ __launch_bounds__(kTile* kBWaves) void newton_jac_kernel(
     sbi_newton_desc d, const double* __restrict__ x, int64_t P, int64_t ldx,
     const double* __restrict__ S, int64_t lds, int n, const double* __restrict__ gtab,
-    int64_t ldt, double* __restrict__ J, double* __restrict__ part) {
+    int64_t ldt, double* __restrict__ J, double* __restrict__ part,
+    const void* __restrict__ GTv = nullptr, int64_t ldg = 0) {
   extern __shared__ double lds_raw[];
   typedef typename lds_t<double>::type LT;
   const int lane = threadIdx.x & 63;
@@ -331,26 +275,48 @@ __global__ __launch_bounds__(kTile* kBWaves) void newton_jac_kernel(
                                    : make_uint4(0u, 0xffffffffu, 0xffffffffu, 0xffffffffu);
       const int rl = row_of_entry(d.vjp_slot_off, d.n_tab_rows, min(mine, e1 - 1));
       const int cnt = min(64, e1 - base);
-      for (int q = 0; q < cnt; ++q) {
-        const int s = __builtin_amdgcn_readlane(rl, q);
-        if (s != s_done) {  // next factor row of the dim
-          dh = row_derivative(d, xp, j, s - row0 + 1);
-          s_done = s;
-        }
-        const int node = __builtin_amdgcn_readlane((int)ev.x, q);
-        const uint32_t ws[3] = {(uint32_t)__builtin_amdgcn_readlane((int)ev.y, q),
-                                (uint32_t)__builtin_amdgcn_readlane((int)ev.z, q),
-                                (uint32_t)__builtin_amdgcn_readlane((int)ev.w, q)};
-        double v = dh;
+      // adjoint: the per-point coefficients of kPre entries are requested together, ahead of
+      // their use (independent loads in flight instead of one at a time)
+      constexpr int kPre = std::is_void_v<TG> ? 1 : 8;
+      for (int q0 = 0; q0 < cnt; q0 += kPre) {
+        double cv[kPre];
+        (void)cv;
+        if constexpr (!std::is_void_v<TG>) {
+          const TG* __restrict__ GT = reinterpret_cast<const TG*>(GTv);
 #pragma unroll
-        for (int f = 0; f < 6; ++f) {
-          const uint32_t r = (ws[f >> 1] >> ((f & 1) * 16)) & 0xffff;
-          if (r != 0xffff) v *= H.at((int)r, lane, p);
+          for (int u = 0; u < kPre; ++u) {
+            const int nq = __builtin_amdgcn_readlane((int)ev.x, min(q0 + u, cnt - 1));
+            cv[u] = (double)GT[(int64_t)nq * ldg + p];
+          }
         }
-        cptr<double> Sn = C(S) + (int64_t)node * lds;
 #pragma unroll
-        for (int o = 0; o < NO; ++o)
-          if (o < n) g[o] = fma(v, Sn[o], g[o]);
+        for (int u = 0; u < kPre; ++u) {
+          const int q = q0 + u;
+          if (q >= cnt) break;
+          const int s = __builtin_amdgcn_readlane(rl, q);
+          if (s != s_done) {  // next factor row of the dim
+            dh = row_derivative(d, xp, j, s - row0 + 1);
+            s_done = s;
+          }
+          const uint32_t ws[3] = {(uint32_t)__builtin_amdgcn_readlane((int)ev.y, q),
+                                  (uint32_t)__builtin_amdgcn_readlane((int)ev.z, q),
+                                  (uint32_t)__builtin_amdgcn_readlane((int)ev.w, q)};
+          double v = dh;
+#pragma unroll
+          for (int f = 0; f < 6; ++f) {
+            const uint32_t r = (ws[f >> 1] >> ((f & 1) * 16)) & 0xffff;
+            if (r != 0xffff) v *= H.at((int)r, lane, p);
+          }
+          if constexpr (std::is_void_v<TG>) {
+            const int node = __builtin_amdgcn_readlane((int)ev.x, q);
+            cptr<double> Sn = C(S) + (int64_t)node * lds;
+#pragma unroll
+            for (int o = 0; o < NO; ++o)
+              if (o < n) g[o] = fma(v, Sn[o], g[o]);
+          } else {
+            g[0] = fma(v, cv[u], g[0]);
+          }
+        }
       }
     }
     if (valid) {
@@ -618,47 +584,100 @@ extern "C" int sbi_newton_rows_f64(const sbi_newton_plan* plan, const double* x,
   return SBI_OK;
 }
 
+static bool jac_global_table(const sbi_newton_desc& d);
+
+extern "C" int64_t sbi_newton_vjp_workspace(const sbi_newton_plan* plan, int64_t P) {
+  const sbi_newton_desc* dp = nullptr;
+  if (sbi_newton_plan_desc(plan, &dp) != SBI_OK || P < 0) return -1;
+  const int64_t P64 = (P + kTile - 1) / kTile * kTile;
+  int64_t bytes = (int64_t)dp->n_jac_partials * P * (int64_t)sizeof(double);
+  if (jac_global_table(*dp)) bytes += (int64_t)dp->n_tab_rows * P64 * (int64_t)sizeof(double);
+  return bytes;
+}
+
 template <typename T>
 static int vjp_launch(const sbi_newton_desc& d, const double* x, int64_t P, int64_t ldx,
-                      const T* GT, int64_t ldg, double* gx, hipStream_t st) {
-  const size_t lds = (size_t)d.n_tab_rows * kTile * sizeof(T);
-  if (lds > (size_t)kMaxLds) {
-    sbi::set_error("sbi_newton_vjp: the tile's factor table exceeds LDS (use FP32: mixed)");
-    return SBI_ERR_CAPACITY;
+                      const T* GT, int64_t ldg, double* gx, void* workspace, hipStream_t st) {
+  const bool g = jac_global_table(d);
+  const int64_t P64 = (P + kTile - 1) / kTile * kTile;
+  double* gtab = g ? (double*)workspace : nullptr;
+  double* part = (double*)workspace + (g ? (int64_t)d.n_tab_rows * P64 : 0);
+  if (g && d.n_tab_dims > 0) {
+    const int64_t ny = std::min<int64_t>(d.n_tab_dims, 64);
+    hipLaunchKernelGGL(newton_htab_kernel, dim3((unsigned)((P + 255) / 256), (unsigned)ny),
+                       dim3(256), 0, st, d, x, P, ldx, gtab, (int64_t)d.n_tab_rows);
+    SBI_HIP_CHECK(hipGetLastError());
   }
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void*)newton_vjp_kernel<T>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
-    attr = true;
+  if (d.n_tab_dims < d.d_in) {  // dims without a factor keep 0
+    const int64_t n = P * d.d_in;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, gx,
+                       n);
+    SBI_HIP_CHECK(hipGetLastError());
   }
-  const int64_t n = P * d.d_in;
-  hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, gx, n);
-  SBI_HIP_CHECK(hipGetLastError());
-  dim3 grid((unsigned)((P + kTile - 1) / kTile));
-  hipLaunchKernelGGL(newton_vjp_kernel<T>, grid, dim3(kTile * kBWaves), lds, st, d, x, P, ldx, GT,
-                     ldg, gx);
-  SBI_HIP_CHECK(hipGetLastError());
+  if (d.n_jac_units > 0) {
+    const size_t lds_bytes = g ? 0 : (size_t)d.n_tab_rows * kTile * sizeof(double);
+    const int64_t tiles = (P + kTile - 1) / kTile;
+    const int64_t ny = std::max<int64_t>(
+        1, std::min<int64_t>((d.n_jac_units + kBWaves - 1) / kBWaves, (4096 + tiles - 1) / tiles));
+    const dim3 grid((unsigned)tiles, (unsigned)ny);
+    if (g) {
+      static bool attr = false;
+      if (!attr) {
+        (void)hipFuncSetAttribute((const void*)newton_jac_kernel<1, true, T>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
+        attr = true;
+      }
+      hipLaunchKernelGGL((newton_jac_kernel<1, true, T>), grid, dim3(kTile * kBWaves), lds_bytes,
+                         st, d, x, P, ldx, nullptr, (int64_t)0, 1, gtab, (int64_t)d.n_tab_rows,
+                         gx, part, (const void*)GT, ldg);
+    } else {
+      static bool attr = false;
+      if (!attr) {
+        (void)hipFuncSetAttribute((const void*)newton_jac_kernel<1, false, T>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
+        attr = true;
+      }
+      hipLaunchKernelGGL((newton_jac_kernel<1, false, T>), grid, dim3(kTile * kBWaves), lds_bytes,
+                         st, d, x, P, ldx, nullptr, (int64_t)0, 1, gtab, (int64_t)d.n_tab_rows,
+                         gx, part, (const void*)GT, ldg);
+    }
+    SBI_HIP_CHECK(hipGetLastError());
+  }
+  if (d.n_jac_red > 0) {
+    hipLaunchKernelGGL(jac_reduce_kernel, dim3((unsigned)((P + 255) / 256), (unsigned)d.n_jac_red),
+                       dim3(256), 0, st, d, P, (int64_t)1, part, gx);
+    SBI_HIP_CHECK(hipGetLastError());
+  }
   return SBI_OK;
 }
 
 extern "C" int sbi_newton_vjp(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
                               const void* GT, int64_t ldg, int32_t dtype, double* gx,
-                              void* stream) {
+                              void* workspace, int64_t ws_bytes, void* stream) {
   const sbi_newton_desc* dp = nullptr;
-  if (sbi_newton_plan_desc(plan, &dp) != SBI_OK || !x || !GT || !gx || P < 0 || ldg < P) {
+  if (sbi_newton_plan_desc(plan, &dp) != SBI_OK || !x || !GT || !gx || P < 0 || ldg < P ||
+      ws_bytes < 0) {
     sbi::set_error("sbi_newton_vjp: invalid argument");
     return SBI_ERR_INVALID;
   }
   const sbi_newton_desc& d = *dp;
-  if (!d.vjp_ent || !d.vjp_slot_off || !d.tab_dim || ldx < d.d_in) {
+  if (!d.vjp_ent || !d.vjp_slot_off || !d.tab_dim || !d.jac_unit || !d.jac_red || ldx < d.d_in) {
     sbi::set_error("sbi_newton_vjp: plan has no adjoint tables");
     return SBI_ERR_INVALID;
   }
   if (P == 0) return SBI_OK;
+  if (P > (int64_t)65535 * kTile) {
+    sbi::set_error("sbi_newton_vjp: at most 65535*64 points per call");
+    return SBI_ERR_INVALID;
+  }
+  const int64_t need = sbi_newton_vjp_workspace(plan, P);
+  if (ws_bytes < need || (need > 0 && !workspace)) {
+    sbi::set_error("sbi_newton_vjp: workspace smaller than sbi_newton_vjp_workspace()");
+    return SBI_ERR_INVALID;
+  }
   hipStream_t st = (hipStream_t)stream;
-  if (dtype == 0) return vjp_launch<double>(d, x, P, ldx, (const double*)GT, ldg, gx, st);
-  if (dtype == 1) return vjp_launch<float>(d, x, P, ldx, (const float*)GT, ldg, gx, st);
+  if (dtype == 0) return vjp_launch<double>(d, x, P, ldx, (const double*)GT, ldg, gx, workspace, st);
+  if (dtype == 1) return vjp_launch<float>(d, x, P, ldx, (const float*)GT, ldg, gx, workspace, st);
   sbi::set_error("sbi_newton_vjp: dtype must be 0 (f64) or 1 (f32)");
   return SBI_ERR_INVALID;
 }

@@ -481,8 +481,11 @@ class SmolyakBarycentricInterpolator:
         S = self._surpluses(kind)
         GT = torch.mm(S, v.to(S.dtype).t()).contiguous()
         gx = torch.empty((P, self._d_in), dtype=torch.float64, device=self._device)
+        nb = int(N.fn["sbi_newton_vjp_workspace"](self._nplan, P))
+        ws = torch.empty(max(1, (nb + 7) // 8), dtype=torch.float64, device=self._device)
         N.check(N.fn["sbi_newton_vjp"](self._nplan, x.data_ptr(), P, x.stride(0), GT.data_ptr(),
-                                       GT.stride(0), code, gx.data_ptr(), self._stream()), "vjp")
+                                       GT.stride(0), code, gx.data_ptr(), ws.data_ptr(), nb,
+                                       self._stream()), "vjp")
         return gx
 
     def differentiable(self, x: torch.Tensor, precision: str = None) -> torch.Tensor:

@@ -127,3 +127,16 @@ def test_jacobian_many_outputs_matches_adjoint(gpu):
     assert rel(g, ip.vjp(x, v, "fp64").cpu().numpy()) < 1e-10
     # the first points against the oracle's Jacobian
     assert rel(J[:3].cpu().numpy(), ref.gradient(x[:3])) < 1e-11
+
+
+def test_vjp_global_factor_table(gpu):
+    """400 input dims: the factor table (369 rows) exceeds LDS and is read from the global table;
+    dims with more than 128 adjoint entries are split into units whose partial sums are added in
+    a fixed order (FP64 and FP32 G)."""
+    ref, ip, rng = make(400, 3, 1000, seed=12)
+    assert ip._nlayout.n_tab_rows * 64 * 8 > 160 * 1024 and ip._nlayout.n_jac_red > 0
+    x = rng.uniform(-1, 1, (70, 400))
+    v = rng.standard_normal((70, 3))
+    g_ref = np.einsum("pm,pmj->pj", v, ref.gradient(x))
+    assert rel(ip.vjp(x, v, "fp64").cpu().numpy(), g_ref) < 1e-10
+    assert rel(ip.vjp(x, v, "mixed").cpu().numpy(), g_ref) < 1e-5
