@@ -9,7 +9,7 @@ sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device); runs on the GPU box")
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
 
 
 def gpu_available():
