@@ -324,7 +324,7 @@ def bench_serving(torch, W, n_requests=600, qps=None, seed=0):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "smolyax_amd runs on one GPU (a multi-GPU run would split the points, see the design notes); --gpus must be 1",
+        print(json.dumps({"error": "smolyax_amd runs on one GPU; --gpus must be 1 (points are independent: several GPUs would each take a share of them)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
