@@ -239,7 +239,9 @@ int sbi_newton_eval_f64(const sbi_newton_plan* plan, const double* x, int64_t P,
  * 2: BF16 rounded from FP32 values); y = H S is then one dense GEMM (large d_out). */
 int sbi_newton_basis(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx, void* HT,
                      int64_t ldh, int32_t dtype, void* stream);
-/* Rows [row0, row1) of the same basis matrix into HT rows 0 .. row1 - row0 - 1. */
+/* Rows [row0, row1) of the same basis matrix into HT rows 0 .. row1 - row0 - 1.  dtype 3: FP32 in
+ * the k-interleaved layout of sbi_contract_split_k4 (HT[((r / 4) * ldh + p) * 4 + r % 4], rows
+ * padded with zeros to a multiple of 16, HT 16-byte aligned). */
 int sbi_newton_basis_rows(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
                           int32_t row0, int32_t row1, void* HT, int64_t ldh, int32_t dtype,
                           void* stream);
@@ -306,6 +308,17 @@ int sbi_contract_split(const double* A64, int64_t lda64, const double* B64, int6
                        int64_t K64, const float* A32, int64_t lda32, const float* B32,
                        int64_t ldb32, int64_t K32, const double* offset, double* Y, int64_t ldy,
                        int64_t P, int64_t n, void* stream);
+
+/* sbi_contract_split with the FP32 operands in the k-interleaved layout: element (k, p) of A32 at
+ * A32[((k / 4) * lda32 + p) * 4 + k % 4], element (k, c) of B32 at B32[((k / 4) * ldb32 + c) * 4
+ * + k % 4], K32 a multiple of 16 (zero rows pad it), both 16-byte aligned.  A lane then loads the
+ * four k of its MFMA steps as one 16-byte vector.  Same results bit for bit as sbi_contract_split
+ * on the same values in the plain layout.  No counterpart in the reference (a layout of the
+ * contraction in interpolation.py:264-304). */
+int sbi_contract_split_k4(const double* A64, int64_t lda64, const double* B64, int64_t ldb64,
+                          int64_t K64, const float* A32, int64_t lda32, const float* B32,
+                          int64_t ldb32, int64_t K32, const double* offset, double* Y,
+                          int64_t ldy, int64_t P, int64_t n, void* stream);
 
 /* Descriptor accessors of the plan objects (shared by the library's translation units). */
 int sbi_bary_plan_desc(const sbi_plan* plan, const sbi_bary_desc** out);

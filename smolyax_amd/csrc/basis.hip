@@ -88,7 +88,11 @@ __device__ __forceinline__ void store_out(__hip_bfloat16* p, double v) {
 }
 
 // H^T[nu, p] for a chunk of nodes.  Node descriptor: 8 x u16 table rows (0xffff = none).
-template <typename T, typename OUT>
+//
+// kQ (FP32 only): k-interleaved output for the contraction, HT[((r / 4) * ldh + p) * 4 + r % 4]
+// for row r = row - row0, rows padded with zeros to a multiple of 16; each lane stores the four
+// rows of a quad as one 16-byte vector.
+template <typename T, typename OUT, bool kQ = false>
 __global__ __launch_bounds__(kTile* kBWaves) void newton_basis_kernel(sbi_newton_desc d,
                                                                       const double* __restrict__ x,
                                                                       int64_t P, int64_t ldx,
@@ -107,14 +111,41 @@ __global__ __launch_bounds__(kTile* kBWaves) void newton_basis_kernel(sbi_newton
   __syncthreads();
 
   const int n0 = row0 + blockIdx.y * chunk;
-  const int n1 = min(row1, n0 + chunk);
+  const int rend = kQ ? row0 + ((row1 - row0 + 15) & ~15) : row1;
+  const int n1 = min(rend, n0 + chunk);
   const uint4* __restrict__ desc = reinterpret_cast<const uint4*>(d.basis_desc);
   // each wave: 64-node groups, descriptors one per lane, broadcast with v_readlane
   for (int base = n0 + wave * 64; base < n1; base += kBWaves * 64) {
     const int mine = base + lane;
-    const uint4 dv = (mine < n1) ? desc[mine] : make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu,
-                                                             0xffffffffu);
+    const uint4 dv = (mine < min(n1, row1)) ? desc[mine]
+                                            : make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu,
+                                                         0xffffffffu);
     const int cnt = min(64, n1 - base);
+    if constexpr (kQ) {
+      for (int i = 0; i < cnt; i += 4) {  // cnt is a multiple of 4 (16-row padding, 64-row groups)
+        float q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t w[4] = {(uint32_t)__builtin_amdgcn_readlane((int)dv.x, i + u),
+                                 (uint32_t)__builtin_amdgcn_readlane((int)dv.y, i + u),
+                                 (uint32_t)__builtin_amdgcn_readlane((int)dv.z, i + u),
+                                 (uint32_t)__builtin_amdgcn_readlane((int)dv.w, i + u)};
+          T v = (T)1;
+#pragma unroll
+          for (int f = 0; f < 8; ++f) {
+            const uint32_t r = (w[f >> 1] >> ((f & 1) * 16)) & 0xffff;
+            if (r != 0xffff) v *= tab[r * kTile + lane];
+          }
+          q[u] = (base + i + u < row1) ? (float)v : 0.0f;
+        }
+        if (valid) {
+          typedef float v4f_t __attribute__((ext_vector_type(4)));
+          *reinterpret_cast<v4f_t*>(HT + ((int64_t)((base + i - row0) >> 2) * ldh + p) * 4) =
+              v4f_t{q[0], q[1], q[2], q[3]};
+        }
+      }
+      continue;
+    }
     for (int i = 0; i < cnt; ++i) {
       const uint32_t w[4] = {(uint32_t)__builtin_amdgcn_readlane((int)dv.x, i),
                              (uint32_t)__builtin_amdgcn_readlane((int)dv.y, i),
@@ -504,7 +535,7 @@ __global__ __launch_bounds__(kTile) void bary_basis_kernel(sbi_bary_desc d,
 
 extern "C" int sbi_newton_plan_desc(const sbi_newton_plan* plan, const sbi_newton_desc** out);
 
-template <typename T, typename OUT>
+template <typename T, typename OUT, bool kQ = false>
 static int basis_launch(const sbi_newton_desc& d, const double* x, int64_t P, int64_t ldx, OUT* HT,
                         int64_t ldh, int row0, int row1, hipStream_t st) {
   const size_t lds = (size_t)d.n_tab_rows * kTile * sizeof(T);
@@ -514,15 +545,16 @@ static int basis_launch(const sbi_newton_desc& d, const double* x, int64_t P, in
   }
   static bool attr = false;
   if (!attr) {
-    (void)hipFuncSetAttribute((const void*)newton_basis_kernel<T, OUT>,
+    (void)hipFuncSetAttribute((const void*)newton_basis_kernel<T, OUT, kQ>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
     attr = true;
   }
   const int chunk = 2048;
   if (row1 <= row0) return SBI_OK;
-  dim3 grid((unsigned)((P + kTile - 1) / kTile), (unsigned)((row1 - row0 + chunk - 1) / chunk));
-  hipLaunchKernelGGL((newton_basis_kernel<T, OUT>), grid, dim3(kTile * kBWaves), lds, st, d, x, P,
-                     ldx, HT, ldh, chunk, row0, row1);
+  const int rows = kQ ? ((row1 - row0 + 15) & ~15) : row1 - row0;
+  dim3 grid((unsigned)((P + kTile - 1) / kTile), (unsigned)((rows + chunk - 1) / chunk));
+  hipLaunchKernelGGL((newton_basis_kernel<T, OUT, kQ>), grid, dim3(kTile * kBWaves), lds, st, d, x,
+                     P, ldx, HT, ldh, chunk, row0, row1);
   SBI_HIP_CHECK(hipGetLastError());
   return SBI_OK;
 }
@@ -552,8 +584,15 @@ extern "C" int sbi_newton_basis_rows(const sbi_newton_plan* plan, const double* 
     case 2:
       return basis_launch<float, __hip_bfloat16>(d, x, P, ldx, (__hip_bfloat16*)HT, ldh, row0,
                                                  row1, st);
+    case 3:
+      if (((uintptr_t)HT & 15) != 0) {
+        sbi::set_error("sbi_newton_basis: the k-interleaved layout needs a 16-byte aligned HT");
+        return SBI_ERR_INVALID;
+      }
+      return basis_launch<float, float, true>(d, x, P, ldx, (float*)HT, ldh, row0, row1, st);
     default:
-      sbi::set_error("sbi_newton_basis: dtype must be 0 (f64), 1 (f32) or 2 (bf16)");
+      sbi::set_error("sbi_newton_basis: dtype must be 0 (f64), 1 (f32), 2 (bf16) or 3 (f32, "
+                     "k-interleaved)");
       return SBI_ERR_INVALID;
   }
 }

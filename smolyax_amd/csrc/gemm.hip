@@ -65,7 +65,7 @@ __device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
 // contracted first with the FP64 MFMA, straight into the FP64 accumulators.  Lane li supplies the
 // A row of point 4*(li & 3) + (li >> 2) of its 16-point block, which puts the FP64 MFMA's result
 // (row g + 4r in register r of lane group g) into the FP32 layout (row 4g + r).
-template <typename T, int FN>
+template <typename T, int FN, bool kK4 = false>
 __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
     const T* __restrict__ A, int64_t lda, const T* __restrict__ B, int64_t ldb,
     const double* __restrict__ offset, double* __restrict__ Y, int64_t ldy, int64_t P, int64_t n,
@@ -146,16 +146,38 @@ __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
 
   const int64_t K16 = K & ~(int64_t)(kChunk - 1);
   T a[2][4], b[FN][4];
+  typedef T v4t __attribute__((ext_vector_type(4)));
   auto load = [&](int64_t k0) {
+    if constexpr (kK4) {
+      // k-interleaved operands: the four k of a lane's steps are one 16-byte vector
+      const int64_t kq = (k0 >> 2) + g;
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int64_t k = k0 + 4 * g + s;
-      const T* __restrict__ ar = A + k * lda;
-      const T* __restrict__ br = B + k * ldb;
+      for (int fm = 0; fm < 2; ++fm) {
+        const v4t v = *reinterpret_cast<const v4t*>(A + (kq * lda + pcol[fm]) * 4);
+        a[fm][0] = v.x;
+        a[fm][1] = v.y;
+        a[fm][2] = v.z;
+        a[fm][3] = v.w;
+      }
 #pragma unroll
-      for (int fm = 0; fm < 2; ++fm) a[fm][s] = ar[pcol[fm]];
+      for (int fn = 0; fn < FN; ++fn) {
+        const v4t v = *reinterpret_cast<const v4t*>(B + (kq * ldb + jcol[fn]) * 4);
+        b[fn][0] = v.x;
+        b[fn][1] = v.y;
+        b[fn][2] = v.z;
+        b[fn][3] = v.w;
+      }
+    } else {
 #pragma unroll
-      for (int fn = 0; fn < FN; ++fn) b[fn][s] = br[jcol[fn]];
+      for (int s = 0; s < 4; ++s) {
+        const int64_t k = k0 + 4 * g + s;
+        const T* __restrict__ ar = A + k * lda;
+        const T* __restrict__ br = B + k * ldb;
+#pragma unroll
+        for (int fm = 0; fm < 2; ++fm) a[fm][s] = ar[pcol[fm]];
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn) b[fn][s] = br[jcol[fn]];
+      }
     }
   };
   auto step = [&]() {
@@ -199,7 +221,7 @@ __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
       if (kF32 && ((k0 + kChunk) % kFlushK) == 0) flush();
     }
   }
-  if (K16 < K) {  // last partial chunk: rows k >= K contribute zero
+  if (!kK4 && K16 < K) {  // last partial chunk: rows k >= K contribute zero (K4: K % 16 == 0)
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const int64_t k = K16 + 4 * g + s;
@@ -238,20 +260,20 @@ __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
     }
 }
 
-template <typename T>
+template <typename T, bool kK4 = false>
 int contract_launch(const T* A, int64_t lda, const T* B, int64_t ldb, const double* offset,
                     double* Y, int64_t ldy, int64_t P, int64_t n, int64_t K, const double* A64,
                     int64_t lda64, const double* B64, int64_t ldb64, int64_t K64, hipStream_t st) {
   const dim3 block(64 * kWaves);
   const unsigned gm = (unsigned)((P + kBlockM - 1) / kBlockM);
   if (n <= 32) {
-    hipLaunchKernelGGL((contract_kernel<T, 1>), dim3(gm, (unsigned)((n + 31) / 32)), block, 0, st,
+    hipLaunchKernelGGL((contract_kernel<T, 1, kK4>), dim3(gm, (unsigned)((n + 31) / 32)), block, 0, st,
                        A, lda, B, ldb, offset, Y, ldy, P, n, K, A64, lda64, B64, ldb64, K64);
   } else if (n <= 64) {
-    hipLaunchKernelGGL((contract_kernel<T, 2>), dim3(gm, (unsigned)((n + 63) / 64)), block, 0, st,
+    hipLaunchKernelGGL((contract_kernel<T, 2, kK4>), dim3(gm, (unsigned)((n + 63) / 64)), block, 0, st,
                        A, lda, B, ldb, offset, Y, ldy, P, n, K, A64, lda64, B64, ldb64, K64);
   } else {
-    hipLaunchKernelGGL((contract_kernel<T, 4>), dim3(gm, (unsigned)((n + 127) / 128)), block, 0,
+    hipLaunchKernelGGL((contract_kernel<T, 4, kK4>), dim3(gm, (unsigned)((n + 127) / 128)), block, 0,
                        st, A, lda, B, ldb, offset, Y, ldy, P, n, K, A64, lda64, B64, ldb64, K64);
   }
   SBI_HIP_CHECK(hipGetLastError());
@@ -420,6 +442,26 @@ extern "C" int sbi_contract(const void* A, int64_t lda, const void* B, int64_t l
                                   n, K, nullptr, 0, nullptr, 0, 0, st);
   sbi::set_error("sbi_contract: dtype must be 0 (f64) or 1 (f32)");
   return SBI_ERR_INVALID;
+}
+
+extern "C" int sbi_contract_split_k4(const double* A64, int64_t lda64, const double* B64,
+                                     int64_t ldb64, int64_t K64, const float* A32, int64_t lda32,
+                                     const float* B32, int64_t ldb32, int64_t K32,
+                                     const double* offset, double* Y, int64_t ldy, int64_t P,
+                                     int64_t n, void* stream) {
+  if (P < 0 || n < 0 || K64 < 0 || K32 < 0 || K32 % 16 != 0 ||
+      (P > 0 && n > 0 && (!Y || ldy < n)) ||
+      (P > 0 && n > 0 && K64 > 0 && (!A64 || !B64 || lda64 < P || ldb64 < n)) ||
+      (P > 0 && n > 0 && K32 > 0 &&
+       (!A32 || !B32 || lda32 < P || ldb32 < n || ((uintptr_t)A32 & 15) || ((uintptr_t)B32 & 15))) ||
+      P > (int64_t)65535 * kBlockM || n > (int64_t)65535 * 32) {
+    sbi::set_error("sbi_contract_split_k4: invalid argument (K32 must be a multiple of 16, the "
+                   "k-interleaved operands 16-byte aligned)");
+    return SBI_ERR_INVALID;
+  }
+  if (P == 0 || n == 0) return SBI_OK;
+  return contract_launch<float, true>(A32, lda32, B32, ldb32, offset, Y, ldy, P, n, K32, A64,
+                                      lda64, B64, ldb64, K64, (hipStream_t)stream);
 }
 
 extern "C" int sbi_contract_split(const double* A64, int64_t lda64, const double* B64,

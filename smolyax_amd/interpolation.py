@@ -421,10 +421,24 @@ class SmolyakBarycentricInterpolator:
             store[kind] = S
         return S
 
+    def _surpluses_k4(self, nh):
+        """FP32 surpluses of basis rows nh.. in the k-interleaved layout of sbi_contract_split_k4:
+        (K32 / 4, d_out, 4) with K32 = rows rounded up to a multiple of 16 (zero rows)."""
+        S = self._Sg.get("f32k4")
+        if S is None:
+            rows = self._surpluses("f32", gemm=True)[nh:]
+            k32 = -(-rows.shape[0] // 16) * 16
+            pad = torch.zeros((k32, self._d_out), dtype=torch.float32, device=self._device)
+            pad[: rows.shape[0]] = rows
+            S = pad.view(k32 // 4, 4, self._d_out).permute(0, 2, 1).contiguous()
+            self._Sg["f32k4"] = S
+        return S
+
     def _eval_newton_gemm(self, x, precision):
         """y = H S: Newton basis matrix (HIP kernel), then the contraction with the surpluses on
-        the matrix cores: sbi_contract for FP64 and FP32 (FP64 accumulation); BF16 operands use
-        the library GEMM (FP32 accumulation)."""
+        the matrix cores: sbi_contract for FP64; for FP32 ("mixed") the heaviest rows in FP64 and
+        the rest in FP32 with FP64 accumulation (sbi_contract_split_k4, k-interleaved operands);
+        BF16 operands use the library GEMM (FP32 accumulation)."""
         P = x.shape[0]
         Nn = self._nlayout.basis_n
         kind, code, dt = {"newton-fp64": ("f64", 0, torch.float64), "mixed": ("f32", 1, torch.float32),
@@ -433,30 +447,37 @@ class SmolyakBarycentricInterpolator:
         y = torch.empty((P, self._d_out), dtype=torch.float64, device=self._device)
         pc = self._point_chunk(Nn, torch.finfo(dt).bits // 8, P)
         nh = self._nlayout.n_head if kind == "f32" else 0
-        if nh:
+        k4 = kind == "f32"  # FP32 remainder in the k-interleaved layout (16-byte operand loads)
+        if k4:
             S64 = self._surpluses("f64", gemm=True)[:nh]
-            S32 = S[nh:]
+            S32 = self._surpluses_k4(nh)
+            K32 = S32.shape[0] * 4
         for a in range(0, P, pc):
             b = min(P, a + pc)
             xc = x[a:b]
-            HT = torch.empty((Nn - nh, b - a), dtype=dt, device=self._device)
+            if k4:
+                HT = torch.empty((K32 // 4, b - a, 4), dtype=dt, device=self._device)
+            else:
+                HT = torch.empty((Nn - nh, b - a), dtype=dt, device=self._device)
             if Nn > nh:
                 N.check(N.fn["sbi_newton_basis_rows"](self._nplan, xc.data_ptr(), b - a,
                                                       xc.stride(0), nh, Nn, HT.data_ptr(), b - a,
-                                                      code, self._stream()), "basis")
-            if kind == "bf16":
-                y[a:b] = torch.mm(HT.t(), S, out_dtype=torch.float32)
-            elif nh:  # FP64 head rows, FP32 remainder
-                H64 = torch.empty((nh, b - a), dtype=torch.float64, device=self._device)
-                N.check(N.fn["sbi_newton_rows_f64"](self._nplan, xc.data_ptr(), b - a,
-                                                    xc.stride(0), self._head_fac.data_ptr(), nh,
-                                                    H64.data_ptr(), b - a, self._stream()), "rows")
+                                                      3 if k4 else code, self._stream()), "basis")
+            if k4:  # FP64 head rows, FP32 remainder
+                H64 = torch.empty((max(nh, 1), b - a), dtype=torch.float64, device=self._device)
+                if nh:
+                    N.check(N.fn["sbi_newton_rows_f64"](self._nplan, xc.data_ptr(), b - a,
+                                                        xc.stride(0), self._head_fac.data_ptr(),
+                                                        nh, H64.data_ptr(), b - a,
+                                                        self._stream()), "rows")
                 yc = y[a:b]
-                N.check(N.fn["sbi_contract_split"](
-                    H64.data_ptr(), H64.stride(0), S64.data_ptr(), S64.stride(0), nh,
-                    HT.data_ptr() if Nn > nh else None, b - a,
-                    S32.data_ptr() if Nn > nh else None, self._d_out, Nn - nh, None,
-                    yc.data_ptr(), yc.stride(0), b - a, self._d_out, self._stream()), "contract")
+                N.check(N.fn["sbi_contract_split_k4"](
+                    H64.data_ptr(), H64.stride(0), S64.data_ptr(), max(S64.stride(0), self._d_out),
+                    nh, HT.data_ptr() if K32 else None, b - a, S32.data_ptr() if K32 else None,
+                    self._d_out, K32, None, yc.data_ptr(), yc.stride(0), b - a, self._d_out,
+                    self._stream()), "contract")
+            elif kind == "bf16":
+                y[a:b] = torch.mm(HT.t(), S, out_dtype=torch.float32)
             else:
                 self._contract(HT, S, None, y[a:b])
         return y

@@ -139,3 +139,34 @@ def test_contract_split_rows_independent_of_batch(gpu):
     for lo, hi in [(0, 1), (3, 4), (100, 211), (499, 500)]:
         Ys = contract_split(A64[:, lo:hi].contiguous(), B64, A32[:, lo:hi].contiguous(), B32)
         assert torch.equal(Ys, Y[lo:hi]), (lo, hi)
+
+
+def _k4(M, rows):
+    """(rows, m) plain layout -> k-interleaved (rows / 4, m, 4), rows a multiple of 16."""
+    return M.view(rows // 4, 4, M.shape[1]).permute(0, 2, 1).contiguous()
+
+
+@pytest.mark.parametrize("P,n,K64,K32", [(64, 32, 0, 16), (100, 70, 5, 48), (333, 257, 64, 1024),
+                                         (17, 3, 3, 0)])
+def test_contract_split_k4_equals_plain_layout(gpu, P, n, K64, K32):
+    """The k-interleaved operands give the same sums in the same order: bit-identical to
+    sbi_contract_split on the same values (K32 a multiple of 16, ragged P and n)."""
+    from smolyax_amd import _native as N
+
+    g = torch.Generator(device="cuda").manual_seed(P + n + K32)
+    A64 = torch.randn((max(K64, 1), P), dtype=torch.float64, device="cuda", generator=g)[:K64]
+    B64 = torch.randn((max(K64, 1), n), dtype=torch.float64, device="cuda", generator=g)[:K64]
+    A32 = torch.randn((max(K32, 16), P), dtype=torch.float32, device="cuda", generator=g)[:K32]
+    B32 = torch.randn((max(K32, 16), n), dtype=torch.float32, device="cuda", generator=g)[:K32]
+    off = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    Y0 = contract_split(A64.contiguous(), B64.contiguous(), A32.contiguous(), B32.contiguous(), off)
+    A4 = _k4(A32.contiguous(), K32) if K32 else A32
+    B4 = _k4(B32.contiguous(), K32) if K32 else B32
+    Y1 = torch.empty((P, n), dtype=torch.float64, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    N.check(N.fn["sbi_contract_split_k4"](
+        A64.data_ptr() if K64 else None, P, B64.data_ptr() if K64 else None, n, K64,
+        A4.data_ptr() if K32 else None, P, B4.data_ptr() if K32 else None, n, K32, off.data_ptr(),
+        Y1.data_ptr(), n, P, n, st), "contract_k4")
+    torch.cuda.synchronize()
+    assert torch.equal(Y0, Y1)
