@@ -218,6 +218,8 @@ typedef struct sbi_newton_desc {
                                  partial slot or -1 (the unit covers its dim)}; units of one dim
                                  are consecutive */
   const int32_t* jac_red;     /* [3 * n_jac_red] {dim, first partial slot, slot count} */
+  const int32_t* vjp_ent_row; /* [n_vjp_ent] factor-table row of every adjoint entry */
+  const int32_t* row_tdim;    /* [n_tab_rows] table dim index of every factor-table row */
 } sbi_newton_desc;
 
 typedef struct sbi_newton_plan sbi_newton_plan;

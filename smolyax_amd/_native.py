@@ -59,7 +59,7 @@ class NewtonDesc(ctypes.Structure):
         ("vjp_slot_off", c_vp), ("vjp_ent", c_vp),
         ("vjp_dim_off", c_vp), ("sing_off", c_vp), ("sing_x", c_vp),
         ("n_jac_units", c_i32), ("n_jac_partials", c_i32), ("n_jac_red", c_i32), ("_pad4", c_i32),
-        ("jac_unit", c_vp), ("jac_red", c_vp),
+        ("jac_unit", c_vp), ("jac_red", c_vp), ("vjp_ent_row", c_vp), ("row_tdim", c_vp),
     ]
 
 

@@ -236,17 +236,6 @@ __global__ __launch_bounds__(256) void newton_htab_kernel(sbi_newton_desc d,
   }
 }
 
-// Factor row of adjoint entry e (the last s < n_rows with off[s] <= e): a branchless binary
-// search, one entry per lane, so that a wave resolves 64 entries with ~log2(n_rows) loads.
-__device__ __forceinline__ int row_of_entry(const int32_t* __restrict__ off, int n_rows, int e) {
-  int lo = 0;
-  for (int step = 1 << (31 - __builtin_clz(max(n_rows, 1))); step > 0; step >>= 1) {
-    const int c = lo + step;
-    if (c < n_rows && off[c] <= e) lo = c;
-  }
-  return lo;
-}
-
 // h'_k(x_j) of factor row s = (j, k) of table dim i (row s = trow[i] + k - 1).
 __device__ __forceinline__ double row_derivative(const sbi_newton_desc& d, const double* xp, int j,
                                                  int k) {
@@ -304,7 +293,7 @@ __global__ __launch_bounds__(kTile* kBWaves) void newton_jac_kernel(
       const int mine = base + lane;
       const uint4 ev = (mine < e1) ? ent[mine]
                                    : make_uint4(0u, 0xffffffffu, 0xffffffffu, 0xffffffffu);
-      const int rl = row_of_entry(d.vjp_slot_off, d.n_tab_rows, min(mine, e1 - 1));
+      const int rl = d.vjp_ent_row[min(mine, e1 - 1)];
       const int cnt = min(64, e1 - base);
       // adjoint: the per-point coefficients of kPre entries are requested together, ahead of
       // their use (independent loads in flight instead of one at a time)
@@ -382,7 +371,8 @@ __global__ __launch_bounds__(256) void jac_reduce_kernel(sbi_newton_desc d, int6
 
 // D^T[e, p] for all adjoint entries, in groups of 64 entries per wave; the workgroups of a tile
 // (grid.y of them, kDTWaves waves each) share out the groups.  The factor row of each entry and
-// its (dim, deg) are found lane-parallel per group; h'_k(x_j) is recomputed when the row changes.
+// its table dim are read per lane from the plan's tables; h'_k(x_j) is recomputed when the row
+// changes.
 constexpr int kDTWaves = 8;
 
 template <bool kG>
@@ -411,8 +401,8 @@ __global__ __launch_bounds__(kTile* kDTWaves) void newton_jac_basis_kernel(
     const int base = gi * 64;
     const int mine = min(base + lane, n_ent - 1);
     const uint4 ev = ent[mine];
-    const int rl = row_of_entry(d.vjp_slot_off, d.n_tab_rows, mine);
-    const int il = row_of_entry(d.tab_row, d.n_tab_dims, rl);  // table dim of the row
+    const int rl = d.vjp_ent_row[mine];
+    const int il = d.row_tdim[rl];  // table dim of the row
     const int kl = rl - d.tab_row[il] + 1;
     const int cnt = min(64, n_ent - base);
     int s_done = -1;
@@ -700,7 +690,8 @@ extern "C" int sbi_newton_vjp(const sbi_newton_plan* plan, const double* x, int6
     return SBI_ERR_INVALID;
   }
   const sbi_newton_desc& d = *dp;
-  if (!d.vjp_ent || !d.vjp_slot_off || !d.tab_dim || !d.jac_unit || !d.jac_red || ldx < d.d_in) {
+  if (!d.vjp_ent || !d.vjp_slot_off || !d.tab_dim || !d.jac_unit || !d.jac_red ||
+      !d.vjp_ent_row || ldx < d.d_in) {
     sbi::set_error("sbi_newton_vjp: plan has no adjoint tables");
     return SBI_ERR_INVALID;
   }
@@ -828,7 +819,7 @@ extern "C" int sbi_newton_jacobian_f64(const sbi_newton_plan* plan, const double
   }
   const sbi_newton_desc& d = *dp;
   if (!d.vjp_ent || !d.vjp_slot_off || !d.tab_dim || !d.vjp_dim_off || !d.sing_off ||
-      !d.sing_x || !d.jac_unit || !d.jac_red || ldx < d.d_in) {
+      !d.sing_x || !d.jac_unit || !d.jac_red || !d.vjp_ent_row || !d.row_tdim || ldx < d.d_in) {
     sbi::set_error("sbi_newton_jacobian_f64: plan has no Jacobian tables");
     return SBI_ERR_INVALID;
   }

@@ -253,6 +253,8 @@ class SmolyakBarycentricInterpolator:
             "vjp_dim_off": _dev_tensor(NL.vjp_dim_off, i32, dev),
             "jac_unit": _dev_tensor(NL.jac_unit, i32, dev),
             "jac_red": _dev_tensor(NL.jac_red, i32, dev),
+            "vjp_ent_row": _dev_tensor(NL.vjp_ent_row, i32, dev),
+            "row_tdim": _dev_tensor(NL.row_tdim, i32, dev),
         }
         t["sing_off"], t["sing_x"] = (_dev_tensor(a, i32 if a.dtype == np.int32 else f64, dev)
                                       for a in self._singular_coordinates())

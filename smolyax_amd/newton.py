@@ -356,6 +356,13 @@ def build_basis_tables(nf: NewtonForm, L: NewtonLayout) -> None:
         off.append(len(ent))
     L.vjp_slot_off = np.array(off, dtype=np.int32)
     L.n_vjp_ent = len(ent)
+    L.vjp_ent_row = np.repeat(np.arange(max(r, 1), dtype=np.int32),
+                              np.diff(np.asarray(off)) if r else [0]).astype(np.int32)
+    if not len(L.vjp_ent_row):
+        L.vjp_ent_row = np.zeros(1, dtype=np.int32)
+    L.row_tdim = np.zeros(max(r, 1), dtype=np.int32)
+    for i, j in enumerate(tab_dims):
+        L.row_tdim[row0[j]: row0[j] + int(nf.maxdeg[j])] = i
     # entries are stored by factor row, rows by dim: CSR of the entries over input dims
     dim_off = np.zeros(nf.d_in + 1, dtype=np.int64)
     for j in tab_dims:

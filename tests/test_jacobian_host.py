@@ -94,6 +94,11 @@ def test_newton_jacobian_tables_match_oracle(d, d_out, m):
     assert L.vjp_dim_off[0] == 0 and L.vjp_dim_off[-1] == L.n_vjp_ent
     assert np.all(np.diff(L.vjp_dim_off) >= 0)
     assert L.n_vjp_ent == sum(len(nu) for nu in nf.nus)
+    # factor row of every entry and table dim of every row, as the kernels read them
+    rows = np.repeat(np.arange(L.n_tab_rows), np.diff(L.vjp_slot_off))
+    assert np.array_equal(L.vjp_ent_row[: L.n_vjp_ent], rows)
+    for i in range(L.n_tab_dims):
+        assert np.all(L.row_tdim[L.tab_row[i]: L.tab_row[i] + L.tab_kmax[i]] == i)
     units = L.jac_unit.reshape(-1, 4)[: L.n_jac_units]
     assert np.all(units[:, 2] - units[:, 1] <= NW.JAC_UNIT)
     assert np.array_equal(np.sort(np.concatenate([np.arange(a, b) for _, a, b, _ in units])),
