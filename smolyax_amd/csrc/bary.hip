@@ -30,6 +30,7 @@
 
 #include "../../include/smolyax_amd.h"
 #include "common.h"
+#include "launch.h"
 
 namespace sbi {
 
@@ -574,11 +575,7 @@ static int launch_eval(const sbi_plan* plan, const double* x, int64_t P, int64_t
                        hipStream_t st) {
   const sbi_bary_desc& d = plan->bary;
   dim3 grid((unsigned)((P + kTile - 1) / kTile), (unsigned)((d.d_out + OC - 1) / OC));
-  static std::once_flag once;
-  std::call_once(once, [] {
-    (void)hipFuncSetAttribute((const void*)bary_eval_kernel<OC>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
-  });
+  sbi::allow_lds((const void*)bary_eval_kernel<OC>);
   hipLaunchKernelGGL(bary_eval_kernel<OC>, grid, dim3(kTile * kWaves), plan->eval_lds_bytes, st, d,
                      x, P, ldx, y);
   SBI_HIP_CHECK(hipGetLastError());
@@ -628,12 +625,7 @@ extern "C" int sbi_bary_jacobian_f64(const sbi_plan* plan, const double* x, int6
     return SBI_ERR_UNSUPPORTED;
   }
   const size_t lds = (size_t)(d.max_n + 1) * d.dmax1 * kTile * sizeof(double);
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void*)bary_jacobian_kernel,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr = true;
-  }
+  sbi::allow_lds((const void*)bary_jacobian_kernel);
   hipLaunchKernelGGL(bary_jacobian_kernel, grid, dim3(64), lds, (hipStream_t)stream, d, x, P, ldx,
                      J);
   SBI_HIP_CHECK(hipGetLastError());

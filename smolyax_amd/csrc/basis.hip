@@ -30,6 +30,7 @@
 
 #include "../../include/smolyax_amd.h"
 #include "common.h"
+#include "launch.h"
 #include "gemm.h"
 
 namespace {
@@ -533,12 +534,7 @@ static int basis_launch(const sbi_newton_desc& d, const double* x, int64_t P, in
     sbi::set_error("sbi_newton_basis: the tile's factor table exceeds LDS");
     return SBI_ERR_CAPACITY;
   }
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void*)newton_basis_kernel<T, OUT, kQ>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
-    attr = true;
-  }
+  sbi::allow_lds((const void*)newton_basis_kernel<T, OUT, kQ>);
   const int chunk = 2048;
   if (row1 <= row0) return SBI_OK;
   const int rows = kQ ? ((row1 - row0 + 15) & ~15) : row1 - row0;
@@ -650,22 +646,12 @@ static int vjp_launch(const sbi_newton_desc& d, const double* x, int64_t P, int6
         1, std::min<int64_t>((d.n_jac_units + kBWaves - 1) / kBWaves, (4096 + tiles - 1) / tiles));
     const dim3 grid((unsigned)tiles, (unsigned)ny);
     if (g) {
-      static bool attr = false;
-      if (!attr) {
-        (void)hipFuncSetAttribute((const void*)newton_jac_kernel<1, true, T>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
-        attr = true;
-      }
+      sbi::allow_lds((const void*)newton_jac_kernel<1, true, T>);
       hipLaunchKernelGGL((newton_jac_kernel<1, true, T>), grid, dim3(kTile * kBWaves), lds_bytes,
                          st, d, x, P, ldx, nullptr, (int64_t)0, 1, gtab, (int64_t)d.n_tab_rows,
                          gx, part, (const void*)GT, ldg);
     } else {
-      static bool attr = false;
-      if (!attr) {
-        (void)hipFuncSetAttribute((const void*)newton_jac_kernel<1, false, T>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
-        attr = true;
-      }
+      sbi::allow_lds((const void*)newton_jac_kernel<1, false, T>);
       hipLaunchKernelGGL((newton_jac_kernel<1, false, T>), grid, dim3(kTile * kBWaves), lds_bytes,
                          st, d, x, P, ldx, nullptr, (int64_t)0, 1, gtab, (int64_t)d.n_tab_rows,
                          gx, part, (const void*)GT, ldg);
@@ -772,12 +758,7 @@ static void jac_direct_launch(const sbi_newton_desc& d, const double* x, int64_t
                               const double* S, int64_t lds, int n, const double* gtab,
                               double* J, double* part, hipStream_t st) {
   const size_t lds_bytes = kG ? 0 : (size_t)d.n_tab_rows * kTile * sizeof(double);
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void*)newton_jac_kernel<NO, kG>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
-    attr = true;
-  }
+  sbi::allow_lds((const void*)newton_jac_kernel<NO, kG>);
   const int64_t tiles = (P + kTile - 1) / kTile;
   // workgroups per tile: enough for ~4096 workgroups, at most one wave per work unit
   const int64_t ny = std::max<int64_t>(
@@ -791,12 +772,7 @@ template <bool kG>
 static void jac_basis_launch(const sbi_newton_desc& d, const double* x, int64_t P, int64_t ldx,
                              const double* gtab, double* DT, hipStream_t st) {
   const size_t lds_bytes = kG ? 0 : (size_t)d.n_tab_rows * kTile * sizeof(double);
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void*)newton_jac_basis_kernel<kG>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
-    attr = true;
-  }
+  sbi::allow_lds((const void*)newton_jac_basis_kernel<kG>);
   const int64_t tiles = (P + kTile - 1) / kTile;
   const int64_t ngrp = (d.n_vjp_ent + 63) / 64;
   // the factor table is built once per workgroup: ~1024 workgroups, each wave looping over groups

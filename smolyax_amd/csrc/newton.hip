@@ -23,6 +23,7 @@
 
 #include "../../include/smolyax_amd.h"
 #include "common.h"
+#include "launch.h"
 
 namespace {
 
@@ -522,14 +523,8 @@ static int newton_launch(const sbi_newton_plan* plan, const double* x, int64_t P
     return SBI_ERR_CAPACITY;
   }
   if (P == 0) return SBI_OK;
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void*)newton_eval_kernel<float>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
-    (void)hipFuncSetAttribute((const void*)newton_eval_kernel<double>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
-    attr = true;
-  }
+  sbi::allow_lds((const void*)newton_eval_kernel<float>);
+  sbi::allow_lds((const void*)newton_eval_kernel<double>);
   const sbi_newton_desc& d = plan->d;
   dim3 grid((unsigned)((P + kTile - 1) / kTile));
   hipLaunchKernelGGL(newton_eval_kernel<T>, grid, dim3(kTile * kWaves), lds, (hipStream_t)stream, d,
@@ -546,14 +541,8 @@ extern "C" int sbi_newton_eval_mixed(const sbi_newton_plan* plan, const double* 
   }
   if (plan->lds_staged > (size_t)kMaxLds) return newton_launch<float>(plan, x, P, ldx, y, stream);
   if (P == 0) return SBI_OK;
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute((const void*)newton_mixed_staged_kernel<true>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
-    (void)hipFuncSetAttribute((const void*)newton_mixed_staged_kernel<false>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLds);
-    attr = true;
-  }
+  sbi::allow_lds((const void*)newton_mixed_staged_kernel<true>);
+  sbi::allow_lds((const void*)newton_mixed_staged_kernel<false>);
   dim3 grid((unsigned)((P + kTile - 1) / kTile));
   // 16-byte coordinate loads need an even row stride, a 16-byte aligned x and d_in >= 2
   const bool pairs = (ldx % 2 == 0) && ((uintptr_t)x % 16 == 0) && plan->d.d_in >= 2;
