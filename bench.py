@@ -264,8 +264,15 @@ def _log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def bench_serving(torch, W, n_requests=600, qps=None, seed=0):
-    """Serving workload: 32 resident Gauss-Hermite interpolants, Poisson arrivals, FP32."""
+SERVE_QPS = 10000.0  # fixed Poisson arrival rate of the headline latency figures (requests/s)
+
+
+def bench_serving(torch, W, n_requests=600, qps=SERVE_QPS, seed=0):
+    """Serving workload: 32 resident Gauss-Hermite interpolants, Poisson arrivals, FP32.
+
+    For each launch mode (direct, HIP graphs, adaptive): one untimed warm-up pass, the capacity
+    (all requests submitted at once), then p50/p99 under Poisson arrivals at `qps`.  The adaptive
+    mode also reports p50/p99 at 25 %, 50 % and 70 % of its capacity (latency against load)."""
     from smolyax_amd import serving
 
     ts = time.perf_counter()
@@ -276,49 +283,59 @@ def bench_serving(torch, W, n_requests=600, qps=None, seed=0):
         torch.cuda.synchronize()
         _log(f"serve: first evaluation of {n} done")
     setup_s = time.perf_counter() - ts
+    inputs = None
 
-    def run(graphs, inputs=None, load_qps=None):
+    def poisson(srv, q):
+        nonlocal inputs
+        srv.completed = []
+        reqs, inputs = serving.run_poisson(srv, names, qps=q, n_requests=n_requests, seed=seed,
+                                           inputs=inputs, time_limit_s=120.0, progress=_log)
+        return srv.stats(reqs)
+
+    def run(graphs, curve=False):
         srv = serving.Server(ips, precision="mixed", n_streams=4, graphs=graphs)
         if graphs is not False:
             t0 = time.perf_counter()
             srv.prewarm()
             _log(f"serve: {sum(len(sl.graphs) for sl in srv.slots)} graphs captured in "
                  f"{time.perf_counter() - t0:.1f} s")
-        # capacity: all requests submitted at once (closed loop), then Poisson arrivals at 70 %.
-        # One untimed pass first: the first pass of a server pays one-time costs (device
-        # allocations of the batch buffers, first replays) that a long-running server does not.
-        _, inputs = serving.run_poisson(srv, names, qps=1e9, n_requests=n_requests, seed=seed,
-                                        inputs=inputs, time_limit_s=120.0, progress=_log)
-        srv.completed = []
-        reqs, inputs = serving.run_poisson(srv, names, qps=1e9, n_requests=n_requests, seed=seed,
-                                           inputs=inputs, time_limit_s=120.0, progress=_log)
-        cap = srv.stats(reqs)
-        _log(f"serve (graphs={graphs}): capacity {cap['requests'] / cap['span_s']:.0f} requests/s")
-        q = load_qps if load_qps is not None else 0.7 * cap["requests"] / cap["span_s"]
-        srv.completed = []
-        reqs, _ = serving.run_poisson(srv, names, qps=q, n_requests=n_requests, seed=seed,
-                                      inputs=inputs, time_limit_s=120.0, progress=_log)
-        st = srv.stats(reqs)
-        _log(f"serve (graphs={graphs}): p50 {st['p50_ms']:.3f} ms p99 {st['p99_ms']:.3f} ms")
-        return cap, st, q, inputs
+        # the first pass of a server pays one-time costs (device allocations of the batch buffers,
+        # first replays) that a long-running server does not
+        poisson(srv, 1e9)
+        cap = poisson(srv, 1e9)  # closed loop: every request queued at once
+        cap_qps = cap["requests"] / cap["span_s"]
+        _log(f"serve (graphs={graphs}): capacity {cap_qps:.0f} requests/s")
+        st = poisson(srv, qps)
+        _log(f"serve (graphs={graphs}): {qps:.0f} requests/s: p50 {st['p50_ms']:.3f} ms "
+             f"p99 {st['p99_ms']:.3f} ms")
+        out = {"capacity_qps": cap_qps, "capacity_point_evals_per_s": cap["point_evals_per_s"],
+               "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+               "point_evals_per_s": st["point_evals_per_s"], "requests": st["requests"]}
+        if curve:
+            out["load_curve"] = []
+            for frac in (0.25, 0.5, 0.7):
+                c = poisson(srv, frac * cap_qps)
+                _log(f"serve (graphs={graphs}): {frac:.0%} of capacity: p50 {c['p50_ms']:.3f} ms "
+                     f"p99 {c['p99_ms']:.3f} ms")
+                out["load_curve"].append({"qps": frac * cap_qps, "p50_ms": c["p50_ms"],
+                                          "p99_ms": c["p99_ms"]})
+        return out
 
-    cap_e, st_e, q_e, inputs = run(False)
-    q = qps if qps is not None else q_e
-    cap_g, st_g, _, _ = run(True, inputs=inputs, load_qps=q)
-    cap, st, load_qps, _ = run("auto", inputs=inputs, load_qps=q)
-    return {"qps": load_qps, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
-            "mean_ms": st["mean_ms"], "point_evals_per_s": st["point_evals_per_s"],
-            "requests": st["requests"], "capacity_point_evals_per_s": cap["point_evals_per_s"],
-            "capacity_qps": cap["requests"] / cap["span_s"],
-            "launches": "HIP graphs under a backlog, direct launches otherwise",
-            "eager": {"p50_ms": st_e["p50_ms"], "p99_ms": st_e["p99_ms"],
-                      "capacity_qps": cap_e["requests"] / cap_e["span_s"]},
-            "graphs": {"p50_ms": st_g["p50_ms"], "p99_ms": st_g["p99_ms"],
-                       "capacity_qps": cap_g["requests"] / cap_g["span_s"]},
-            "precision": "mixed (FP32)",
-            "interpolants": len(names), "setup_s": setup_s,
-            "workload": "32 GH interpolants, d_in logU[8,500], nodes logU[1e2,2e4], d_out "
-                        "logU[1,1024], batch logU[1,4096], Poisson arrivals at 70% of capacity"}
+    e = run(False)
+    g = run(True)
+    a = run("auto", curve=True)
+    res = {"qps": qps, "p50_ms": a["p50_ms"], "p99_ms": a["p99_ms"], "mean_ms": a["mean_ms"],
+           "point_evals_per_s": a["point_evals_per_s"], "requests": a["requests"],
+           "capacity_qps": a["capacity_qps"],
+           "capacity_point_evals_per_s": a["capacity_point_evals_per_s"],
+           "load_curve": a["load_curve"],
+           "launches": "HIP graphs under a backlog, direct launches otherwise",
+           "eager": {k: e[k] for k in ("p50_ms", "p99_ms", "capacity_qps")},
+           "graphs": {k: g[k] for k in ("p50_ms", "p99_ms", "capacity_qps")},
+           "precision": "mixed (FP32)", "interpolants": len(names), "setup_s": setup_s,
+           "workload": "32 GH interpolants, d_in logU[8,500], nodes logU[1e2,2e4], d_out "
+                       "logU[1,1024], batch logU[1,4096], Poisson arrivals at qps requests/s"}
+    return res
 
 
 def main():
