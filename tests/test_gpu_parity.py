@@ -127,6 +127,28 @@ def test_gradient_nan_at_nodes(gpu):
     assert rel(J[ok], Jr[ok]) <= 1e-12
 
 
+def test_gradient_nan_at_gauss_hermite_nodes(gpu):
+    """Non-nested nodes (Newton form on the Leja proxy grid): NaN exactly where x_j is a node of a
+    Gauss-Hermite degree that some block uses in dim j; equal to the oracle elsewhere."""
+    d, d_out = 3, 2
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((d_out, d))
+    f = lambda x: np.sin(A @ x)
+    ref, ip, ong = build_pair(d, d_out, 60, f, kind="gh")
+    x = sample(ong, 30, seed=3)
+    p = 0
+    for j in range(d):
+        for deg in range(0, 6):
+            x[p, j] = ong[j](deg)[deg // 2]
+            p += 1
+    J = ip.gradient(x).cpu().numpy()
+    Jr = ref.gradient(x)
+    assert np.array_equal(np.isnan(J), np.isnan(Jr))
+    assert np.isnan(Jr).any() and not np.isnan(Jr).all()
+    ok = ~np.isnan(Jr)
+    assert rel(J[ok], Jr[ok]) <= 1e-11, rel(J[ok], Jr[ok])
+
+
 @pytest.mark.parametrize("d_out", [1, 12])
 def test_gradient_global_factor_table(gpu, d_out):
     """400 input dims: the Newton factor table (369 rows) exceeds LDS and is kept in global

@@ -137,3 +137,42 @@ def test_singular_coordinates_give_the_oracle_nan_pattern():
             if j == p % d:
                 seen.add(in_table)
     assert seen == {True, False}  # node coordinates inside and beyond the table both occur
+
+
+def test_singular_coordinates_gauss_hermite():
+    """Non-nested nodes: every degree has its own node set.  x_j set to a node of a degree that
+    some block uses in dim j gives NaN in the oracle's J[p, :, j]; a node of an unused degree does
+    not; the table reproduces both."""
+    from smolyax_amd import nodes as SN
+    from smolyax_amd import packing
+    from smolyax_amd.interpolation import SmolyakBarycentricInterpolator as SBI
+
+    d, d_out = 3, 2
+    rng = np.random.default_rng(5)
+    k = np.log((2 + np.arange(d)) / np.log(2))
+    ong, sng = ON.gauss_hermite(dim=d), SN.GaussHermite(dim=d)
+    t = OI.find_approximate_threshold(k, 60, False)
+    A = rng.standard_normal((d_out, d))
+    ref = OS.Interpolant(d_out=d_out, node_gen=ong, k=k, t=t, f=lambda x: np.sin(A @ x))
+    L = packing.build_layout(packing.nonzero_blocks(k, t), sng, d, d_out)
+    off, vals = SBI._singular_coordinates(types.SimpleNamespace(_layout=L, _d_in=d))
+    x = rng.standard_normal((40, d)) / np.sqrt(2)
+    used = {}
+    for dim, deg in L.slot_key:
+        used.setdefault(dim, set()).add(deg)
+    p = 0
+    for j in range(d):
+        for deg in range(0, 7):
+            if p >= 40:
+                break
+            x[p, j] = sng[j](deg)[deg // 2]
+            p += 1
+    Jr = ref.gradient(x)
+    seen = set()
+    for q in range(40):
+        for j in range(d):
+            in_table = bool(np.any((x[q, j] - vals[off[j]:off[j + 1]]) ** 2 == 0.0))
+            assert np.isnan(Jr[q, :, j]).all() == in_table, (q, j)
+            assert np.isnan(Jr[q, :, j]).any() == in_table, (q, j)
+            seen.add(in_table)
+    assert seen == {True, False}
