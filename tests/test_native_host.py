@@ -1,6 +1,7 @@
 """The native library: it loads, exports every declared symbol, and its host-side index machinery
 is bit-identical to the oracle (no GPU needed)."""
 
+import json
 import os
 import re
 import subprocess
@@ -61,13 +62,29 @@ def test_native_indices_bit_exact(d, m, nested):
     assert list(a[0].keys()) == list(b[0].keys())
 
 
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                      "reference_known_answers.json")))
+
+
 def test_native_known_cardinalities():
+    """tests/test_indices_runtime.py:10-12 of the reference, on the C++ port."""
     from smolyax_amd import indices as SI
 
-    k = np.log([2 + i for i in range(10000)]) / np.log(2)
-    for t, m in [(5.3, 98), (7.78, 1000), (10.317, 10003)]:
-        assert SI.indexset_cardinality(k, t) == m
-        assert len(SI.indexset(k, t)) == m
+    for case in GOLDEN["indexset_cardinality"]:
+        k = np.log([2 + i for i in range(case["d"])]) / np.log(2)
+        assert SI.indexset_cardinality(k, case["t"]) == case["m"]
+        assert len(SI.indexset(k, case["t"])) == case["m"]
+
+
+@pytest.mark.parametrize("case", GOLDEN["find_approximate_threshold"])
+def test_native_known_threshold_accuracy(case):
+    """tests/test_indices_runtime.py:21-34 of the reference, on the C++ port (d up to 10000)."""
+    from smolyax_amd import indices as SI
+
+    k = np.log([2 + i for i in range(case["d"])]) / np.log(2)
+    t = SI.find_approximate_threshold(k, case["m"], case["nested"], accuracy=case["accuracy"])
+    n = SI.nodeset_cardinality(k, t, case["nested"])
+    assert abs(n - case["m"]) / case["m"] < case["accuracy"]
 
 
 def test_native_smolyak_coefficient_random():

@@ -1,6 +1,8 @@
 """The oracle's multi-index machinery against brute force and the reference's known answers."""
 
 import itertools as it
+import json
+import os
 
 import numpy as np
 import pytest
@@ -12,12 +14,27 @@ def k_log2(d):
     return np.log([2 + i for i in range(d)]) / np.log(2)
 
 
-@pytest.mark.parametrize("t,m", [(5.3, 98), (7.78, 1000), (10.317, 10003)])
-def test_known_cardinalities(t, m):
-    """tests/test_indices_runtime.py:10-12 of the reference (d = 10000, k_j = log2(2+j))."""
-    k = k_log2(10000)
-    assert OI.indexset_cardinality(k, t) == m
-    assert len(OI.indexset(k, t)) == m
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                      "reference_known_answers.json")))
+
+
+@pytest.mark.parametrize("case", GOLDEN["indexset_cardinality"])
+def test_known_cardinalities(case):
+    """tests/test_indices_runtime.py:10-12 of the reference (k_j = log2(2+j))."""
+    k = k_log2(case["d"])
+    assert OI.indexset_cardinality(k, case["t"]) == case["m"]
+    assert len(OI.indexset(k, case["t"])) == case["m"]
+
+
+@pytest.mark.parametrize("case", [c for c in GOLDEN["find_approximate_threshold"] if c["d"] <= 100])
+def test_known_threshold_accuracy(case):
+    """tests/test_indices_runtime.py:21-34 of the reference: the threshold's node count is within
+    the requested accuracy of the target (the d = 10000 cases run on the C++ port)."""
+    k = k_log2(case["d"])
+    t = OI.find_approximate_threshold(k, case["m"], nested=case["nested"],
+                                      accuracy=case["accuracy"])
+    n = OI.nodeset_cardinality(k, t, nested=case["nested"])
+    assert abs(n - case["m"]) / case["m"] < case["accuracy"]
 
 
 def random_k(rng):
