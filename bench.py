@@ -33,7 +33,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="c2_exp_d1000")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-points", type=int, default=1024)
+    ap.add_argument("--cpu-points", type=int, default=8192)
     ap.add_argument("--oracle-points", type=int, default=256)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-requests", type=int, default=600)
@@ -457,8 +457,8 @@ def main():
         c1 = time.perf_counter()
         cpu = {"value": nc * reps / (c1 - c0), "unit": "point-evals/s", "cores": CB.threads(),
                "kind": "port",
-               "sample": f"{nc} of the {P} points x {reps} calls, FP64 torch-CPU restatement of the "
-                         f"reference's padded-group einsum evaluation"}
+               "sample": f"{nc} of the {P} points x {reps} calls ({c1 - c0:.1f} s), FP64 torch-CPU "
+                         f"restatement of the reference's padded-group einsum evaluation"}
 
     serve = None if args.no_serve else bench_serving(torch, W, n_requests=args.serve_requests)
     value = results["mixed"]["value"]
