@@ -1,8 +1,8 @@
 """FP64 CPU oracle: a plain numpy/Python restatement of the reference smolyax algorithms.
 
 This package is test infrastructure. Only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` import it, and only as the checker. The product package
-``smolyax_amd`` never imports it and has no CPU fallback.
+``cpu_baseline`` leg of ``bench.py`` import it, and only to check or time results against.
+The product package ``smolyax_amd`` never imports it and has no CPU fallback.
 
 Modules
 -------
