@@ -237,6 +237,29 @@ __global__ __launch_bounds__(256) void newton_htab_kernel(sbi_newton_desc d,
   }
 }
 
+// v times the lane's table values at the other factor rows of adjoint entry q of the wave's 64
+// (16-bit rows packed first in ev.y, ev.z, ev.w, 0xffff after the last): the rows are read and
+// tested only as far as the entry has factors, in the stored order.
+template <bool kG>
+__device__ __forceinline__ double times_other_factors(const HTab<kG>& H, const uint4& ev, int q,
+                                                      int lane, int64_t p, double v) {
+  const uint32_t w0 = (uint32_t)__builtin_amdgcn_readlane((int)ev.y, q);
+  if ((w0 & 0xffff) == 0xffff) return v;
+  v *= H.at((int)(w0 & 0xffff), lane, p);
+  if ((w0 >> 16) == 0xffff) return v;
+  v *= H.at((int)(w0 >> 16), lane, p);
+  const uint32_t w1 = (uint32_t)__builtin_amdgcn_readlane((int)ev.z, q);
+  if ((w1 & 0xffff) == 0xffff) return v;
+  v *= H.at((int)(w1 & 0xffff), lane, p);
+  if ((w1 >> 16) == 0xffff) return v;
+  v *= H.at((int)(w1 >> 16), lane, p);
+  const uint32_t w2 = (uint32_t)__builtin_amdgcn_readlane((int)ev.w, q);
+  if ((w2 & 0xffff) == 0xffff) return v;
+  v *= H.at((int)(w2 & 0xffff), lane, p);
+  if ((w2 >> 16) == 0xffff) return v;
+  return v * H.at((int)(w2 >> 16), lane, p);
+}
+
 // h'_k(x_j) of factor row s = (j, k) of table dim i (row s = trow[i] + k - 1).
 __device__ __forceinline__ double row_derivative(const sbi_newton_desc& d, const double* xp, int j,
                                                  int k) {
@@ -319,15 +342,7 @@ __global__ __launch_bounds__(kTile* kBWaves) void newton_jac_kernel(
             dh = row_derivative(d, xp, j, s - row0 + 1);
             s_done = s;
           }
-          const uint32_t ws[3] = {(uint32_t)__builtin_amdgcn_readlane((int)ev.y, q),
-                                  (uint32_t)__builtin_amdgcn_readlane((int)ev.z, q),
-                                  (uint32_t)__builtin_amdgcn_readlane((int)ev.w, q)};
-          double v = dh;
-#pragma unroll
-          for (int f = 0; f < 6; ++f) {
-            const uint32_t r = (ws[f >> 1] >> ((f & 1) * 16)) & 0xffff;
-            if (r != 0xffff) v *= H.at((int)r, lane, p);
-          }
+          const double v = times_other_factors(H, ev, q, lane, p, dh);
           if constexpr (std::is_void_v<TG>) {
             const int node = __builtin_amdgcn_readlane((int)ev.x, q);
             cptr<double> Sn = C(S) + (int64_t)node * lds;
@@ -416,15 +431,7 @@ __global__ __launch_bounds__(kTile* kDTWaves) void newton_jac_basis_kernel(
         dh = row_derivative(d, xp, tdim[i], k);
         s_done = s;
       }
-      const uint32_t ws[3] = {(uint32_t)__builtin_amdgcn_readlane((int)ev.y, q),
-                              (uint32_t)__builtin_amdgcn_readlane((int)ev.z, q),
-                              (uint32_t)__builtin_amdgcn_readlane((int)ev.w, q)};
-      double v = dh;
-#pragma unroll
-      for (int f = 0; f < 6; ++f) {
-        const uint32_t r = (ws[f >> 1] >> ((f & 1) * 16)) & 0xffff;
-        if (r != 0xffff) v *= H.at((int)r, lane, p);
-      }
+      const double v = times_other_factors(H, ev, q, lane, p, dh);
       if (valid) DT[(int64_t)e * ldd + p] = v;
     }
   }
