@@ -237,27 +237,34 @@ __global__ __launch_bounds__(256) void newton_htab_kernel(sbi_newton_desc d,
   }
 }
 
-// v times the lane's table values at the other factor rows of adjoint entry q of the wave's 64
-// (16-bit rows packed first in ev.y, ev.z, ev.w, 0xffff after the last): the rows are read and
-// tested only as far as the entry has factors, in the stored order.
+// Other factor rows of an adjoint entry: 16-bit rows packed first in ev.y, ev.z, ev.w, 0xffff
+// after the last.  other_factor_count gives each lane's count (once per 64 entries);
+// times_other_factors multiplies v by the lane's table values at the nf rows of entry q of the
+// wave's 64, in the stored order, with the nf loads issued together.
+__device__ __forceinline__ int other_factor_count(const uint4& ev) {
+  return (int)((ev.y & 0xffff) != 0xffff) + (int)((ev.y >> 16) != 0xffff) +
+         (int)((ev.z & 0xffff) != 0xffff) + (int)((ev.z >> 16) != 0xffff) +
+         (int)((ev.w & 0xffff) != 0xffff) + (int)((ev.w >> 16) != 0xffff);
+}
+
 template <bool kG>
 __device__ __forceinline__ double times_other_factors(const HTab<kG>& H, const uint4& ev, int q,
-                                                      int lane, int64_t p, double v) {
+                                                      int nf, int lane, int64_t p, double v) {
+  if (nf == 0) return v;
   const uint32_t w0 = (uint32_t)__builtin_amdgcn_readlane((int)ev.y, q);
-  if ((w0 & 0xffff) == 0xffff) return v;
-  v *= H.at((int)(w0 & 0xffff), lane, p);
-  if ((w0 >> 16) == 0xffff) return v;
-  v *= H.at((int)(w0 >> 16), lane, p);
+  const double a = H.at((int)(w0 & 0xffff), lane, p);
+  if (nf == 1) return v * a;
+  const double b = H.at((int)(w0 >> 16), lane, p);
+  if (nf == 2) return (v * a) * b;
   const uint32_t w1 = (uint32_t)__builtin_amdgcn_readlane((int)ev.z, q);
-  if ((w1 & 0xffff) == 0xffff) return v;
-  v *= H.at((int)(w1 & 0xffff), lane, p);
-  if ((w1 >> 16) == 0xffff) return v;
-  v *= H.at((int)(w1 >> 16), lane, p);
+  const double c = H.at((int)(w1 & 0xffff), lane, p);
+  if (nf == 3) return ((v * a) * b) * c;
+  const double e = H.at((int)(w1 >> 16), lane, p);
+  if (nf == 4) return (((v * a) * b) * c) * e;
   const uint32_t w2 = (uint32_t)__builtin_amdgcn_readlane((int)ev.w, q);
-  if ((w2 & 0xffff) == 0xffff) return v;
-  v *= H.at((int)(w2 & 0xffff), lane, p);
-  if ((w2 >> 16) == 0xffff) return v;
-  return v * H.at((int)(w2 >> 16), lane, p);
+  const double f = H.at((int)(w2 & 0xffff), lane, p);
+  if (nf == 5) return ((((v * a) * b) * c) * e) * f;
+  return (((((v * a) * b) * c) * e) * f) * H.at((int)(w2 >> 16), lane, p);
 }
 
 // h'_k(x_j) of factor row s = (j, k) of table dim i (row s = trow[i] + k - 1).
@@ -318,7 +325,11 @@ __global__ __launch_bounds__(kTile* kBWaves) void newton_jac_kernel(
       const uint4 ev = (mine < e1) ? ent[mine]
                                    : make_uint4(0u, 0xffffffffu, 0xffffffffu, 0xffffffffu);
       const int rl = d.vjp_ent_row[min(mine, e1 - 1)];
+      const int nfl = other_factor_count(ev);
       const int cnt = min(64, e1 - base);
+      // one output: each lane fetches the surplus of its entry, broadcast per entry below
+      double sl = 0.0;
+      if constexpr (std::is_void_v<TG> && NO == 1) sl = S[(int64_t)ev.x * lds];
       // adjoint: the per-point coefficients of kPre entries are requested together, ahead of
       // their use (independent loads in flight instead of one at a time)
       constexpr int kPre = std::is_void_v<TG> ? 1 : 8;
@@ -342,8 +353,14 @@ __global__ __launch_bounds__(kTile* kBWaves) void newton_jac_kernel(
             dh = row_derivative(d, xp, j, s - row0 + 1);
             s_done = s;
           }
-          const double v = times_other_factors(H, ev, q, lane, p, dh);
-          if constexpr (std::is_void_v<TG>) {
+          const double v =
+              times_other_factors(H, ev, q, __builtin_amdgcn_readlane(nfl, q), lane, p, dh);
+          if constexpr (std::is_void_v<TG> && NO == 1) {
+            const long long sb = __double_as_longlong(sl);
+            const int lo = __builtin_amdgcn_readlane((int)(sb & 0xffffffff), q);
+            const int hi = __builtin_amdgcn_readlane((int)(sb >> 32), q);
+            g[0] = fma(v, __longlong_as_double(((long long)hi << 32) | (unsigned int)lo), g[0]);
+          } else if constexpr (std::is_void_v<TG>) {
             const int node = __builtin_amdgcn_readlane((int)ev.x, q);
             cptr<double> Sn = C(S) + (int64_t)node * lds;
 #pragma unroll
@@ -418,6 +435,7 @@ __global__ __launch_bounds__(kTile* kDTWaves) void newton_jac_basis_kernel(
     const int mine = min(base + lane, n_ent - 1);
     const uint4 ev = ent[mine];
     const int rl = d.vjp_ent_row[mine];
+    const int nfl = other_factor_count(ev);
     const int il = d.row_tdim[rl];  // table dim of the row
     const int kl = rl - d.tab_row[il] + 1;
     const int cnt = min(64, n_ent - base);
@@ -431,7 +449,8 @@ __global__ __launch_bounds__(kTile* kDTWaves) void newton_jac_basis_kernel(
         dh = row_derivative(d, xp, tdim[i], k);
         s_done = s;
       }
-      const double v = times_other_factors(H, ev, q, lane, p, dh);
+      const double v =
+          times_other_factors(H, ev, q, __builtin_amdgcn_readlane(nfl, q), lane, p, dh);
       if (valid) DT[(int64_t)e * ldd + p] = v;
     }
   }
