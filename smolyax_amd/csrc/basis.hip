@@ -267,6 +267,19 @@ __device__ __forceinline__ double times_other_factors(const HTab<kG>& H, const u
   return (((((v * a) * b) * c) * e) * f) * H.at((int)(w2 >> 16), lane, p);
 }
 
+// x_j is a node coordinate of dim j: the reference's singular mask (x - xi)^2 == 0 over the nodes
+// of the dim's (dim, degree) factors (barycentric.py:152-154); the gradient is NaN there.
+__device__ __forceinline__ bool on_node(const sbi_newton_desc& d, double xv, int j) {
+  cptr<int32_t> so = C(d.sing_off);
+  cptr<double> sx = C(d.sing_x);
+  bool hit = false;
+  for (int i = so[j]; i < so[j + 1]; ++i) {
+    const double dv = xv - sx[i];
+    hit |= (dv * dv == 0.0);
+  }
+  return hit;
+}
+
 // h'_k(x_j) of factor row s = (j, k) of table dim i (row s = trow[i] + k - 1).
 __device__ __forceinline__ double row_derivative(const sbi_newton_desc& d, const double* xp, int j,
                                                  int k) {
@@ -374,6 +387,12 @@ __global__ __launch_bounds__(kTile* kBWaves) void newton_jac_kernel(
     }
     if (valid) {
       if (slot < 0) {
+        if constexpr (std::is_void_v<TG>) {  // Jacobian: NaN at node coordinates
+          if (on_node(d, xp[j], j)) {
+#pragma unroll
+            for (int o = 0; o < NO; ++o) g[o] = NAN;
+          }
+        }
 #pragma unroll
         for (int o = 0; o < NO; ++o)
           if (o < n) J[(p * n + o) * d.d_in + j] = g[o];
@@ -386,19 +405,22 @@ __global__ __launch_bounds__(kTile* kBWaves) void newton_jac_kernel(
   }
 }
 
-// J[p, o, j] = sum of the partial sums of dim j's units, in unit order.
+// J[p, o, j] = sum of the partial sums of dim j's units, in unit order (with x: the Jacobian,
+// NaN at node coordinates; without: the adjoint).
 __global__ __launch_bounds__(256) void jac_reduce_kernel(sbi_newton_desc d, int64_t P, int64_t n,
                                                         const double* __restrict__ part,
-                                                        double* __restrict__ J) {
+                                                        double* __restrict__ J,
+                                                        const double* __restrict__ x, int64_t ldx) {
   const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (p >= P) return;
   cptr<int32_t> red = C(d.jac_red);
   const int r = blockIdx.y;
   const int j = red[3 * r], s0 = red[3 * r + 1], ns = red[3 * r + 2];
+  const bool nan_here = x && on_node(d, x[p * ldx + j], j);
   for (int64_t o = 0; o < n; ++o) {
     double acc = 0.0;
     for (int u = 0; u < ns; ++u) acc += part[((int64_t)(s0 + u) * P + p) * n + o];
-    J[(p * n + o) * d.d_in + j] = acc;
+    J[(p * n + o) * d.d_in + j] = nan_here ? NAN : acc;
   }
 }
 
@@ -686,7 +708,8 @@ static int vjp_launch(const sbi_newton_desc& d, const double* x, int64_t P, int6
   }
   if (d.n_jac_red > 0) {
     hipLaunchKernelGGL(jac_reduce_kernel, dim3((unsigned)((P + 255) / 256), (unsigned)d.n_jac_red),
-                       dim3(256), 0, st, d, P, (int64_t)1, part, gx);
+                       dim3(256), 0, st, d, P, (int64_t)1, part, gx, (const double*)nullptr,
+                       (int64_t)0);
     SBI_HIP_CHECK(hipGetLastError());
   }
   return SBI_OK;
@@ -872,9 +895,10 @@ extern "C" int sbi_newton_jacobian_f64(const sbi_newton_plan* plan, const double
     }
     if (d.n_jac_red > 0) {
       hipLaunchKernelGGL(jac_reduce_kernel, dim3((unsigned)((P + 255) / 256), (unsigned)d.n_jac_red),
-                         dim3(256), 0, st, d, P, n, part, J);
+                         dim3(256), 0, st, d, P, n, part, J, x, ldx);
       SBI_HIP_CHECK(hipGetLastError());
     }
+    return SBI_OK;  // the register pass and the reduction wrote the NaNs at node coordinates
   } else {
     if (d.n_vjp_ent > 0) {
       if (g) jac_basis_launch<true>(d, x, P, ldx, gtab, DT, st);

@@ -256,8 +256,13 @@ class SmolyakBarycentricInterpolator:
             "vjp_ent_row": _dev_tensor(NL.vjp_ent_row, i32, dev),
             "row_tdim": _dev_tensor(NL.row_tdim, i32, dev),
         }
-        t["sing_off"], t["sing_x"] = (_dev_tensor(a, i32 if a.dtype == np.int32 else f64, dev)
-                                      for a in self._singular_coordinates())
+        sing_off, sing_x = self._singular_coordinates()
+        # the Jacobian kernels mark NaNs while writing the dims that have factors: every dim with
+        # singular coordinates is one of them (a block's dims are dims of Lambda's nodes)
+        sing_dims = np.nonzero(np.diff(sing_off))[0]
+        if not np.isin(sing_dims, NL.tab_dim[: NL.n_tab_dims]).all():
+            raise RuntimeError("singular coordinates in a dim without factors")
+        t["sing_off"], t["sing_x"] = _dev_tensor(sing_off, i32, dev), _dev_tensor(sing_x, f64, dev)
         if NL.node_sq is not None:  # staged kernel tables (else the unstaged kernel runs)
             t["node_sq"] = _dev_tensor(NL.node_sq, i32, dev)
         # surpluses in nf.nus order (adjoint), and in the basis-row order of the GEMM paths (the
