@@ -786,8 +786,12 @@ extern "C" int sbi_bary_basis_f32(const sbi_plan* plan, const double* x, int64_t
 // ------------------------------------------------------------------------------------------
 constexpr int kJacDirectMax = 8;  // outputs handled in one register pass
 
+// The Jacobian and adjoint passes are bound by latency per entry, so they need many waves per
+// CU: a factor table above kJacLdsMax (which would leave room for too few workgroups per CU) goes
+// to global memory, written once by newton_htab_kernel and read through the caches.
+constexpr size_t kJacLdsMax = 40 * 1024;
 static bool jac_global_table(const sbi_newton_desc& d) {
-  return (size_t)d.n_tab_rows * kTile * sizeof(double) > (size_t)kMaxLds;
+  return (size_t)d.n_tab_rows * kTile * sizeof(double) > kJacLdsMax;
 }
 
 extern "C" int64_t sbi_newton_jacobian_workspace(const sbi_newton_plan* plan, int64_t P,
