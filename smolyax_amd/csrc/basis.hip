@@ -36,7 +36,8 @@
 namespace {
 
 constexpr int kTile = 64;
-constexpr int kBWaves = 4;
+constexpr int kBWaves = 4;   // waves per workgroup of the Jacobian / adjoint passes
+constexpr int kHWaves = 16;  // waves per workgroup of the basis kernel (one LDS table per tile)
 constexpr int kMaxLds = 160 * 1024;
 
 template <typename T>
@@ -88,13 +89,52 @@ __device__ __forceinline__ void store_out(__hip_bfloat16* p, double v) {
   *p = __float2bfloat16((float)v);
 }
 
+// Number of factor rows of each lane's node descriptor (8 x u16 rows packed first, 0xffff after).
+__device__ __forceinline__ int desc_factor_count(const uint4& dv) {
+  int c = 0;
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    const uint32_t w = f == 0 ? dv.x : f == 1 ? dv.y : f == 2 ? dv.z : dv.w;
+    c += (int)((w & 0xffff) != 0xffff) + (int)((w >> 16) != 0xffff);
+  }
+  return c;
+}
+
+// Product of the lane's table values at the nf factor rows of node i of the wave's 64, in the
+// stored order (the same products as multiplying row by row from 1).
+template <typename T, typename LT>
+__device__ __forceinline__ T node_product(const LT* tab, const uint4& dv, int i, int nf,
+                                          int lane) {
+  T v = (T)1;
+  if (nf == 0) return v;
+  const uint32_t w0 = (uint32_t)__builtin_amdgcn_readlane((int)dv.x, i);
+  v *= tab[(w0 & 0xffff) * kTile + lane];
+  if (nf == 1) return v;
+  v *= tab[(w0 >> 16) * kTile + lane];
+  if (nf == 2) return v;
+  const uint32_t w1 = (uint32_t)__builtin_amdgcn_readlane((int)dv.y, i);
+  v *= tab[(w1 & 0xffff) * kTile + lane];
+  if (nf == 3) return v;
+  v *= tab[(w1 >> 16) * kTile + lane];
+  if (nf == 4) return v;
+  const uint32_t w2 = (uint32_t)__builtin_amdgcn_readlane((int)dv.z, i);
+  v *= tab[(w2 & 0xffff) * kTile + lane];
+  if (nf == 5) return v;
+  v *= tab[(w2 >> 16) * kTile + lane];
+  if (nf == 6) return v;
+  const uint32_t w3 = (uint32_t)__builtin_amdgcn_readlane((int)dv.w, i);
+  v *= tab[(w3 & 0xffff) * kTile + lane];
+  if (nf == 7) return v;
+  return v * tab[(w3 >> 16) * kTile + lane];
+}
+
 // H^T[nu, p] for a chunk of nodes.  Node descriptor: 8 x u16 table rows (0xffff = none).
 //
 // kQ (FP32 only): k-interleaved output for the contraction, HT[((r / 4) * ldh + p) * 4 + r % 4]
 // for row r = row - row0, rows padded with zeros to a multiple of 16; each lane stores the four
 // rows of a quad as one 16-byte vector.
 template <typename T, typename OUT, bool kQ = false>
-__global__ __launch_bounds__(kTile* kBWaves) void newton_basis_kernel(sbi_newton_desc d,
+__global__ __launch_bounds__(kTile* kHWaves) void newton_basis_kernel(sbi_newton_desc d,
                                                                       const double* __restrict__ x,
                                                                       int64_t P, int64_t ldx,
                                                                       OUT* __restrict__ HT,
@@ -108,7 +148,7 @@ __global__ __launch_bounds__(kTile* kBWaves) void newton_basis_kernel(sbi_newton
   const int64_t p_raw = (int64_t)blockIdx.x * kTile + lane;
   const bool valid = p_raw < P;
   const int64_t p = valid ? p_raw : (P - 1);
-  build_h_table<T>(d, x + p * ldx, tab, nullptr, wave, lane, kBWaves);
+  build_h_table<T>(d, x + p * ldx, tab, nullptr, wave, lane, kHWaves);
   __syncthreads();
 
   const int n0 = row0 + blockIdx.y * chunk;
@@ -116,27 +156,20 @@ __global__ __launch_bounds__(kTile* kBWaves) void newton_basis_kernel(sbi_newton
   const int n1 = min(rend, n0 + chunk);
   const uint4* __restrict__ desc = reinterpret_cast<const uint4*>(d.basis_desc);
   // each wave: 64-node groups, descriptors one per lane, broadcast with v_readlane
-  for (int base = n0 + wave * 64; base < n1; base += kBWaves * 64) {
+  for (int base = n0 + wave * 64; base < n1; base += kHWaves * 64) {
     const int mine = base + lane;
     const uint4 dv = (mine < min(n1, row1)) ? desc[mine]
                                             : make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu,
                                                          0xffffffffu);
+    const int nfl = desc_factor_count(dv);
     const int cnt = min(64, n1 - base);
     if constexpr (kQ) {
       for (int i = 0; i < cnt; i += 4) {  // cnt is a multiple of 4 (16-row padding, 64-row groups)
         float q[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const uint32_t w[4] = {(uint32_t)__builtin_amdgcn_readlane((int)dv.x, i + u),
-                                 (uint32_t)__builtin_amdgcn_readlane((int)dv.y, i + u),
-                                 (uint32_t)__builtin_amdgcn_readlane((int)dv.z, i + u),
-                                 (uint32_t)__builtin_amdgcn_readlane((int)dv.w, i + u)};
-          T v = (T)1;
-#pragma unroll
-          for (int f = 0; f < 8; ++f) {
-            const uint32_t r = (w[f >> 1] >> ((f & 1) * 16)) & 0xffff;
-            if (r != 0xffff) v *= tab[r * kTile + lane];
-          }
+          const T v = node_product<T>(tab, dv, i + u,
+                                      __builtin_amdgcn_readlane(nfl, i + u), lane);
           q[u] = (base + i + u < row1) ? (float)v : 0.0f;
         }
         if (valid) {
@@ -148,16 +181,7 @@ __global__ __launch_bounds__(kTile* kBWaves) void newton_basis_kernel(sbi_newton
       continue;
     }
     for (int i = 0; i < cnt; ++i) {
-      const uint32_t w[4] = {(uint32_t)__builtin_amdgcn_readlane((int)dv.x, i),
-                             (uint32_t)__builtin_amdgcn_readlane((int)dv.y, i),
-                             (uint32_t)__builtin_amdgcn_readlane((int)dv.z, i),
-                             (uint32_t)__builtin_amdgcn_readlane((int)dv.w, i)};
-      T v = (T)1;
-#pragma unroll
-      for (int f = 0; f < 8; ++f) {
-        const uint32_t r = (w[f >> 1] >> ((f & 1) * 16)) & 0xffff;
-        if (r != 0xffff) v *= tab[r * kTile + lane];
-      }
+      const T v = node_product<T>(tab, dv, i, __builtin_amdgcn_readlane(nfl, i), lane);
       if (valid) store_out(HT + (int64_t)(base + i - row0) * ldh + p, (double)v);
     }
   }
@@ -587,7 +611,7 @@ static int basis_launch(const sbi_newton_desc& d, const double* x, int64_t P, in
   if (row1 <= row0) return SBI_OK;
   const int rows = kQ ? ((row1 - row0 + 15) & ~15) : row1 - row0;
   dim3 grid((unsigned)((P + kTile - 1) / kTile), (unsigned)((rows + chunk - 1) / chunk));
-  hipLaunchKernelGGL((newton_basis_kernel<T, OUT, kQ>), grid, dim3(kTile * kBWaves), lds, st, d, x,
+  hipLaunchKernelGGL((newton_basis_kernel<T, OUT, kQ>), grid, dim3(kTile * kHWaves), lds, st, d, x,
                      P, ldx, HT, ldh, chunk, row0, row1);
   SBI_HIP_CHECK(hipGetLastError());
   return SBI_OK;
