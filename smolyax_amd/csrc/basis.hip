@@ -530,8 +530,9 @@ __global__ void fill_zero_kernel(double* __restrict__ g, int64_t n) {
 }
 
 // W^T[e, p] for the entries of a chunk of barycentric blocks.  Factor order on the device:
-// prefix factors, then the streamed one; entries in C order (last factor fastest).  Each factor
-// value Lhat_mu(x) = w_mu prod_{nu != mu} (x - xi_nu) is formed on the fly (exact one-hot at nodes).
+// prefix factors, then the streamed one; entries in C order (last factor fastest).  A factor
+// value Lhat_mu(x) = w_mu prod_{nu != mu} (x - xi_nu) (exact one-hot at nodes) is formed once per
+// block for the last factor and once per index of the other factors for those.
 __device__ __forceinline__ double lagrange_first_form(double xv, cptr<double> xi, cptr<double> w,
                                                       int deg, int mu) {
   double v = w[mu];
@@ -549,6 +550,8 @@ __device__ __forceinline__ double lagrange_first_form(double xv, cptr<double> xi
   if (hit_other) return 0.0;
   return v;
 }
+
+constexpr int kBLMax = 12;  // last-factor basis values held in registers (degree < 12)
 
 template <typename OUT>
 __global__ __launch_bounds__(kTile) void bary_basis_kernel(sbi_bary_desc d,
@@ -569,27 +572,45 @@ __global__ __launch_bounds__(kTile) void bary_basis_kernel(sbi_bary_desc d,
   for (int b = b0; b < b1; ++b) {
     const int f0 = blk_fac[b], npre = blk_fac[b + 1] - f0, ls = last[b];
     const int nf = npre + (ls >= 0 ? 1 : 0);
-    int slots[16];
-    double xs[16];
-    int64_t nent = 1;
-    for (int i = 0; i < nf && i < 16; ++i) {
-      slots[i] = (i < npre) ? fac_slot[f0 + i] : ls;
-      xs[i] = xp[slot_dim[slots[i]]];
-      nent *= slot_deg[slots[i]] + 1;
-    }
     const int64_t r0 = foff[b];
-    for (int64_t e = 0; e < nent; ++e) {
-      int64_t rem = e;
-      double wgt = 1.0;
-      for (int i = nf - 1; i >= 0; --i) {
-        const int s = slots[i];
-        const int di1 = slot_deg[s] + 1;
-        const int mu = (int)(rem % di1);
+    if (nf == 0) {  // the constant block: one entry of weight 1
+      if (valid) WT[r0 * ldw + p] = (OUT)1.0;
+      continue;
+    }
+    // entries e = q * (dl + 1) + m in C order: m indexes the last factor, q the others (the
+    // second-to-last fastest).  The last factor's basis is computed once per block, the product
+    // of the others once per q.
+    auto slot_of = [&](int i) { return (i < npre) ? fac_slot[f0 + i] : ls; };
+    const int sl = slot_of(nf - 1);
+    const int dl = slot_deg[sl];
+    const double xl = xp[slot_dim[sl]];
+    cptr<double> nl = nodes + (int64_t)sl * d.dmax1, wl = wts + (int64_t)sl * d.dmax1;
+    double Ll[kBLMax];
+#pragma unroll
+    for (int m = 0; m < kBLMax; ++m)
+      Ll[m] = (m <= dl) ? lagrange_first_form(xl, nl, wl, dl, m) : 0.0;
+    int nq = 1;
+    for (int i = 0; i < nf - 1; ++i) nq *= slot_deg[slot_of(i)] + 1;
+    for (int q = 0; q < nq; ++q) {
+      double pre = 1.0;
+      int rem = q;
+      for (int i = nf - 2; i >= 0; --i) {
+        const int si = slot_of(i);
+        const int di1 = slot_deg[si] + 1;
+        const int mu = rem % di1;
         rem /= di1;
-        wgt *= lagrange_first_form(xs[i], nodes + (int64_t)s * d.dmax1, wts + (int64_t)s * d.dmax1,
-                                   di1 - 1, mu);
+        pre *= lagrange_first_form(xp[slot_dim[si]], nodes + (int64_t)si * d.dmax1,
+                                   wts + (int64_t)si * d.dmax1, di1 - 1, mu);
       }
-      if (valid) WT[(r0 + e) * ldw + p] = (OUT)wgt;
+      OUT* __restrict__ out = WT + (r0 + (int64_t)q * (dl + 1)) * ldw + p;
+      if (dl < kBLMax) {
+#pragma unroll
+        for (int m = 0; m < kBLMax; ++m)
+          if (m <= dl && valid) out[(int64_t)m * ldw] = (OUT)(Ll[m] * pre);
+      } else {  // degree beyond the register table: the last factor's basis entry by entry
+        for (int m = 0; m <= dl; ++m)
+          if (valid) out[(int64_t)m * ldw] = (OUT)(lagrange_first_form(xl, nl, wl, dl, m) * pre);
+      }
     }
   }
 }
