@@ -258,14 +258,14 @@ int sbi_newton_rows_f64(const sbi_newton_plan* plan, const double* x, int64_t P,
  * (dtype 0: FP64, 1: FP32), rows in the adjoint's node order.  Work units of at most 128 adjoint
  * entries of one dim; a dim split over several units adds their partial sums in unit order, so
  * the result is deterministic.  workspace: sbi_newton_vjp_workspace(plan, P) bytes (partial sums,
- * and the factor table when it exceeds LDS). */
+ * and the factor table when it takes more than 40 KB per 64 points). */
 int64_t sbi_newton_vjp_workspace(const sbi_newton_plan* plan, int64_t P);
 int sbi_newton_vjp(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
                    const void* GT, int64_t ldg, int32_t dtype, double* gx, void* workspace,
                    int64_t ws_bytes, void* stream);
 /* Bytes of device workspace sbi_newton_jacobian_f64 needs for P points and n outputs: the factor
- * table when it exceeds LDS; for n <= 8 the partial sums of split dims; for n > 8 the derivative
- * basis D^T (adjoint entries x P). */
+ * table when it takes more than 40 KB per 64 points; for n <= 8 the partial sums of split dims;
+ * for n > 8 the derivative basis D^T (adjoint entries x P). */
 int64_t sbi_newton_jacobian_workspace(const sbi_newton_plan* plan, int64_t P, int64_t n);
 /* Dense Jacobian of the Newton form, J[P, n, d_in] (FP64, contiguous), the reference's gradient
  * (interpolation.py:306-345, barycentric.py:126-229) of the same polynomial:
@@ -274,8 +274,9 @@ int64_t sbi_newton_jacobian_workspace(const sbi_newton_plan* plan, int64_t P, in
  * sbi_newton_vjp).  n <= 8: one pass over work units of at most 128 entries, sums in registers
  * (a dim split over several units adds their partial sums in unit order);
  * n > 8: D^T is written to the workspace and contracted on the FP64 matrix cores with the
- * entries' surplus rows, one group of rows per input dim, eight dims per workgroup.  Then J[p, :, j] = NaN where x_j is one of sing_x of dim j (the reference's result
- * at a node coordinate).  Deterministic. */
+ * entries' surplus rows, one group of rows per input dim, eight dims per workgroup.
+ * J[p, :, j] = NaN where x_j is one of sing_x of dim j (the reference's result at a node
+ * coordinate).  Deterministic. */
 int sbi_newton_jacobian_f64(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
                             const double* S, int64_t lds, int64_t n, double* J, void* workspace,
                             int64_t ws_bytes, void* stream);
