@@ -18,6 +18,7 @@ from .nodes import Generator
 
 PRECISIONS = ("fp64", "mixed", "newton-fp64", "bf16", "bary-fp32")
 GEMM_MIN_DOUT = 16  # from this d_out on, the d_out contraction is one dense GEMM
+MAX_CALL_POINTS = 65535 * 64  # points per call of sbi_contract / _jacobian_f64 / sbi_newton_vjp
 
 
 def _dev_tensor(a, dtype, device):
@@ -379,9 +380,12 @@ class SmolyakBarycentricInterpolator:
         return self._eval_newton_gemm(x, precision)
 
     def _point_chunk(self, rows, itemsize, P):
-        """Points per chunk so that the basis matrix stays within memory_limit GiB."""
+        """Points per chunk so that the basis matrix stays within memory_limit GiB, and within
+        the 65535 * 64 points one call of the contraction, Jacobian and adjoint entry points
+        takes."""
         per_point = max(1, rows) * itemsize
-        return max(64, min(P, int(self._memory_limit * 1024**3 // per_point) // 64 * 64 or 64))
+        chunk = int(self._memory_limit * 1024**3 // per_point) // 64 * 64 or 64
+        return max(64, min(P, chunk, MAX_CALL_POINTS))
 
     def _contract(self, A, B, offset, y):
         """y[P, n] = offset + A^T B on the matrix cores (sbi_contract: FP64 or FP32 MFMA with
@@ -507,13 +511,20 @@ class SmolyakBarycentricInterpolator:
             return torch.einsum("pm,pmj->pj", v.to(torch.float64), J)
         kind, code = ("f32", 1) if precision in ("mixed", "bf16") else ("f64", 0)
         S = self._surpluses(kind)
-        GT = torch.mm(S, v.to(S.dtype).t()).contiguous()
         gx = torch.empty((P, self._d_in), dtype=torch.float64, device=self._device)
-        nb = int(N.fn["sbi_newton_vjp_workspace"](self._nplan, P))
-        ws = torch.empty(max(1, (nb + 7) // 8), dtype=torch.float64, device=self._device)
-        N.check(N.fn["sbi_newton_vjp"](self._nplan, x.data_ptr(), P, x.stride(0), GT.data_ptr(),
-                                       GT.stride(0), code, gx.data_ptr(), ws.data_ptr(), nb,
-                                       self._stream()), "vjp")
+        # point chunks: G^T (nodes x points) and the workspace within memory_limit
+        ws1 = int(N.fn["sbi_newton_vjp_workspace"](self._nplan, 64))
+        chunk = self._point_chunk(S.shape[0] * S.element_size() + max(0, ws1) // 64 + 1, 1, P)
+        for p0 in range(0, P, chunk):
+            p1 = min(P, p0 + chunk)
+            xc = x[p0:p1]
+            GT = torch.mm(S, v[p0:p1].to(S.dtype).t()).contiguous()
+            nb = int(N.fn["sbi_newton_vjp_workspace"](self._nplan, p1 - p0))
+            ws = torch.empty(max(1, (nb + 7) // 8), dtype=torch.float64, device=self._device)
+            N.check(N.fn["sbi_newton_vjp"](self._nplan, xc.data_ptr(), p1 - p0, xc.stride(0),
+                                           GT.data_ptr(), GT.stride(0), code,
+                                           gx[p0:p1].data_ptr(), ws.data_ptr(), nb,
+                                           self._stream()), "vjp")
         return gx
 
     def differentiable(self, x: torch.Tensor, precision: str = None) -> torch.Tensor:
@@ -541,7 +552,7 @@ class SmolyakBarycentricInterpolator:
         sbi_newton_jacobian_f64 in point chunks whose workspace stays within memory_limit."""
         P, n = x.shape[0], self._d_out
         ws1 = int(N.fn["sbi_newton_jacobian_workspace"](self._nplan, 64, n))
-        chunk = self._point_chunk(max(1, ws1 // 64), 1, P) if ws1 > 0 else P
+        chunk = self._point_chunk(max(1, ws1 // 64), 1, P)
         S = self._S["f64"]
         for p0 in range(0, P, chunk):
             p1 = min(P, p0 + chunk)

@@ -62,6 +62,24 @@ def test_gemm_point_chunks(gpu):
     assert rel(y_b, ref(x)) < 1e-11
 
 
+def test_point_chunks_of_the_adjoint_and_jacobian(gpu):
+    """The adjoint and the Jacobian in chunks of 64 points (memory_limit) equal one call: each
+    point's adjoint and Jacobian are computed from its own row of x and column of G^T only; the
+    library GEMM that forms G^T may sum in another order per chunk (1e-13)."""
+    ref, ip, rng = make(10, 12, 400, seed=4)
+    x = rng.uniform(-1, 1, (300, 10))
+    v = rng.standard_normal((300, 12))
+    g_all = ip.vjp(x, v, "fp64").cpu().numpy()
+    J_all = ip.gradient(x).cpu().numpy()
+    ip._memory_limit = 1e-6  # chunks of 64 points
+    g_ch = ip.vjp(x, v, "fp64").cpu().numpy()
+    J_ch = ip.gradient(x).cpu().numpy()
+    assert rel(g_ch, g_all) <= 1e-13
+    assert np.array_equal(J_ch, J_all)
+    gr = np.einsum("pm,pmj->pj", v, ref.gradient(x))
+    assert rel(g_ch, gr) <= 1e-10
+
+
 @pytest.mark.parametrize("d,d_out,m", [(3, 2, 40), (12, 20, 600), (40, 16, 1500)])
 def test_vjp_matches_oracle_jacobian(gpu, d, d_out, m):
     ref, ip, rng = make(d, d_out, m, seed=10 + d)
