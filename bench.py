@@ -272,7 +272,8 @@ def bench_serving(torch, W, n_requests=600, qps=SERVE_QPS, seed=0):
 
     For each launch mode (direct, HIP graphs, adaptive): one untimed warm-up pass, the capacity
     (all requests submitted at once), then p50/p99 under Poisson arrivals at `qps`.  The adaptive
-    mode also reports p50/p99 at 25 %, 50 % and 70 % of its capacity (latency against load)."""
+    mode also reports p50/p99 at 25 %, 50 % and 70 % of its capacity (latency against load), each
+    after one untimed pass at that load (the first pass allocates buffers for its batch sizes)."""
     from smolyax_amd import serving
 
     ts = time.perf_counter()
@@ -314,6 +315,7 @@ def bench_serving(torch, W, n_requests=600, qps=SERVE_QPS, seed=0):
         if curve:
             out["load_curve"] = []
             for frac in (0.25, 0.5, 0.7):
+                poisson(srv, frac * cap_qps)  # untimed: batch sizes of this load seen once
                 c = poisson(srv, frac * cap_qps)
                 _log(f"serve (graphs={graphs}): {frac:.0%} of capacity: p50 {c['p50_ms']:.3f} ms "
                      f"p99 {c['p99_ms']:.3f} ms")
