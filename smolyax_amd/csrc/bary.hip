@@ -466,13 +466,12 @@ __global__ __launch_bounds__(kIntThreads) void bary_integral_partial_kernel(
   double acc = 0.0;
   if (o < d_out) {
     int64_t e = e0 + ty;
-    for (; e + 3 * kE < e1; e += 4 * kE) {  // four independent loads in flight
-      const double f0 = F[e * d_out + o], f1 = F[(e + kE) * d_out + o];
-      const double f2 = F[(e + 2 * kE) * d_out + o], f3 = F[(e + 3 * kE) * d_out + o];
-      acc = fma(q[e], f0, acc);
-      acc = fma(q[e + kE], f1, acc);
-      acc = fma(q[e + 2 * kE], f2, acc);
-      acc = fma(q[e + 3 * kE], f3, acc);
+    for (; e + 7 * kE < e1; e += 8 * kE) {  // eight independent loads in flight
+      double f[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) f[u] = F[(e + u * kE) * d_out + o];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = fma(q[e + u * kE], f[u], acc);
     }
     for (; e < e1; e += kE) acc = fma(q[e], F[e * d_out + o], acc);
   }
@@ -653,9 +652,13 @@ extern "C" int sbi_bary_integral_f64(const sbi_plan* plan, double* Q, double* pa
   }
   const sbi_bary_desc& d = plan->bary;
   hipStream_t st = (hipStream_t)stream;
-  // output tile width: the smallest power of two >= d_out, at most 64 (>= 4 entry lanes)
+  // output tile width: the smallest power of two >= d_out, at most 64 (4 entry lanes), or 256
+  // (one entry lane: a workgroup reads 2 KB runs of each F row) from d_out = 1024 on, where F is
+  // large enough to keep ~2048 such workgroups busy (config 3: 75 % against 65 % of HBM; config
+  // 4, d_out 256: 33 against 25 us)
+  const int wmax = d.d_out >= 1024 ? 256 : 64;
   int w = 1;
-  while (w < d.d_out && w < 64) w <<= 1;
+  while (w < d.d_out && w < wmax) w <<= 1;
   const int64_t otiles = (d.d_out + w - 1) / w;
   // entry chunks: ~2048 workgroups in all, each chunk at least 4 entries per entry lane
   const int64_t lanes = kIntThreads / w;
@@ -674,7 +677,9 @@ extern "C" int sbi_bary_integral_f64(const sbi_plan* plan, double* Q, double* pa
     case 8: integral_launch<8>(d, (int)nc, chunk, partial, st); break;
     case 16: integral_launch<16>(d, (int)nc, chunk, partial, st); break;
     case 32: integral_launch<32>(d, (int)nc, chunk, partial, st); break;
-    default: integral_launch<64>(d, (int)nc, chunk, partial, st); break;
+    case 64: integral_launch<64>(d, (int)nc, chunk, partial, st); break;
+    case 128: integral_launch<128>(d, (int)nc, chunk, partial, st); break;
+    default: integral_launch<256>(d, (int)nc, chunk, partial, st); break;
   }
   SBI_HIP_CHECK(hipGetLastError());
   dim3 g2((unsigned)((d.d_out + kFinO - 1) / kFinO));
