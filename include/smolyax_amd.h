@@ -36,7 +36,9 @@ const char* sbi_version(void);
 /* ----------------------------------------------------------------------------------------------
  * Multi-index sets (host, exact).  Bit-identical restatements of the numba kernels of
  * src/smolyax/indices.py: same traversal order, same sequence of IEEE double operations.
- * `k` is the anisotropy vector (length d), `t` the threshold.
+ * `k` is the anisotropy vector (length d), `t` the threshold.  A NULL k with d > 0 or a negative
+ * d is an error: -1 from the integer functions, NaN from the threshold search, SBI_ERR_INVALID
+ * from the two set builders (sbi_last_error() says why).
  * -------------------------------------------------------------------------------------------- */
 
 /* indices.py:72-118  indexset_cardinality(k, t) */

@@ -557,7 +557,10 @@ int sbi_bary_plan_create(const sbi_bary_desc* desc, sbi_plan** out) {
 }
 
 int sbi_bary_plan_desc(const sbi_plan* plan, const sbi_bary_desc** out) {
-  if (!plan || !out || plan->kind != 1) return SBI_ERR_INVALID;
+  if (!plan || !out || plan->kind != 1) {
+    sbi::set_error("sbi_bary_plan_desc: null argument or not a barycentric plan");
+    return SBI_ERR_INVALID;
+  }
   *out = &plan->bary;
   return SBI_OK;
 }
@@ -632,7 +635,10 @@ extern "C" int sbi_bary_jacobian_f64(const sbi_plan* plan, const double* x, int6
 }
 
 extern "C" int64_t sbi_bary_integral_workspace(const sbi_plan* plan) {
-  if (!plan) return -1;
+  if (!plan) {
+    sbi::set_error("sbi_bary_integral_workspace: null plan");
+    return -1;
+  }
   return (int64_t)kIntMaxChunks * plan->bary.d_out;
 }
 

@@ -706,7 +706,10 @@ static bool jac_global_table(const sbi_newton_desc& d);
 
 extern "C" int64_t sbi_newton_vjp_workspace(const sbi_newton_plan* plan, int64_t P) {
   const sbi_newton_desc* dp = nullptr;
-  if (sbi_newton_plan_desc(plan, &dp) != SBI_OK || P < 0) return -1;
+  if (sbi_newton_plan_desc(plan, &dp) != SBI_OK || P < 0) {
+    sbi::set_error("sbi_newton_vjp_workspace: null plan or negative P");
+    return -1;
+  }
   const int64_t P64 = (P + kTile - 1) / kTile * kTile;
   int64_t bytes = (int64_t)dp->n_jac_partials * P * (int64_t)sizeof(double);
   if (jac_global_table(*dp)) bytes += (int64_t)dp->n_tab_rows * P64 * (int64_t)sizeof(double);
@@ -842,7 +845,10 @@ static bool jac_global_table(const sbi_newton_desc& d) {
 extern "C" int64_t sbi_newton_jacobian_workspace(const sbi_newton_plan* plan, int64_t P,
                                                  int64_t n) {
   const sbi_newton_desc* dp = nullptr;
-  if (sbi_newton_plan_desc(plan, &dp) != SBI_OK || P < 0 || n < 0) return -1;
+  if (sbi_newton_plan_desc(plan, &dp) != SBI_OK || P < 0 || n < 0) {
+    sbi::set_error("sbi_newton_jacobian_workspace: null plan or negative P or n");
+    return -1;
+  }
   int64_t bytes = 0;
   const int64_t P64 = (P + kTile - 1) / kTile * kTile;
   if (jac_global_table(*dp)) bytes += (int64_t)dp->n_tab_rows * P64 * (int64_t)sizeof(double);

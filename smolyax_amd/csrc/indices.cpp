@@ -116,24 +116,37 @@ struct Pool {
 
 }  // namespace
 
+// k must hold d >= 0 values (k may be NULL only when d == 0)
+static bool bad_k(const double* k, int64_t d, const char* fn) {
+  if (d < 0 || (d > 0 && !k)) {
+    sbi::set_error(std::string(fn) + ": null k or negative d");
+    return true;
+  }
+  return false;
+}
+
 extern "C" {
 
 int64_t sbi_indexset_cardinality(const double* k, int64_t d, double t) {
+  if (bad_k(k, d, "sbi_indexset_cardinality")) return -1;
   return indexset_cardinality_impl(k, d, t);
 }
 
 int64_t sbi_smolyak_coefficient(const double* k, int64_t d, double rem_t, int32_t parity) {
+  if (bad_k(k, d, "sbi_smolyak_coefficient")) return -1;
   std::vector<std::pair<int64_t, std::pair<double, int32_t>>> st;
   return smolyak_coefficient_impl(k, d, rem_t, parity, st);
 }
 
 int64_t sbi_nodeset_cardinality(const double* k, int64_t d, double t, int32_t nested) {
+  if (bad_k(k, d, "sbi_nodeset_cardinality")) return -1;
   return nodeset_cardinality_impl(k, d, t, nested);
 }
 
 // indices.py:292-355
 double sbi_find_approximate_threshold(const double* k, int64_t d, int64_t m, int32_t nested,
                                       int32_t max_iter, double accuracy) {
+  if (bad_k(k, d, "sbi_find_approximate_threshold")) return NAN;
   if (m <= 0) {
     sbi::set_error("find_approximate_threshold: m must be > 0");
     return NAN;

@@ -495,12 +495,18 @@ extern "C" int sbi_newton_plan_create(const sbi_newton_desc* desc, sbi_newton_pl
 }
 
 extern "C" int sbi_newton_mixed_staged(const sbi_newton_plan* plan) {
-  if (!plan) return SBI_ERR_INVALID;
+  if (!plan) {
+    sbi::set_error("sbi_newton_mixed_staged: null plan");
+    return SBI_ERR_INVALID;
+  }
   return plan->lds_staged <= (size_t)kMaxLds ? 1 : 0;
 }
 
 extern "C" int sbi_newton_plan_desc(const sbi_newton_plan* plan, const sbi_newton_desc** out) {
-  if (!plan || !out) return SBI_ERR_INVALID;
+  if (!plan || !out) {
+    sbi::set_error("sbi_newton_plan_desc: null argument");
+    return SBI_ERR_INVALID;
+  }
   *out = &plan->d;
   return SBI_OK;
 }

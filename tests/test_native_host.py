@@ -5,6 +5,7 @@ import json
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -38,6 +39,38 @@ def test_error_reporting():
     rc = N.fn["sbi_bary_plan_create"](None, ctypes.byref(plan))
     assert rc == -1
     assert b"null" in N.fn["sbi_last_error"]()
+
+
+NULL_CALLS = r"""
+import ctypes, math, sys
+sys.path.insert(0, sys.argv[1])
+from smolyax_amd import _native as N
+skip = {"sbi_last_error", "sbi_version", "sbi_plan_destroy", "sbi_newton_plan_destroy"}
+for name, (res, args) in sorted(N.SIGNATURES.items()):
+    if name in skip:
+        continue
+    vals = [1 if a in (ctypes.c_int64, ctypes.c_int32) else 1.0 if a is ctypes.c_double else None
+            for a in args]
+    rc = N.fn[name](*vals)
+    print(name, "nan" if isinstance(rc, float) and math.isnan(rc) else rc,
+          N.fn["sbi_last_error"]().decode().split(":")[0], flush=True)
+"""
+
+
+def test_every_entry_point_rejects_null_arguments():
+    """Every entry point, called with NULL pointers (and counts of 1), returns its error value
+    (a negative code, -1, or NaN for the threshold search) and sets sbi_last_error, without
+    touching a device.  In a child process, so a crash is a failure, not a lost test run."""
+    out = subprocess.run([sys.executable, "-c", NULL_CALLS, ROOT], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stderr[-2000:]
+    rows = [line.split() for line in out.stdout.splitlines()]
+    names = {r[0] for r in rows}
+    assert names == set(declared_symbols()) - {"sbi_last_error", "sbi_version", "sbi_plan_destroy",
+                                               "sbi_newton_plan_destroy"}
+    for name, rc, who in rows:
+        assert rc == "nan" or int(rc) < 0, (name, rc)
+        assert who.startswith("sbi_") and who.split("_")[1] == name.split("_")[1], (name, who)
 
 
 def k_rule(d):
