@@ -151,8 +151,19 @@ __device__ __forceinline__ void contract_block(const sbi_bary_desc& d, int b, cp
 #pragma unroll
     for (int o = 0; o < OC; ++o) acc[o] += inner[o];
   } else if (npre == 1) {
+    // two prefix entries per step: both F rows are requested before either is used (one scalar
+    // load round trip per pair), the accumulation order is unchanged
     const int d1 = (w1 >> 8) & 31, r1 = w2 & 0xffff;
-    for (int m1 = 0; m1 <= d1; ++m1) {
+    int m1 = 0;
+    for (; m1 + 1 <= d1; m1 += 2) {
+      double inner2[OC];
+      row_dot<DL, OC>(Fb + m1 * rs, dout, oc_n, Ll, inner);
+      row_dot<DL, OC>(Fb + (m1 + 1) * rs, dout, oc_n, Ll, inner2);
+      const double p1 = tab[(r1 + m1) * kTile + lane], q1 = tab[(r1 + m1 + 1) * kTile + lane];
+#pragma unroll
+      for (int o = 0; o < OC; ++o) acc[o] = fma(q1, inner2[o], fma(p1, inner[o], acc[o]));
+    }
+    if (m1 <= d1) {
       const double p1 = tab[(r1 + m1) * kTile + lane];
       row_dot<DL, OC>(Fb + m1 * rs, dout, oc_n, Ll, inner);
 #pragma unroll
@@ -164,11 +175,22 @@ __device__ __forceinline__ void contract_block(const sbi_bary_desc& d, int b, cp
     cptr<double> Fe = Fb;
     for (int m1 = 0; m1 <= d1; ++m1) {
       const double p1 = tab[(r1 + m1) * kTile + lane];
-      for (int m2 = 0; m2 <= d2; ++m2, Fe += rs) {
+      int m2 = 0;
+      for (; m2 + 1 <= d2; m2 += 2, Fe += 2 * rs) {  // two entries per step, as above
+        double inner2[OC];
+        row_dot<DL, OC>(Fe, dout, oc_n, Ll, inner);
+        row_dot<DL, OC>(Fe + rs, dout, oc_n, Ll, inner2);
+        const double p2 = p1 * tab[(r2 + m2) * kTile + lane];
+        const double q2 = p1 * tab[(r2 + m2 + 1) * kTile + lane];
+#pragma unroll
+        for (int o = 0; o < OC; ++o) acc[o] = fma(q2, inner2[o], fma(p2, inner[o], acc[o]));
+      }
+      if (m2 <= d2) {
         const double p2 = p1 * tab[(r2 + m2) * kTile + lane];
         row_dot<DL, OC>(Fe, dout, oc_n, Ll, inner);
 #pragma unroll
         for (int o = 0; o < OC; ++o) acc[o] = fma(p2, inner[o], acc[o]);
+        Fe += rs;
       }
     }
   } else if (npre == 3) {
