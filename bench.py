@@ -60,23 +60,23 @@ def measured_traffic(kernel, config):
 
 
 def timed_loop(fn, steps, warmup, torch):
-    """Warm up, then time `steps` calls; also per-call HIP event durations on the current stream."""
+    """Warm up, then time `steps` calls between two synchronisations; HIP events recorded on the
+    launch stream at the start and the end of the timed region give the average device time per
+    launch (no event between launches, so the launches queue back to back)."""
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(steps)]
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for a, b in evs:
-        a.record(stream)
+    e0.record(stream)
+    for _ in range(steps):
         fn()
-        b.record(stream)
+    e1.record(stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    ev_ms = [a.elapsed_time(b) for a, b in evs]
-    return (t1 - t0), float(np.mean(ev_ms)), float(np.median(ev_ms))
+    return (t1 - t0), e0.elapsed_time(e1) / steps
 
 
 def bench_gemm_config(args, torch, W, cfg):
@@ -107,7 +107,7 @@ def bench_gemm_config(args, torch, W, cfg):
         def step():
             box["y"] = ip.evaluate(x, precision=name)
 
-        wall, ev_mean_ms, ev_med_ms = timed_loop(step, steps, min(args.warmup, 2), torch)
+        wall, ev_mean_ms = timed_loop(step, steps, min(args.warmup, 2), torch)
         outputs[name] = box["y"].cpu().numpy()
         results[name] = {"value": P * steps / wall, "ms_per_step": wall / steps * 1e3,
                          "steps": steps}
@@ -130,7 +130,7 @@ def bench_gemm_config(args, torch, W, cfg):
             def vstep():
                 box["g"] = ip.vjp(x, v, precision=name)
 
-            wall, _, _ = timed_loop(vstep, args.steps, min(args.warmup, 2), torch)
+            wall, _ = timed_loop(vstep, args.steps, min(args.warmup, 2), torch)
             results["vjp-" + name] = {"value": P * args.steps / wall,
                                       "ms_per_step": wall / args.steps * 1e3}
             outputs["vjp-" + name] = box["g"].cpu().numpy()
@@ -232,7 +232,7 @@ def bench_gradient_integral(ip, x, torch, steps=10):
         js = max(1, min(steps, int(5.0 / max(once, 1e-6))))
         _log(f"[bench] jacobian ({kernel}): {Pg} points, first call {once * 1e3:.1f} ms, "
              f"{js} timed calls")
-        _, jm, _ = timed_loop(jac, js, 1, torch)
+        _, jm = timed_loop(jac, js, 1, torch)
         flops = 2.0 * d_out * terms * Pg
         jbytes = 8.0 * Pg * d_out * d_in + 8.0 * Pg * d_in
         t_s = jm * 1e-3
@@ -251,7 +251,7 @@ def bench_gradient_integral(ip, x, torch, steps=10):
     except Exception as e:  # a documented limit of the Jacobian kernels
         out["gradient"] = {"error": str(e)}
     _log("[bench] integral")
-    _, im, _ = timed_loop(integ, steps, 2, torch)
+    _, im = timed_loop(integ, steps, 2, torch)
     ib = 8.0 * L.k_raw * d_out
     out["integral"] = {"kernel": "bary_integral_partial_kernel + final", "ms": im,
                        "roofline": {"bound": "hbm", "achieved": ib / (im * 1e-3) / 1e9,
@@ -401,11 +401,11 @@ def main():
     results = {}
     outputs = {}
     for name, (fn_name, plan) in paths.items():
-        wall, ev_mean_ms, ev_med_ms = timed_loop(make_step(fn_name, plan), args.steps, args.warmup,
+        wall, ev_mean_ms = timed_loop(make_step(fn_name, plan), args.steps, args.warmup,
                                                  torch)
         outputs[name] = y.cpu().numpy().copy()
         results[name] = {"value": P * args.steps / wall, "ms_per_step": wall / args.steps * 1e3,
-                         "kernel_ms": ev_mean_ms, "kernel_ms_median": ev_med_ms}
+                         "kernel_ms": ev_mean_ms}
 
     # accuracy against the FP64 CPU oracle on a sample of the same points, and of every path
     # against the FP64 GPU path on all points
