@@ -396,7 +396,15 @@ __global__ __launch_bounds__(kTile* kWaves) void newton_mixed_staged_kernel(
           t0 = fmaf(__int_as_float(p3.x), wld(p3.y), t0);
           t1 = fmaf(__int_as_float(p3.z), wld(p3.w), t1);
         }
-        for (; i2 < e2; ++i2) {
+        if (i2 + 2 <= e2) {  // tail: two pairs together, then at most one
+          const v4i p0 = N2[i2], p1 = N2[i2 + 1];
+          t0 = fmaf(__int_as_float(p0.x), wld(p0.y), t0);
+          t1 = fmaf(__int_as_float(p0.z), wld(p0.w), t1);
+          t0 = fmaf(__int_as_float(p1.x), wld(p1.y), t0);
+          t1 = fmaf(__int_as_float(p1.z), wld(p1.w), t1);
+          i2 += 2;
+        }
+        if (i2 < e2) {
           const v4i p0 = N2[i2];
           t0 = fmaf(__int_as_float(p0.x), wld(p0.y), t0);
           t1 = fmaf(__int_as_float(p0.z), wld(p0.w), t1);
