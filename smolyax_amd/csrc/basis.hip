@@ -187,6 +187,78 @@ __global__ __launch_bounds__(kTile* kHWaves) void newton_basis_kernel(sbi_newton
   }
 }
 
+// Factor-table value of row r (table dim i = row_tdim[r], dim j, degree k = r - tab_row[i] + 1)
+// computed from the coordinate, with the same operations as build_h_table.
+template <typename T>
+__device__ __forceinline__ T factor_value(const sbi_newton_desc& d, const double* __restrict__ xp,
+                                          int r) {
+  const int i = C(d.row_tdim)[r];
+  const int j = C(d.tab_dim)[i], k = r - C(d.tab_row)[i] + 1;
+  cptr<double> xi = C(d.dim_xi) + (int64_t)j * d.kmax1;
+  const double xj = xp[j];
+  double pr = 1.0;
+  for (int m = 0; m < k; ++m) pr *= (xj - xi[m]);
+  return (T)(pr * C(d.dim_c)[(int64_t)j * d.kmax1 + k]);
+}
+
+// H^T like newton_basis_kernel when the tile's factor table does not fit in LDS: each factor
+// value is computed from the coordinate where the node needs it (same values, same products).
+template <typename T, typename OUT, bool kQ = false>
+__global__ __launch_bounds__(kTile* kHWaves) void newton_basis_direct_kernel(
+    sbi_newton_desc d, const double* __restrict__ x, int64_t P, int64_t ldx,
+    OUT* __restrict__ HT, int64_t ldh, int chunk, int row0, int row1) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t p_raw = (int64_t)blockIdx.x * kTile + lane;
+  const bool valid = p_raw < P;
+  const int64_t p = valid ? p_raw : (P - 1);
+  const double* __restrict__ xp = x + p * ldx;
+  const int n0 = row0 + blockIdx.y * chunk;
+  const int rend = kQ ? row0 + ((row1 - row0 + 15) & ~15) : row1;
+  const int n1 = min(rend, n0 + chunk);
+  const uint4* __restrict__ desc = reinterpret_cast<const uint4*>(d.basis_desc);
+  auto product = [&](const uint4& dv, int i) {
+    T v = (T)1;
+    const uint32_t w[4] = {(uint32_t)__builtin_amdgcn_readlane((int)dv.x, i),
+                           (uint32_t)__builtin_amdgcn_readlane((int)dv.y, i),
+                           (uint32_t)__builtin_amdgcn_readlane((int)dv.z, i),
+                           (uint32_t)__builtin_amdgcn_readlane((int)dv.w, i)};
+    for (int f = 0; f < 8; ++f) {
+      const uint32_t r = (w[f >> 1] >> ((f & 1) * 16)) & 0xffff;
+      if (r == 0xffff) break;  // rows are packed first
+      v *= factor_value<T>(d, xp, (int)r);
+    }
+    return v;
+  };
+  for (int base = n0 + wave * 64; base < n1; base += kHWaves * 64) {
+    const int mine = base + lane;
+    const uint4 dv = (mine < min(n1, row1)) ? desc[mine]
+                                            : make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu,
+                                                         0xffffffffu);
+    const int cnt = min(64, n1 - base);
+    if constexpr (kQ) {
+      for (int i = 0; i < cnt; i += 4) {
+        float q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const T v = product(dv, i + u);
+          q[u] = (base + i + u < row1) ? (float)v : 0.0f;
+        }
+        if (valid) {
+          typedef float v4f_t __attribute__((ext_vector_type(4)));
+          *reinterpret_cast<v4f_t*>(HT + ((int64_t)((base + i - row0) >> 2) * ldh + p) * 4) =
+              v4f_t{q[0], q[1], q[2], q[3]};
+        }
+      }
+      continue;
+    }
+    for (int i = 0; i < cnt; ++i) {
+      const T v = product(dv, i);
+      if (valid) store_out(HT + (int64_t)(base + i - row0) * ldh + p, (double)v);
+    }
+  }
+}
+
 // Rows of H^T computed directly in FP64 from the coordinates (no factor table): each row is a
 // product of at most 8 factors h_k(x_j), given as (dim, deg) pairs (dim < 0: none).  Used for the
 // few heaviest rows, which the mixed contraction keeps in FP64.
@@ -623,15 +695,23 @@ template <typename T, typename OUT, bool kQ = false>
 static int basis_launch(const sbi_newton_desc& d, const double* x, int64_t P, int64_t ldx, OUT* HT,
                         int64_t ldh, int row0, int row1, hipStream_t st) {
   const size_t lds = (size_t)d.n_tab_rows * kTile * sizeof(T);
-  if (lds > (size_t)kMaxLds) {
-    sbi::set_error("sbi_newton_basis: the tile's factor table exceeds LDS");
+  const bool direct = lds > (size_t)kMaxLds;  // factor values computed where needed
+  if (direct && !d.row_tdim) {
+    sbi::set_error("sbi_newton_basis: the tile's factor table exceeds LDS and the plan has no "
+                   "row_tdim table");
     return SBI_ERR_CAPACITY;
   }
-  sbi::allow_lds((const void*)newton_basis_kernel<T, OUT, kQ>);
   const int chunk = 2048;
   if (row1 <= row0) return SBI_OK;
   const int rows = kQ ? ((row1 - row0 + 15) & ~15) : row1 - row0;
   dim3 grid((unsigned)((P + kTile - 1) / kTile), (unsigned)((rows + chunk - 1) / chunk));
+  if (direct) {
+    hipLaunchKernelGGL((newton_basis_direct_kernel<T, OUT, kQ>), grid, dim3(kTile * kHWaves), 0,
+                       st, d, x, P, ldx, HT, ldh, chunk, row0, row1);
+    SBI_HIP_CHECK(hipGetLastError());
+    return SBI_OK;
+  }
+  sbi::allow_lds((const void*)newton_basis_kernel<T, OUT, kQ>);
   hipLaunchKernelGGL((newton_basis_kernel<T, OUT, kQ>), grid, dim3(kTile * kHWaves), lds, st, d, x,
                      P, ldx, HT, ldh, chunk, row0, row1);
   SBI_HIP_CHECK(hipGetLastError());

@@ -50,6 +50,19 @@ def test_gemm_paths(gpu, d, d_out, m):
     assert rel(ip.evaluate(x, "bf16").cpu().numpy(), yr) < 5e-2
 
 
+def test_gemm_paths_with_factor_table_beyond_lds(gpu):
+    """d_in = 1000: the tile's factor table (764 rows) exceeds LDS in FP64 and in FP32, so the
+    Newton basis computes each factor value from the coordinate where a node needs it; the GEMM
+    paths still match the oracle."""
+    ref, ip, rng = make(1000, 3, 2500, seed=9)
+    assert ip._nlayout.n_tab_rows * 64 * 4 > 160 * 1024  # beyond LDS in FP32 (and FP64)
+    x = rng.uniform(-1, 1, (300, 1000))
+    yr = ref(x)
+    assert rel(ip.evaluate(x, "newton-fp64").cpu().numpy(), yr) < 1e-11
+    assert rel(ip.evaluate(x, "mixed").cpu().numpy(), yr) < 1e-5
+    assert rel(ip.evaluate(x, "bf16").cpu().numpy(), yr) < 5e-2
+
+
 def test_gemm_point_chunks(gpu):
     """Chunking over points (memory_limit) gives the same result."""
     ref, ip, rng = make(8, 24, 300, seed=3)
