@@ -229,7 +229,10 @@ typedef struct sbi_newton_plan sbi_newton_plan;
 int sbi_newton_plan_create(const sbi_newton_desc* desc, sbi_newton_plan** out);
 int sbi_newton_plan_destroy(sbi_newton_plan* plan);
 /* y[P] (FP64) at x[P, ldx]: FP32 surpluses, products and per-group sums; FP64 node differences
- * x - xi, Newton factors and accumulation across groups. */
+ * x - xi, Newton factors and accumulation across groups.  SBI_ERR_CAPACITY when the tile's table
+ * of internal-node products exceeds LDS (n_internal + 1 > 640; FP64: > 320): the plan's basis,
+ * Jacobian and adjoint entry points still work, and y = H S through sbi_newton_basis_rows and
+ * sbi_contract gives the same polynomial. */
 int sbi_newton_eval_mixed(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
                           double* y, void* stream);
 /* 1 if sbi_newton_eval_mixed runs the LDS-staged kernel for this plan (node tables fit in LDS),

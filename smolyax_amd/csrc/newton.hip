@@ -493,11 +493,8 @@ extern "C" int sbi_newton_plan_create(const sbi_newton_desc* desc, sbi_newton_pl
                         ? tot
                         : (size_t)-1;
   }
-  if (p->lds_f32 > (size_t)kMaxLds) {
-    delete p;
-    sbi::set_error("sbi_newton_plan_create: too many internal nodes for the on-chip table");
-    return SBI_ERR_CAPACITY;
-  }
+  // a node table beyond LDS leaves the plan to the basis, Jacobian and adjoint entry points: the
+  // streaming kernels then return SBI_ERR_CAPACITY (the caller takes the GEMM form)
   *out = p;
   return SBI_OK;
 }
@@ -533,7 +530,8 @@ static int newton_launch(const sbi_newton_plan* plan, const double* x, int64_t P
   }
   const size_t lds = sizeof(T) == 4 ? plan->lds_f32 : plan->lds_f64;
   if (lds > (size_t)kMaxLds) {
-    sbi::set_error("sbi_newton_eval: FP64 table exceeds LDS");
+    sbi::set_error(sizeof(T) == 4 ? "sbi_newton_eval: the internal-node table exceeds LDS"
+                                  : "sbi_newton_eval: the FP64 internal-node table exceeds LDS");
     return SBI_ERR_CAPACITY;
   }
   if (P == 0) return SBI_OK;

@@ -372,8 +372,11 @@ class SmolyakBarycentricInterpolator:
                 N.check(rc, "eval")
             else:
                 fn = "sbi_newton_eval_mixed" if precision == "mixed" else "sbi_newton_eval_f64"
-                N.check(N.fn[fn](self._nplan, x.data_ptr(), P, x.stride(0), y.data_ptr(),
-                                 self._stream()), precision)
+                rc = N.fn[fn](self._nplan, x.data_ptr(), P, x.stride(0), y.data_ptr(),
+                              self._stream())
+                if rc == N.SBI_ERR_CAPACITY:  # internal-node table beyond LDS: GEMM form
+                    return self._eval_newton_gemm(x, precision)
+                N.check(rc, precision)
             return y
         if precision == "fp64":
             return self._eval_bary_gemm(x)

@@ -63,6 +63,28 @@ def test_gemm_paths_with_factor_table_beyond_lds(gpu):
     assert rel(ip.evaluate(x, "bf16").cpu().numpy(), yr) < 5e-2
 
 
+def test_newton_plan_with_node_table_beyond_lds(gpu):
+    """d_in = 6, |Lambda| ~ 4000: 1122 internal nodes, so the streaming kernels' node table does
+    not fit in LDS; the plan is kept and __call__ (d_out = 1) takes the basis + contraction, the
+    Jacobian and the adjoint their usual kernels; all match the oracle."""
+    from smolyax_amd import _native as N
+
+    ref, ip, rng = make(6, 1, 4000, seed=12)
+    assert ip._nplan is not None and ip._nlayout.n_internal + 1 > 640
+    x = rng.uniform(-1, 1, (200, 6))
+    y = torch.empty((200, 1), dtype=torch.float64, device="cuda")
+    xt = torch.as_tensor(x, device="cuda")
+    rc = N.fn["sbi_newton_eval_mixed"](ip._nplan, xt.data_ptr(), 200, 6, y.data_ptr(), None)
+    assert rc == N.SBI_ERR_CAPACITY
+    yr = ref(x)
+    assert rel(ip.evaluate(x, "newton-fp64").cpu().numpy(), yr) < 1e-11
+    assert rel(ip.evaluate(x, "mixed").cpu().numpy(), yr) < 1e-5
+    assert rel(ip.gradient(x).cpu().numpy(), ref.gradient(x)) < 1e-10
+    v = rng.standard_normal((200, 1))
+    gr = np.einsum("pm,pmj->pj", v, ref.gradient(x))
+    assert rel(ip.vjp(x, v, "fp64").cpu().numpy(), gr) < 1e-10
+
+
 def test_gemm_point_chunks(gpu):
     """Chunking over points (memory_limit) gives the same result."""
     ref, ip, rng = make(8, 24, 300, seed=3)
