@@ -886,8 +886,8 @@ static int bary_basis_launch(const sbi_plan* plan, const double* x, int64_t P, i
     return SBI_ERR_INVALID;
   }
   const sbi_bary_desc& d = *dp;
-  if (ldx < d.d_in || d.max_n > 16) {
-    sbi::set_error("sbi_bary_basis: invalid argument (at most 16 active dims per block)");
+  if (ldx < d.d_in) {
+    sbi::set_error("sbi_bary_basis: invalid argument (ldx < d_in)");
     return SBI_ERR_INVALID;
   }
   if (P == 0 || d.n_blocks == 0) return SBI_OK;
