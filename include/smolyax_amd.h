@@ -243,12 +243,14 @@ int sbi_newton_eval_f64(const sbi_newton_plan* plan, const double* x, int64_t P,
                         double* y, void* stream);
 
 /* Basis matrix H^T[basis_n, ldh] of the Newton form at x[P, ldx] (dtype 0: FP64, 1: FP32,
- * 2: BF16 rounded from FP32 values); y = H S is then one dense GEMM (large d_out). */
+ * 2: BF16 rounded from FP32 values; 3 and 4: the k-interleaved layouts of sbi_newton_basis_rows);
+ * y = H S is then one dense GEMM (large d_out). */
 int sbi_newton_basis(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx, void* HT,
                      int64_t ldh, int32_t dtype, void* stream);
 /* Rows [row0, row1) of the same basis matrix into HT rows 0 .. row1 - row0 - 1.  dtype 3: FP32 in
  * the k-interleaved layout of sbi_contract_split_k4 (HT[((r / 4) * ldh + p) * 4 + r % 4], rows
- * padded with zeros to a multiple of 16, HT 16-byte aligned). */
+ * padded with zeros to a multiple of 16, HT 16-byte aligned); dtype 4: BF16 in the layout of
+ * sbi_contract_bf16_k8 (HT[((r / 8) * ldh + p) * 8 + r % 8], rows padded to a multiple of 32). */
 int sbi_newton_basis_rows(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
                           int32_t row0, int32_t row1, void* HT, int64_t ldh, int32_t dtype,
                           void* stream);
@@ -327,6 +329,18 @@ int sbi_contract_split_k4(const double* A64, int64_t lda64, const double* B64, i
                           int64_t K64, const float* A32, int64_t lda32, const float* B32,
                           int64_t ldb32, int64_t K32, const double* offset, double* Y,
                           int64_t ldy, int64_t P, int64_t n, void* stream);
+
+/* BF16 form of sbi_contract_split_k4 (gfx950 BF16 matrix cores): the K16 remainder rows of A and
+ * B in BF16, k-interleaved by 8 (element (k, p) of A16 at A16[((k / 8) * lda16 + p) * 8 + k % 8],
+ * element (k, c) of B16 at B16[((k / 8) * ldb16 + c) * 8 + k % 8], K16 a multiple of 32 with zero
+ * rows padding it, both 16-byte aligned); the K64 head rows in FP64 as in sbi_contract_split.  The
+ * BF16 products are exact in FP32; FP32 partial sums are added into FP64 every 1024 terms.  Same
+ * determinism and concurrency guarantees as sbi_contract (no inter-workgroup waits).  Replaces
+ * the BF16 form of the contraction of interpolation.py:264-304 (barycentric.py:123). */
+int sbi_contract_bf16_k8(const double* A64, int64_t lda64, const double* B64, int64_t ldb64,
+                         int64_t K64, const void* A16, int64_t lda16, const void* B16,
+                         int64_t ldb16, int64_t K16, const double* offset, double* Y, int64_t ldy,
+                         int64_t P, int64_t n, void* stream);
 
 /* Descriptor accessors of the plan objects (shared by the library's translation units). */
 int sbi_bary_plan_desc(const sbi_plan* plan, const sbi_bary_desc** out);

@@ -13,6 +13,16 @@ Pinned by (tests/test_oracle_smolyak.py):
   * gradient exactness against analytic derivatives (tests/test_smolyak.py:52-76) and central
     finite differences; NaN at grid nodes,
   * sum of zeta = 1 (constant reproduction).
+
+Extended precision: `__call__`, `gradient` and `integral` take dtype=np.longdouble (x87 80-bit,
+64-bit significand on x86-64).  The FP64 data of the interpolant (node values, f at the nodes,
+quadrature weights) is used as it is; the barycentric weights are recomputed from the FP64 node
+values in the extended type (`compute_weights_ext`), so the second-form basis is the Lagrange
+basis of those nodes to ~1e-19, and every product and sum runs in the extended type.  The result
+is the exact polynomial interpolant of the FP64 data up to ~1e-19 * sum_nu |zeta_nu| |I^nu|,
+far below the FP64 rounding (~1e-16 * the same sum) it serves to measure.  Pinned by
+tests/test_oracle_extended.py: it agrees with exact rational arithmetic (fractions) on small
+sets, and with the FP64 mode to the FP64 rounding the coefficients amplify.
 """
 
 import itertools as it
@@ -24,6 +34,16 @@ import numpy as np
 from . import indices
 from .nodes import compute_weights
 
+EXT = np.longdouble
+
+
+def compute_weights_ext(nodes):
+    """barycentric.py:13-31 in the extended type, from the FP64 node values."""
+    nodes = np.asarray(nodes, dtype=np.float64).astype(EXT)
+    diffs = nodes[:, None] - nodes
+    diffs = np.where(diffs == 0, EXT(1), diffs)
+    return np.prod(EXT(1) / diffs, axis=0)
+
 
 def _basis_unnormalized(x, xi, w, nu_i):
     """barycentric.py:34-66.  x (..., 1), xi/w (..., m) -> (..., m)."""
@@ -34,7 +54,7 @@ def _basis_unnormalized(x, xi, w, nu_i):
     mask_zero = np.any(zero & mask_cols, axis=-1, keepdims=True)
     with np.errstate(divide="ignore", invalid="ignore"):
         wd = w / diffs
-    b = np.where(mask_zero, zero.astype(float), wd)
+    b = np.where(mask_zero, zero.astype(diffs.dtype), wd)
     return np.where(mask_cols, b, 0.0)
 
 
@@ -96,6 +116,7 @@ class Interpolant:
             tmax = max(tau)
             nodes = np.zeros((nn, n, tmax + 1))
             weights = np.zeros((nn, n, tmax + 1))
+            weights_ext = np.zeros((nn, n, tmax + 1), dtype=EXT)
             for s in range(n):
                 grp = defaultdict(list)
                 for i in range(nn):
@@ -104,6 +125,7 @@ class Interpolant:
                     pts = self.node_gen[dim](deg)
                     nodes[idxs, s, : deg + 1] = pts
                     weights[idxs, s, : deg + 1] = compute_weights(pts)
+                    weights_ext[idxs, s, : deg + 1] = compute_weights_ext(pts)
             F = np.zeros((nn,) + tuple(v + 1 for v in tau) + (self.d_out,))
             for i, nu in enumerate(nus):
                 x = zero.copy()
@@ -120,28 +142,36 @@ class Interpolant:
                 if not self.is_nested:
                     f_evals[nu] = fe
             self.groups[n] = dict(
-                F=np.moveaxis(F, -1, 1), nodes=nodes, weights=weights, dims=sdims, degs=sdegs,
+                F=np.moveaxis(F, -1, 1), nodes=nodes, weights=weights, weights_ext=weights_ext,
+                dims=sdims, degs=sdegs,
                 zetas=np.asarray(n2zetas[n], dtype=float), tau=tau, nus=list(nus),
             )
         return f_evals
 
     # ---------------------------------------------------------------- helpers
-    def _batches(self, n, per_summand_elems):
+    def _batches(self, n, per_summand_elems, itemsize=8):
         """Summand batches under memory_limit GiB (interpolation.py:285-287, 327-329)."""
         nn = self.groups[n]["F"].shape[0]
-        mem = per_summand_elems * 8 / 1024**3
+        mem = per_summand_elems * itemsize / 1024**3
         bs = max(1, int(np.floor(self.memory_limit / mem)))
         for s in range(0, nn, bs):
             yield s, min(s + bs, nn)
 
-    def _norm_bases(self, x, g, a, b):
+    @staticmethod
+    def _slot(x, g, a, b, s, m, dt):
+        """Coordinates, nodes and barycentric weights of slot s of blocks a..b in dtype dt."""
+        ext = np.dtype(dt) != np.float64
+        xs = x[:, g["dims"][a:b, s]].T[:, :, None].astype(dt)          # (B, P, 1)
+        xi = g["nodes"][a:b, s, None, :m].astype(dt)                    # (B, 1, m)
+        w = (g["weights_ext"] if ext else g["weights"])[a:b, s, None, :m]
+        return xs, xi, w
+
+    def _norm_bases(self, x, g, a, b, dt=np.float64):
         """Per slot: normalised basis (blocks, P, m_s) as in barycentric.py:114-117."""
         out = []
         for s, m in enumerate(g["tau"]):
             m = m + 1
-            xs = x[:, g["dims"][a:b, s]].T[:, :, None]                  # (B, P, 1)
-            xi = g["nodes"][a:b, s, None, :m]                            # (B, 1, m)
-            w = g["weights"][a:b, s, None, :m]
+            xs, xi, w = self._slot(x, g, a, b, s, m, dt)
             nu = g["degs"][a:b, s][:, None]                              # (B, 1)
             bb = _basis_unnormalized(xs, xi, w, nu)
             out.append(bb / bb.sum(axis=-1, keepdims=True))
@@ -155,37 +185,37 @@ class Interpolant:
         return x
 
     # ---------------------------------------------------------------- __call__ (interpolation.py:264-304)
-    def __call__(self, x):
+    def __call__(self, x, dtype=np.float64):
         assert self.groups or self.d_in == 0 or self.offset is not None
         x = self._prepare(x)
         P = x.shape[0]
-        y = np.broadcast_to(self.offset, (P, self.d_out)).astype(float).copy()
+        dt = np.dtype(dtype)
+        y = np.broadcast_to(np.asarray(self.offset, dtype=dt), (P, self.d_out)).astype(dt).copy()
         for n, g in self.groups.items():
             letters = string.ascii_letters
             fa = letters[:n]
             subs = "zo" + fa + "," + ",".join("zp" + c for c in fa) + "->zpo"
-            for a, b in self._batches(n, P * self.d_out * np.prod(g["F"].shape[3:])):
-                bs = self._norm_bases(x, g, a, b)
-                r = np.einsum(subs, g["F"][a:b], *bs, optimize=True)
-                y += np.einsum("z,zpo->po", g["zetas"][a:b], r)
+            for a, b in self._batches(n, P * self.d_out * np.prod(g["F"].shape[3:]), dt.itemsize):
+                bs = self._norm_bases(x, g, a, b, dt)
+                r = np.einsum(subs, g["F"][a:b].astype(dt), *bs, optimize=True)
+                y += np.einsum("z,zpo->po", g["zetas"][a:b].astype(dt), r)
         return y
 
     # ---------------------------------------------------------------- gradient (interpolation.py:306-345)
-    def gradient(self, x):
+    def gradient(self, x, dtype=np.float64):
         x = self._prepare(x)
         P = x.shape[0]
-        J = np.zeros((P, self.d_out, self.d_in))
+        dt = np.dtype(dtype)
+        J = np.zeros((P, self.d_out, self.d_in), dtype=dt)
         for n, g in self.groups.items():
             letters = string.ascii_letters
             fa = letters[:n]
             subs = "zo" + fa + "," + ",".join("zpq" + c for c in fa) + "->zpoq"
-            for a, b in self._batches(n, J.size * np.prod(g["F"].shape[3:])):
+            for a, b in self._batches(n, J.size * np.prod(g["F"].shape[3:]), dt.itemsize):
                 Bs = []
                 for s, m in enumerate(g["tau"]):
                     m = m + 1
-                    xs = x[:, g["dims"][a:b, s]].T[:, :, None]
-                    xi = g["nodes"][a:b, s, None, :m]
-                    w = g["weights"][a:b, s, None, :m]
+                    xs, xi, w = self._slot(x, g, a, b, s, m, dt)
                     nu = g["degs"][a:b, s][:, None]
                     bb = _basis_unnormalized(xs, xi, w, nu)
                     bn = bb.sum(axis=-1, keepdims=True)
@@ -196,18 +226,19 @@ class Interpolant:
                     B = np.repeat(bh[:, :, None, :], n, axis=2)                  # (B, P, n, m)
                     B[:, :, s, :] = Bi
                     Bs.append(B)
-                r = np.einsum(subs, g["F"][a:b], *Bs, optimize=True)       # (B, P, o, n)
-                r = r * g["zetas"][a:b, None, None, None]
+                r = np.einsum(subs, g["F"][a:b].astype(dt), *Bs, optimize=True)  # (B, P, o, n)
+                r = r * g["zetas"][a:b, None, None, None].astype(dt)
                 for q in range(b - a):
                     J[:, :, g["dims"][a + q]] += r[q]
         return J
 
     # ---------------------------------------------------------------- integral (interpolation.py:347-390)
-    def integral(self):
-        Q = np.broadcast_to(self.offset, (self.d_out,)).astype(float).copy()
+    def integral(self, dtype=np.float64):
+        dt = np.dtype(dtype)
+        Q = np.broadcast_to(np.asarray(self.offset, dtype=dt), (self.d_out,)).astype(dt).copy()
         for n, g in self.groups.items():
             nn = g["F"].shape[0]
-            wl = [np.zeros((nn, v + 1)) for v in g["tau"]]
+            wl = [np.zeros((nn, v + 1), dtype=dt) for v in g["tau"]]
             for s in range(n):
                 grp = defaultdict(list)
                 for i in range(nn):
@@ -218,5 +249,5 @@ class Interpolant:
             fa = string.ascii_lowercase[:n]
             o = string.ascii_lowercase[n]
             subs = "z," + ",".join("z" + c for c in fa) + ",z" + o + fa + "->" + o
-            Q += np.einsum(subs, g["zetas"], *wl, g["F"], optimize=True)
+            Q += np.einsum(subs, g["zetas"].astype(dt), *wl, g["F"].astype(dt), optimize=True)
         return Q

@@ -106,6 +106,8 @@ SIGNATURES = {
     "sbi_newton_rows_f64": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i32, c_vp, c_i64, c_vp]),
     "sbi_contract_split_k4": (c_i32, [c_vp, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64,
                                       c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp]),
+    "sbi_contract_bf16_k8": (c_i32, [c_vp, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64,
+                                     c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp]),
     "sbi_contract_split": (c_i32, [c_vp, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64,
                                    c_vp, c_vp, c_i64, c_i64, c_i64, c_vp]),
     "sbi_contract": (c_i32, [c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64,

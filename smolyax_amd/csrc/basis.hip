@@ -128,12 +128,48 @@ __device__ __forceinline__ T node_product(const LT* tab, const uint4& dv, int i,
   return v * tab[(w3 >> 16) * kTile + lane];
 }
 
+// k-interleaved outputs for the contraction (kQ = 4: FP32, kQ = 8: BF16): row r = row - row0 of
+// point p at HT[((r / kQ) * ldh + p) * kQ + r % kQ], rows padded with zeros to a multiple of 16
+// (FP32) or 32 (BF16, one MFMA k step); each lane stores the kQ rows of a group as one 16-byte
+// vector.
+template <int kQ>
+struct kq_traits {
+  static constexpr int pad = 1;
+};
+template <>
+struct kq_traits<4> {
+  static constexpr int pad = 16;
+};
+template <>
+struct kq_traits<8> {
+  static constexpr int pad = 32;
+};
+
+__device__ __forceinline__ uint32_t bf16_bits(float v) {
+  const __hip_bfloat16 b = __float2bfloat16(v);
+  return (uint32_t)__bfloat16_as_ushort(b);
+}
+
+template <int kQ>
+__device__ __forceinline__ void store_group(void* HT, int64_t ldh, int r, int64_t p,
+                                            const float* q) {
+  if constexpr (kQ == 4) {
+    typedef float v4f_t __attribute__((ext_vector_type(4)));
+    *reinterpret_cast<v4f_t*>((float*)HT + ((int64_t)(r >> 2) * ldh + p) * 4) =
+        v4f_t{q[0], q[1], q[2], q[3]};
+  } else {
+    uint4 v;
+    v.x = bf16_bits(q[0]) | (bf16_bits(q[1]) << 16);
+    v.y = bf16_bits(q[2]) | (bf16_bits(q[3]) << 16);
+    v.z = bf16_bits(q[4]) | (bf16_bits(q[5]) << 16);
+    v.w = bf16_bits(q[6]) | (bf16_bits(q[7]) << 16);
+    reinterpret_cast<uint4*>(HT)[(int64_t)(r >> 3) * ldh + p] = v;
+  }
+}
+
 // H^T[nu, p] for a chunk of nodes.  Node descriptor: 8 x u16 table rows (0xffff = none).
-//
-// kQ (FP32 only): k-interleaved output for the contraction, HT[((r / 4) * ldh + p) * 4 + r % 4]
-// for row r = row - row0, rows padded with zeros to a multiple of 16; each lane stores the four
-// rows of a quad as one 16-byte vector.
-template <typename T, typename OUT, bool kQ = false>
+// kQ = 0: plain layout HT[r * ldh + p] of type OUT; kQ = 4 / 8: k-interleaved (see above).
+template <typename T, typename OUT, int kQ = 0>
 __global__ __launch_bounds__(kTile* kHWaves) void newton_basis_kernel(sbi_newton_desc d,
                                                                       const double* __restrict__ x,
                                                                       int64_t P, int64_t ldx,
@@ -152,7 +188,8 @@ __global__ __launch_bounds__(kTile* kHWaves) void newton_basis_kernel(sbi_newton
   __syncthreads();
 
   const int n0 = row0 + blockIdx.y * chunk;
-  const int rend = kQ ? row0 + ((row1 - row0 + 15) & ~15) : row1;
+  constexpr int kPad = kq_traits<kQ>::pad;
+  const int rend = kQ ? row0 + ((row1 - row0 + kPad - 1) & ~(kPad - 1)) : row1;
   const int n1 = min(rend, n0 + chunk);
   const uint4* __restrict__ desc = reinterpret_cast<const uint4*>(d.basis_desc);
   // each wave: 64-node groups, descriptors one per lane, broadcast with v_readlane
@@ -163,20 +200,17 @@ __global__ __launch_bounds__(kTile* kHWaves) void newton_basis_kernel(sbi_newton
                                                          0xffffffffu);
     const int nfl = desc_factor_count(dv);
     const int cnt = min(64, n1 - base);
-    if constexpr (kQ) {
-      for (int i = 0; i < cnt; i += 4) {  // cnt is a multiple of 4 (16-row padding, 64-row groups)
-        float q[4];
+    if constexpr (kQ != 0) {
+      // cnt is a multiple of kQ (row padding to kPad, 64-row groups)
+      for (int i = 0; i < cnt; i += kQ) {
+        float q[kQ];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kQ; ++u) {
           const T v = node_product<T>(tab, dv, i + u,
                                       __builtin_amdgcn_readlane(nfl, i + u), lane);
           q[u] = (base + i + u < row1) ? (float)v : 0.0f;
         }
-        if (valid) {
-          typedef float v4f_t __attribute__((ext_vector_type(4)));
-          *reinterpret_cast<v4f_t*>(HT + ((int64_t)((base + i - row0) >> 2) * ldh + p) * 4) =
-              v4f_t{q[0], q[1], q[2], q[3]};
-        }
+        if (valid) store_group<kQ>(HT, ldh, base + i - row0, p, q);
       }
       continue;
     }
@@ -203,7 +237,7 @@ __device__ __forceinline__ T factor_value(const sbi_newton_desc& d, const double
 
 // H^T like newton_basis_kernel when the tile's factor table does not fit in LDS: each factor
 // value is computed from the coordinate where the node needs it (same values, same products).
-template <typename T, typename OUT, bool kQ = false>
+template <typename T, typename OUT, int kQ = 0>
 __global__ __launch_bounds__(kTile* kHWaves) void newton_basis_direct_kernel(
     sbi_newton_desc d, const double* __restrict__ x, int64_t P, int64_t ldx,
     OUT* __restrict__ HT, int64_t ldh, int chunk, int row0, int row1) {
@@ -214,7 +248,8 @@ __global__ __launch_bounds__(kTile* kHWaves) void newton_basis_direct_kernel(
   const int64_t p = valid ? p_raw : (P - 1);
   const double* __restrict__ xp = x + p * ldx;
   const int n0 = row0 + blockIdx.y * chunk;
-  const int rend = kQ ? row0 + ((row1 - row0 + 15) & ~15) : row1;
+  constexpr int kPad = kq_traits<kQ>::pad;
+  const int rend = kQ ? row0 + ((row1 - row0 + kPad - 1) & ~(kPad - 1)) : row1;
   const int n1 = min(rend, n0 + chunk);
   const uint4* __restrict__ desc = reinterpret_cast<const uint4*>(d.basis_desc);
   auto product = [&](const uint4& dv, int i) {
@@ -236,19 +271,15 @@ __global__ __launch_bounds__(kTile* kHWaves) void newton_basis_direct_kernel(
                                             : make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu,
                                                          0xffffffffu);
     const int cnt = min(64, n1 - base);
-    if constexpr (kQ) {
-      for (int i = 0; i < cnt; i += 4) {
-        float q[4];
+    if constexpr (kQ != 0) {
+      for (int i = 0; i < cnt; i += kQ) {
+        float q[kQ];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kQ; ++u) {
           const T v = product(dv, i + u);
           q[u] = (base + i + u < row1) ? (float)v : 0.0f;
         }
-        if (valid) {
-          typedef float v4f_t __attribute__((ext_vector_type(4)));
-          *reinterpret_cast<v4f_t*>(HT + ((int64_t)((base + i - row0) >> 2) * ldh + p) * 4) =
-              v4f_t{q[0], q[1], q[2], q[3]};
-        }
+        if (valid) store_group<kQ>(HT, ldh, base + i - row0, p, q);
       }
       continue;
     }
@@ -691,7 +722,7 @@ __global__ __launch_bounds__(kTile) void bary_basis_kernel(sbi_bary_desc d,
 
 extern "C" int sbi_newton_plan_desc(const sbi_newton_plan* plan, const sbi_newton_desc** out);
 
-template <typename T, typename OUT, bool kQ = false>
+template <typename T, typename OUT, int kQ = 0>
 static int basis_launch(const sbi_newton_desc& d, const double* x, int64_t P, int64_t ldx, OUT* HT,
                         int64_t ldh, int row0, int row1, hipStream_t st) {
   const size_t lds = (size_t)d.n_tab_rows * kTile * sizeof(T);
@@ -703,7 +734,8 @@ static int basis_launch(const sbi_newton_desc& d, const double* x, int64_t P, in
   }
   const int chunk = 2048;
   if (row1 <= row0) return SBI_OK;
-  const int rows = kQ ? ((row1 - row0 + 15) & ~15) : row1 - row0;
+  constexpr int kPad = kq_traits<kQ>::pad;
+  const int rows = kQ ? ((row1 - row0 + kPad - 1) & ~(kPad - 1)) : row1 - row0;
   dim3 grid((unsigned)((P + kTile - 1) / kTile), (unsigned)((rows + chunk - 1) / chunk));
   if (direct) {
     hipLaunchKernelGGL((newton_basis_direct_kernel<T, OUT, kQ>), grid, dim3(kTile * kHWaves), 0,
@@ -744,14 +776,18 @@ extern "C" int sbi_newton_basis_rows(const sbi_newton_plan* plan, const double* 
       return basis_launch<float, __hip_bfloat16>(d, x, P, ldx, (__hip_bfloat16*)HT, ldh, row0,
                                                  row1, st);
     case 3:
+    case 4:
       if (((uintptr_t)HT & 15) != 0) {
-        sbi::set_error("sbi_newton_basis: the k-interleaved layout needs a 16-byte aligned HT");
+        sbi::set_error("sbi_newton_basis: the k-interleaved layouts need a 16-byte aligned HT");
         return SBI_ERR_INVALID;
       }
-      return basis_launch<float, float, true>(d, x, P, ldx, (float*)HT, ldh, row0, row1, st);
+      if (dtype == 3)
+        return basis_launch<float, float, 4>(d, x, P, ldx, (float*)HT, ldh, row0, row1, st);
+      return basis_launch<float, __hip_bfloat16, 8>(d, x, P, ldx, (__hip_bfloat16*)HT, ldh, row0,
+                                                     row1, st);
     default:
-      sbi::set_error("sbi_newton_basis: dtype must be 0 (f64), 1 (f32), 2 (bf16) or 3 (f32, "
-                     "k-interleaved)");
+      sbi::set_error("sbi_newton_basis: dtype must be 0 (f64), 1 (f32), 2 (bf16), 3 (f32, "
+                     "k-interleaved by 4) or 4 (bf16, k-interleaved by 8)");
       return SBI_ERR_INVALID;
   }
 }

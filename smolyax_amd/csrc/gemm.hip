@@ -281,6 +281,141 @@ int contract_launch(const T* A, int64_t lda, const T* B, int64_t ldb, const doub
 }
 
 // ------------------------------------------------------------------------------------------
+// BF16 contraction (gfx950 v_mfma_f32_16x16x32_bf16): Y = offset + A64^T B64 + A^T B with A and B
+// in BF16, k-interleaved by 8 (element (k, p) of A at A[((k / 8) * lda + p) * 8 + k % 8], the same
+// for B; K a multiple of 32, zero rows pad it), and an optional FP64 head of K64 rows in the plain
+// layout (FP64 MFMA, as in contract_kernel).  BF16 products are exact in FP32; the FP32 partial
+// sums are added into FP64 registers every kBfFlush k.
+//
+// Work split: a workgroup of 4 waves owns a 128-point x 128-output tile; each wave a 64 x 64
+// sub-tile of 4 x 4 MFMA 16x16 fragments.  Per 32-k step a lane loads one 16-byte vector (its
+// eight k) per fragment row and column, the next step's vectors in flight during the current one.
+// Lane group g = lane / 16 supplies k = 8g .. 8g + 7 of the step to both operands.  Every
+// workgroup is independent and each Y row depends only on its own column of A (fixed order).
+// ------------------------------------------------------------------------------------------
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+constexpr int kBfWM = 4, kBfWN = 4;           // fragments per wave (points, outputs)
+constexpr int kBfTileM = 32 * kBfWM;          // points per workgroup (2 waves along M)
+constexpr int kBfTileN = 32 * kBfWN;          // outputs per workgroup (2 waves along N)
+constexpr int kBfFlush = 1024;                // k per FP32 partial sum
+
+__device__ __forceinline__ v8bf as_v8bf(const uint4& u) {
+  v8bf r;
+  __builtin_memcpy(&r, &u, sizeof(r));
+  return r;
+}
+
+__global__ __launch_bounds__(256) void contract_bf16_kernel(
+    const uint4* __restrict__ A, int64_t lda, const uint4* __restrict__ B, int64_t ldb,
+    int64_t K, const double* __restrict__ A64, int64_t lda64, const double* __restrict__ B64,
+    int64_t ldb64, int64_t K64, const double* __restrict__ offset, double* __restrict__ Y,
+    int64_t ldy, int64_t P, int64_t n) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int li = lane & 15, g = lane >> 4;
+  const int64_t m_base = (int64_t)blockIdx.x * kBfTileM + wm * (16 * kBfWM);
+  const int64_t n_base = (int64_t)blockIdx.y * kBfTileN + wn * (16 * kBfWN);
+  int64_t pcol[kBfWM], jcol[kBfWN];
+#pragma unroll
+  for (int fm = 0; fm < kBfWM; ++fm) pcol[fm] = min(m_base + fm * 16 + li, P - 1);
+#pragma unroll
+  for (int fn = 0; fn < kBfWN; ++fn) jcol[fn] = min(n_base + fn * 16 + li, n - 1);
+
+  v4d acc64[kBfWM][kBfWN];
+#pragma unroll
+  for (int fm = 0; fm < kBfWM; ++fm)
+#pragma unroll
+    for (int fn = 0; fn < kBfWN; ++fn) acc64[fm][fn] = v4d{0, 0, 0, 0};
+
+  if (K64 > 0) {  // FP64 head: lane li supplies point 4 * (li & 3) + (li >> 2) of its block, so
+                  // the FP64 MFMA's rows land in the FP32 layout (row 4g + r in register r)
+    int64_t ph[kBfWM];
+#pragma unroll
+    for (int fm = 0; fm < kBfWM; ++fm)
+      ph[fm] = min(m_base + fm * 16 + 4 * (li & 3) + (li >> 2), P - 1);
+    for (int64_t k0 = 0; k0 < K64; k0 += 4) {
+      const int64_t k = k0 + g;
+      const bool in = k < K64;
+      const int64_t kc = in ? k : K64 - 1;
+      double ha[kBfWM], hb[kBfWN];
+#pragma unroll
+      for (int fm = 0; fm < kBfWM; ++fm) {
+        const double v = A64[kc * lda64 + ph[fm]];
+        ha[fm] = in ? v : 0.0;
+      }
+#pragma unroll
+      for (int fn = 0; fn < kBfWN; ++fn) {
+        const double v = B64[kc * ldb64 + jcol[fn]];
+        hb[fn] = in ? v : 0.0;
+      }
+#pragma unroll
+      for (int fm = 0; fm < kBfWM; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < kBfWN; ++fn)
+          acc64[fm][fn] = __builtin_amdgcn_mfma_f64_16x16x4f64(ha[fm], hb[fn], acc64[fm][fn], 0,
+                                                               0, 0);
+    }
+  }
+
+  v4f acc[kBfWM][kBfWN];
+#pragma unroll
+  for (int fm = 0; fm < kBfWM; ++fm)
+#pragma unroll
+    for (int fn = 0; fn < kBfWN; ++fn) acc[fm][fn] = v4f{0, 0, 0, 0};
+  uint4 a[kBfWM], b[kBfWN];
+  auto load = [&](int64_t kq) {  // kq: index of the lane's 8-k group
+#pragma unroll
+    for (int fm = 0; fm < kBfWM; ++fm) a[fm] = A[kq * lda + pcol[fm]];
+#pragma unroll
+    for (int fn = 0; fn < kBfWN; ++fn) b[fn] = B[kq * ldb + jcol[fn]];
+  };
+  auto flush = [&]() {
+#pragma unroll
+    for (int fm = 0; fm < kBfWM; ++fm)
+#pragma unroll
+      for (int fn = 0; fn < kBfWN; ++fn) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc64[fm][fn][r] += (double)acc[fm][fn][r];
+        acc[fm][fn] = v4f{0, 0, 0, 0};
+      }
+  };
+  if (K > 0) {
+    load(g);
+    for (int64_t k0 = 0; k0 < K; k0 += 32) {
+      uint4 ca[kBfWM], cb[kBfWN];
+#pragma unroll
+      for (int fm = 0; fm < kBfWM; ++fm) ca[fm] = a[fm];
+#pragma unroll
+      for (int fn = 0; fn < kBfWN; ++fn) cb[fn] = b[fn];
+      if (k0 + 32 < K) load(((k0 + 32) >> 3) + g);  // next step in flight during this one
+#pragma unroll
+      for (int fm = 0; fm < kBfWM; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < kBfWN; ++fn)
+          acc[fm][fn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_v8bf(ca[fm]), as_v8bf(cb[fn]),
+                                                                acc[fm][fn], 0, 0, 0);
+      if (((k0 + 32) % kBfFlush) == 0) flush();
+    }
+    flush();
+  }
+
+#pragma unroll
+  for (int fn = 0; fn < kBfWN; ++fn) {
+    const int64_t col = n_base + fn * 16 + li;
+    if (col >= n) continue;
+    const double off = offset ? offset[col] : 0.0;
+#pragma unroll
+    for (int fm = 0; fm < kBfWM; ++fm)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = m_base + fm * 16 + 4 * g + r;
+        if (row < P) Y[row * ldy + col] = off + acc64[fm][fn][r];
+      }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
 // Jacobian contraction: for each input dim z, J[p, c, z] = sum_{k in [zoff[z], zoff[z+1])}
 // A[k, p] B[brow[k * brs], c] (B rows gathered per row of A).  A workgroup owns a
 // 64-point x (32 * FN)-output tile for JG consecutive dims, streams their rows
@@ -509,4 +644,28 @@ extern "C" int sbi_contract_split(const double* A64, int64_t lda64, const double
   if (P == 0 || n == 0) return SBI_OK;
   return contract_launch<float>(A32, lda32, B32, ldb32, offset, Y, ldy, P, n, K32, A64, lda64,
                                 B64, ldb64, K64, (hipStream_t)stream);
+}
+
+extern "C" int sbi_contract_bf16_k8(const double* A64, int64_t lda64, const double* B64,
+                                    int64_t ldb64, int64_t K64, const void* A16, int64_t lda16,
+                                    const void* B16, int64_t ldb16, int64_t K16,
+                                    const double* offset, double* Y, int64_t ldy, int64_t P,
+                                    int64_t n, void* stream) {
+  if (P < 0 || n < 0 || K64 < 0 || K16 < 0 || K16 % 32 != 0 ||
+      (P > 0 && n > 0 && (!Y || ldy < n)) ||
+      (P > 0 && n > 0 && K64 > 0 && (!A64 || !B64 || lda64 < P || ldb64 < n)) ||
+      (P > 0 && n > 0 && K16 > 0 &&
+       (!A16 || !B16 || lda16 < P || ldb16 < n || ((uintptr_t)A16 & 15) || ((uintptr_t)B16 & 15))) ||
+      (P + kBfTileM - 1) / kBfTileM > 0x7fffffff || (n + kBfTileN - 1) / kBfTileN > 65535) {
+    sbi::set_error("sbi_contract_bf16_k8: invalid argument (K16 must be a multiple of 32, the "
+                   "k-interleaved operands 16-byte aligned)");
+    return SBI_ERR_INVALID;
+  }
+  if (P == 0 || n == 0) return SBI_OK;
+  const dim3 grid((unsigned)((P + kBfTileM - 1) / kBfTileM), (unsigned)((n + kBfTileN - 1) / kBfTileN));
+  hipLaunchKernelGGL(contract_bf16_kernel, grid, dim3(256), 0, (hipStream_t)stream,
+                     (const uint4*)A16, lda16, (const uint4*)B16, ldb16, K16, A64, lda64, B64,
+                     ldb64, K64, offset, Y, ldy, P, n);
+  SBI_HIP_CHECK(hipGetLastError());
+  return SBI_OK;
 }

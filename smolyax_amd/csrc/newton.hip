@@ -73,12 +73,14 @@ __device__ __forceinline__ XWin load_window(const double* __restrict__ xp, int j
   for (int i = 0; i < kXW; ++i) w.v[i] = xp[min(j0 + i, d_in - 1)];
   return w;
 }
-// Same window with two coordinates per 16-byte load (row stride even, x 16-byte aligned,
-// d_in >= 2): half the load instructions, each of which touches one cache line per lane.  Pairs
-// past the row end re-read the last full pair (even index, in bounds); those values are unused.
+// Same window with two coordinates per 16-byte load (row stride even, x 16-byte aligned, d_in
+// even and >= 2): half the load instructions, each of which touches one cache line per lane.
+// Pairs past the row end re-read the last pair (index d_in - 2, in bounds); those values are
+// unused.  With an odd d_in the last coordinate has no partner inside the row, so the launcher
+// takes the 8-byte form for such plans.
 typedef double xv2d __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ XWin load_window2(const double* __restrict__ xp, int j0, int d_in) {
-  const int jmax = (d_in - 2) & ~1;
+  const int jmax = d_in - 2;
   XWin w;
 #pragma unroll
   for (int q = 0; q < kXW / 2; ++q) {
@@ -556,8 +558,10 @@ extern "C" int sbi_newton_eval_mixed(const sbi_newton_plan* plan, const double* 
   sbi::allow_lds((const void*)newton_mixed_staged_kernel<true>);
   sbi::allow_lds((const void*)newton_mixed_staged_kernel<false>);
   dim3 grid((unsigned)((P + kTile - 1) / kTile));
-  // 16-byte coordinate loads need an even row stride, a 16-byte aligned x and d_in >= 2
-  const bool pairs = (ldx % 2 == 0) && ((uintptr_t)x % 16 == 0) && plan->d.d_in >= 2;
+  // 16-byte coordinate loads need an even row stride, a 16-byte aligned x and an even d_in >= 2
+  // (every pair {2i, 2i+1} with 2i < d_in then lies inside the row)
+  const bool pairs = (ldx % 2 == 0) && ((uintptr_t)x % 16 == 0) && plan->d.d_in >= 2 &&
+                     plan->d.d_in % 2 == 0;
   if (pairs)
     hipLaunchKernelGGL(newton_mixed_staged_kernel<true>, grid, dim3(kTile * kWaves),
                        plan->lds_staged, (hipStream_t)stream, plan->d, x, P, ldx, y);

@@ -7,6 +7,7 @@ device; inputs may be numpy arrays or torch tensors, results are float64 torch t
 """
 
 import ctypes
+import warnings
 from typing import Callable, Sequence
 
 import numpy as np
@@ -19,6 +20,16 @@ from .nodes import Generator
 PRECISIONS = ("fp64", "mixed", "newton-fp64", "bf16", "bary-fp32")
 GEMM_MIN_DOUT = 16  # from this d_out on, the d_out contraction is one dense GEMM
 MAX_CALL_POINTS = 65535 * 64  # points per call of sbi_contract / _jacobian_f64 / sbi_newton_vjp
+# Newton form of a non-nested interpolant on a nested proxy grid (see _newton_proxy): its largest
+# relative distance from the FP64 barycentric path on sample points decides what it may serve.
+PROXY_TOL_FP64 = 1e-12   # up to this: every path (newton-fp64, gradient, adjoint in FP64)
+PROXY_TOL_MIXED = 1e-7   # up to this: the FP32-accurate paths (mixed, bf16) only
+PROXY_SAMPLES = 256
+
+
+class SmolyaxPrecisionWarning(UserWarning):
+    """A requested precision path is unavailable for an interpolant and a less accurate one
+    serves the call (stated in the message)."""
 
 
 def _dev_tensor(a, dtype, device):
@@ -54,13 +65,17 @@ class SmolyakBarycentricInterpolator:
                         differences; d_out == 1: FP64 accumulation across groups; larger d_out:
                         the heaviest rows in FP64, the rest FP32 on MFMA with FP64 accumulation;
           "newton-fp64" hierarchical Newton form in FP64;
-          "bf16"        hierarchical Newton form, BF16 basis and surpluses with FP32 accumulation
-                        on MFMA;
+          "bf16"        hierarchical Newton form: the heaviest rows in FP64, the rest with BF16
+                        basis and surpluses on the BF16 matrix cores (FP32 partial sums added into
+                        FP64);
           "bary-fp32"   barycentric form with FP64 node differences, FP32 entry weights and FP32
                         contraction on MFMA.
         The Newton form is built on the interpolant's own nodes when they are nested (Leja) and on
         a nested proxy grid otherwise (Gauss-Hermite: the same polynomial, see _newton_proxy);
-        when the index set exceeds its layout limits "mixed" falls back to "bary-fp32".
+        when the index set exceeds the Newton layout's limits (more than 6 factors in a node, or
+        a factor table beyond 65534 rows), "mixed" is served by "bary-fp32" and a
+        SmolyaxPrecisionWarning says so (`mixed_path` names the path in use); "newton-fp64" and
+        "bf16" raise ValueError.
         """
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {PRECISIONS}")
@@ -79,6 +94,9 @@ class SmolyakBarycentricInterpolator:
         self._plan = None
         self._nplan = None
         self._newton_off = "set_f has not been called"
+        self._proxy_err = None      # non-nested: proxy Newton form vs the FP64 barycentric path
+        self._newton_fp64_ok = False
+        self._warned_mixed = False
         self._layout = None
         if f is not None:
             self.set_f(f=f)
@@ -103,6 +121,8 @@ class SmolyakBarycentricInterpolator:
         self._nlayout = None
         self._nform = None
         self._newton_off = None  # why the Newton form is unavailable (None: available)
+        self._newton_fp64_ok = self._is_nested
+        self._proxy_err = None
         if self._is_nested:
             try:
                 nf = newton.NewtonForm(self._k, self._t, self._node_gen, self._d_out)
@@ -122,6 +142,8 @@ class SmolyakBarycentricInterpolator:
             except newton.CapacityError as e:
                 self._nform = self._nlayout = None
                 self._newton_off = str(e)
+            if self._nplan is not None:
+                self._check_proxy()
         if self._newton_off and self._precision in ("newton-fp64", "bf16"):
             raise ValueError(f"precision {self._precision!r}: {self._newton_off}")
         if self._n_inputs is not None:
@@ -145,6 +167,7 @@ class SmolyakBarycentricInterpolator:
         for nu in indices.indexset(self._k, self._t):
             for j, v in nu:
                 maxdeg[j] = max(maxdeg[j], v)
+        self._maxdeg = maxdeg
         dom = np.zeros((d, 2))
         for j, g in enumerate(self._node_gen):
             pts = np.asarray(g(int(maxdeg[j])), dtype=np.float64)
@@ -160,6 +183,44 @@ class SmolyakBarycentricInterpolator:
         self._n_inputs = keep
         nf.set_values(Y.cpu().numpy())
         return nf
+
+    def _proxy_samples(self):
+        """Sample points of a non-nested interpolant: half from its nodes' measure (Gauss-Hermite:
+        mean + scaling * N(0, 1/2) per dim), half uniform over the span of each dim's nodes."""
+        rng = np.random.default_rng(12345)
+        d, n = self._d_in, PROXY_SAMPLES // 2
+        Z = rng.standard_normal((n, d)) / np.sqrt(2.0)
+        A = np.stack([np.asarray(g.scale(Z[:, j]), dtype=np.float64)
+                      for j, g in enumerate(self._node_gen)], axis=1)
+        B = np.empty((n, d))
+        for j, g in enumerate(self._node_gen):
+            pts = np.asarray(g(int(self._maxdeg[j])), dtype=np.float64)
+            B[:, j] = rng.uniform(pts.min(), pts.max(), n) if pts.size > 1 else pts[0]
+        return np.concatenate([A, B])
+
+    def _check_proxy(self):
+        """Measure the proxy Newton form against the FP64 barycentric path (the reference's
+        algorithm) on sample points and keep it only for the paths whose accuracy it meets: all
+        of them up to PROXY_TOL_FP64, the FP32-accurate ones up to PROXY_TOL_MIXED, none above.
+        The proxy's error grows with the degree (its hierarchisation runs on nodes spanning the
+        Gauss-Hermite ones; at degree ~30 it reaches ~1e-8)."""
+        X = torch.as_tensor(self._proxy_samples(), device=self._device)
+        keep = self._n_inputs
+        y64 = self.evaluate(X, precision="fp64")
+        self._newton_fp64_ok = True  # for this measurement only
+        yn = self.evaluate(X, precision="newton-fp64")
+        self._n_inputs = keep
+        scale = float(y64.abs().max())
+        err = float((yn - y64).abs().max()) / scale if scale > 0 else float((yn - y64).abs().max())
+        self._proxy_err = err
+        self._newton_fp64_ok = err <= PROXY_TOL_FP64
+        if err > PROXY_TOL_MIXED:
+            self._newton_off = (f"the Newton form on the proxy grid is {err:.1e} from the FP64 "
+                                f"barycentric path (above {PROXY_TOL_MIXED:.0e})")
+            for attr, fn in (("_nplan", "sbi_newton_plan_destroy"),):
+                N.fn[fn](getattr(self, attr))
+            self._nplan = None
+            self._nform = self._nlayout = None
 
     def _upload(self):
         if not torch.cuda.is_available():
@@ -350,7 +411,17 @@ class SmolyakBarycentricInterpolator:
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {PRECISIONS}")
         if precision == "mixed" and self._nplan is None:
+            if not self._warned_mixed:
+                warnings.warn(f"precision 'mixed': the Newton form is unavailable "
+                              f"({self._newton_off}); evaluating with 'bary-fp32' (FP32 "
+                              f"barycentric, about 1e-4 relative)", SmolyaxPrecisionWarning,
+                              stacklevel=2)
+                self._warned_mixed = True
             precision = "bary-fp32"
+        if precision == "newton-fp64" and self._nplan is not None and not self._newton_fp64_ok:
+            raise ValueError(f"precision 'newton-fp64': the proxy Newton form of this non-nested "
+                             f"interpolant is {self._proxy_err:.1e} from the FP64 barycentric "
+                             f"path (above {PROXY_TOL_FP64:.0e}); use 'fp64'")
         x = self._validate_input(x)
         P = x.shape[0]
         if P == 0:  # empty batch: nothing to launch (an empty tensor may have no data pointer)
@@ -435,6 +506,16 @@ class SmolyakBarycentricInterpolator:
             store[kind] = S
         return S
 
+    def _surpluses_t(self, kind):
+        """Transposed surpluses S^T (d_out x nodes, nf.nus order) in dtype `kind` ("f64" or
+        "f32"): the A operand of the adjoint's contraction G^T = S V^T."""
+        key = kind + "T"
+        ST = self._S.get(key)
+        if ST is None:
+            ST = self._surpluses(kind).t().contiguous()
+            self._S[key] = ST
+        return ST
+
     def _surpluses_k4(self, nh):
         """FP32 surpluses of basis rows nh.. in the k-interleaved layout of sbi_contract_split_k4:
         (K32 / 4, d_out, 4) with K32 = rows rounded up to a multiple of 16 (zero rows)."""
@@ -448,60 +529,79 @@ class SmolyakBarycentricInterpolator:
             self._Sg["f32k4"] = S
         return S
 
+    def _surpluses_k8(self, nh):
+        """BF16 surpluses of basis rows nh.. in the k-interleaved layout of sbi_contract_bf16_k8:
+        (K16 / 8, d_out, 8) with K16 = rows rounded up to a multiple of 32 (zero rows); values
+        rounded FP64 -> FP32 -> BF16 like the basis kernel's."""
+        S = self._Sg.get("bf16k8")
+        if S is None:
+            rows = self._surpluses("f32", gemm=True)[nh:]
+            k16 = -(-rows.shape[0] // 32) * 32
+            pad = torch.zeros((k16, self._d_out), dtype=torch.float32, device=self._device)
+            pad[: rows.shape[0]] = rows
+            S = pad.view(k16 // 8, 8, self._d_out).permute(0, 2, 1).contiguous().to(torch.bfloat16)
+            self._Sg["bf16k8"] = S
+        return S
+
     def _eval_newton_gemm(self, x, precision):
         """y = H S: Newton basis matrix (HIP kernel), then the contraction with the surpluses on
-        the matrix cores: sbi_contract for FP64; for FP32 ("mixed") the heaviest rows in FP64 and
-        the rest in FP32 with FP64 accumulation (sbi_contract_split_k4, k-interleaved operands);
-        BF16 operands use the library GEMM (FP32 accumulation)."""
+        the matrix cores: sbi_contract for FP64; for "mixed" (FP32) and "bf16" the heaviest rows
+        in FP64 (sbi_newton_rows_f64, FP64 MFMA) and the rest in FP32 (sbi_contract_split_k4) or
+        BF16 (sbi_contract_bf16_k8), k-interleaved operands, FP64 accumulation."""
         P = x.shape[0]
         Nn = self._nlayout.basis_n
-        kind, code, dt = {"newton-fp64": ("f64", 0, torch.float64), "mixed": ("f32", 1, torch.float32),
-                          "bf16": ("bf16", 2, torch.bfloat16)}[precision]
-        S = self._surpluses(kind, gemm=True)
+        kind = {"newton-fp64": "f64", "mixed": "f32", "bf16": "bf16"}[precision]
         y = torch.empty((P, self._d_out), dtype=torch.float64, device=self._device)
-        pc = self._point_chunk(Nn, torch.finfo(dt).bits // 8, P)
-        nh = self._nlayout.n_head if kind == "f32" else 0
-        k4 = kind == "f32"  # FP32 remainder in the k-interleaved layout (16-byte operand loads)
-        if k4:
-            S64 = self._surpluses("f64", gemm=True)[:nh]
-            S32 = self._surpluses_k4(nh)
-            K32 = S32.shape[0] * 4
+        if kind == "f64":
+            S = self._surpluses("f64", gemm=True)
+            pc = self._point_chunk(Nn, 8, P)
+            for a in range(0, P, pc):
+                b = min(P, a + pc)
+                xc = x[a:b]
+                HT = torch.empty((Nn, b - a), dtype=torch.float64, device=self._device)
+                N.check(N.fn["sbi_newton_basis_rows"](self._nplan, xc.data_ptr(), b - a,
+                                                      xc.stride(0), 0, Nn, HT.data_ptr(), b - a, 0,
+                                                      self._stream()), "basis")
+                self._contract(HT, S, None, y[a:b])
+            return y
+        # FP64 head rows, FP32 / BF16 remainder in the k-interleaved layout (16-byte operand loads)
+        nh = self._nlayout.n_head
+        S64 = self._surpluses("f64", gemm=True)[:nh]
+        if kind == "f32":
+            SQ, q, dt, code, fn = self._surpluses_k4(nh), 4, torch.float32, 3, "sbi_contract_split_k4"
+        else:
+            SQ, q, dt, code, fn = self._surpluses_k8(nh), 8, torch.bfloat16, 4, "sbi_contract_bf16_k8"
+        KQ = SQ.shape[0] * q
+        pc = self._point_chunk(KQ + 2 * nh, torch.finfo(dt).bits // 8, P)
         for a in range(0, P, pc):
             b = min(P, a + pc)
             xc = x[a:b]
-            if k4:
-                HT = torch.empty((K32 // 4, b - a, 4), dtype=dt, device=self._device)
-            else:
-                HT = torch.empty((Nn - nh, b - a), dtype=dt, device=self._device)
+            HT = torch.empty((KQ // q, b - a, q), dtype=dt, device=self._device)
             if Nn > nh:
                 N.check(N.fn["sbi_newton_basis_rows"](self._nplan, xc.data_ptr(), b - a,
                                                       xc.stride(0), nh, Nn, HT.data_ptr(), b - a,
-                                                      3 if k4 else code, self._stream()), "basis")
-            if k4:  # FP64 head rows, FP32 remainder
-                H64 = torch.empty((max(nh, 1), b - a), dtype=torch.float64, device=self._device)
-                if nh:
-                    N.check(N.fn["sbi_newton_rows_f64"](self._nplan, xc.data_ptr(), b - a,
-                                                        xc.stride(0), self._head_fac.data_ptr(),
-                                                        nh, H64.data_ptr(), b - a,
-                                                        self._stream()), "rows")
-                yc = y[a:b]
-                N.check(N.fn["sbi_contract_split_k4"](
-                    H64.data_ptr(), H64.stride(0), S64.data_ptr(), max(S64.stride(0), self._d_out),
-                    nh, HT.data_ptr() if K32 else None, b - a, S32.data_ptr() if K32 else None,
-                    self._d_out, K32, None, yc.data_ptr(), yc.stride(0), b - a, self._d_out,
-                    self._stream()), "contract")
-            elif kind == "bf16":
-                y[a:b] = torch.mm(HT.t(), S, out_dtype=torch.float32)
-            else:
-                self._contract(HT, S, None, y[a:b])
+                                                      code, self._stream()), "basis")
+            H64 = torch.empty((max(nh, 1), b - a), dtype=torch.float64, device=self._device)
+            if nh:
+                N.check(N.fn["sbi_newton_rows_f64"](self._nplan, xc.data_ptr(), b - a,
+                                                    xc.stride(0), self._head_fac.data_ptr(),
+                                                    nh, H64.data_ptr(), b - a,
+                                                    self._stream()), "rows")
+            yc = y[a:b]
+            N.check(N.fn[fn](
+                H64.data_ptr(), H64.stride(0), S64.data_ptr(), max(S64.stride(0), self._d_out),
+                nh, HT.data_ptr() if KQ else None, b - a, SQ.data_ptr() if KQ else None,
+                self._d_out, KQ, None, yc.data_ptr(), yc.stride(0), b - a, self._d_out,
+                self._stream()), "contract")
         return y
 
     def vjp(self, x, v, precision: str = None) -> torch.Tensor:
         """Vector-Jacobian product g[p, j] = sum_m v[p, m] dy_m/dx_j (FP64 result).
 
-        Nested nodes: hand-written adjoint of the Newton form after one GEMM G^T = S V^T
-        ("mixed" runs the GEMM and the adjoint tables in FP32).  Non-nested nodes: contraction of
-        the barycentric Jacobian (interpolation.py:306-345).
+        Nested nodes: hand-written adjoint of the Newton form after G^T = S V^T on the matrix
+        cores (sbi_contract; "mixed" and "bf16" take FP32 operands with FP64 accumulation).
+        Without a Newton form that meets FP64 accuracy: the barycentric Jacobian contracted with
+        v (interpolation.py:306-345).
         """
         precision = precision or self._precision
         x = self._validate_input(x)
@@ -509,23 +609,36 @@ class SmolyakBarycentricInterpolator:
         v = torch.as_tensor(v, device=self._device).reshape(P, self._d_out)
         if P == 0:
             return torch.empty((0, self._d_in), dtype=torch.float64, device=self._device)
-        if self._nplan is None:
-            J = self.gradient(x)
-            return torch.einsum("pm,pmj->pj", v.to(torch.float64), J)
         kind, code = ("f32", 1) if precision in ("mixed", "bf16") else ("f64", 0)
-        S = self._surpluses(kind)
+        if self._nplan is None or (kind == "f64" and not self._newton_fp64_ok):
+            # barycentric Jacobian contracted with v over the outputs, in point chunks of at
+            # most memory_limit GiB of Jacobian
+            gx = torch.empty((P, self._d_in), dtype=torch.float64, device=self._device)
+            chunk = self._point_chunk(self._d_out * self._d_in, 8, P)
+            for p0 in range(0, P, chunk):
+                p1 = min(P, p0 + chunk)
+                J = self.gradient(x[p0:p1])
+                gx[p0:p1] = (v[p0:p1].to(torch.float64).unsqueeze(2) * J).sum(dim=1)
+            return gx
+        ST = self._surpluses_t(kind)
+        dt = ST.dtype
+        n_nodes = ST.shape[1]
         gx = torch.empty((P, self._d_in), dtype=torch.float64, device=self._device)
-        # point chunks: G^T (nodes x points) and the workspace within memory_limit
+        # point chunks: G^T (nodes x points, FP64) and the workspace within memory_limit
         ws1 = int(N.fn["sbi_newton_vjp_workspace"](self._nplan, 64))
-        chunk = self._point_chunk(S.shape[0] * S.element_size() + max(0, ws1) // 64 + 1, 1, P)
+        chunk = self._point_chunk(n_nodes * 8 + max(0, ws1) // 64 + 1, 1, P)
         for p0 in range(0, P, chunk):
             p1 = min(P, p0 + chunk)
             xc = x[p0:p1]
-            GT = torch.mm(S, v[p0:p1].to(S.dtype).t()).contiguous()
+            # G^T[nu, p] = sum_m S[nu, m] v[p, m] on the matrix cores (sbi_contract with
+            # A = S^T and B = v^T; FP64, or FP32 operands with FP64 accumulation)
+            VT = v[p0:p1].t().to(dt).contiguous()
+            GT = torch.empty((n_nodes, p1 - p0), dtype=torch.float64, device=self._device)
+            self._contract(ST, VT, None, GT)
             nb = int(N.fn["sbi_newton_vjp_workspace"](self._nplan, p1 - p0))
             ws = torch.empty(max(1, (nb + 7) // 8), dtype=torch.float64, device=self._device)
             N.check(N.fn["sbi_newton_vjp"](self._nplan, xc.data_ptr(), p1 - p0, xc.stride(0),
-                                           GT.data_ptr(), GT.stride(0), code,
+                                           GT.data_ptr(), GT.stride(0), 0,
                                            gx[p0:p1].data_ptr(), ws.data_ptr(), nb,
                                            self._stream()), "vjp")
         return gx
@@ -543,7 +656,7 @@ class SmolyakBarycentricInterpolator:
         J = torch.empty((P, self._d_out, self._d_in), dtype=torch.float64, device=self._device)
         if P == 0:
             return J
-        if self._nplan is not None:
+        if self._nplan is not None and self._newton_fp64_ok:
             self._jacobian_newton(x, J)
         else:
             N.check(N.fn["sbi_bary_jacobian_f64"](self._plan, x.data_ptr(), P, x.stride(0),
@@ -583,3 +696,15 @@ class SmolyakBarycentricInterpolator:
     @property
     def layout(self):
         return self._layout
+
+    @property
+    def mixed_path(self) -> str:
+        """The path that serves precision 'mixed': "newton" or "bary-fp32" (Newton form
+        unavailable, see the constructor)."""
+        return "newton" if self._nplan is not None else "bary-fp32"
+
+    @property
+    def proxy_error(self):
+        """Non-nested interpolants: largest relative distance of the proxy Newton form from the
+        FP64 barycentric path on sample points (None for nested nodes or without a proxy)."""
+        return self._proxy_err

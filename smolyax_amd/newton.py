@@ -262,9 +262,11 @@ def build_newton_layout(nf: NewtonForm, n_waves: int = N_WAVES) -> NewtonLayout:
     L.hs_dim = np.array(hs_dims or [0], dtype=np.int32)
     L.hs_kmax = np.array([hs_k[j] for j in hs_dims] or [0], dtype=np.int32)
     L.hs_row = np.array([rows[j] for j in hs_dims] or [0], dtype=np.int32)
-    irec = [int(L.int_parent[q]) | ((rows[int(L.int_dim[q])] + int(L.int_deg[q]) - 1) << 16)
-            for q in range(L.n_internal)]
-    assert L.n_internal + 1 < 65536 and r < 65536
+    if L.n_internal + 1 < 65536 and r < 65536:
+        irec = [int(L.int_parent[q]) | ((rows[int(L.int_dim[q])] + int(L.int_deg[q]) - 1) << 16)
+                for q in range(L.n_internal)]
+    else:  # 16-bit record fields overflow: no staged kernel (its tables are left out)
+        irec, L.node_sq = [], None
     L.int_rec = np.array(irec or [0], dtype=np.int32)
     build_basis_tables(nf, L)
     # basis rows for the GEMM paths: the heaviest rows first (FP64 head of the mixed contraction)

@@ -8,10 +8,9 @@ pending request for the same interpolant joins its batch up to `max_batch_points
 free HBM budget (the batch's basis matrix, output and inputs are counted before launch).  The
 basis kernels and the matrix-core contraction (sbi_contract) compute every point independently of
 the batch it is in, so a request's result is the same bit for bit alone and in the mix, and their
-workgroups never wait on each other, so batches on different streams may overlap freely.  The BF16
-precision runs on the library GEMM instead, whose kernels may rely on all workgroups of a launch
-being resident (stream-K): two such launches overlapping on different streams can deadlock, so
-that precision is served on one stream.
+workgroups never wait on each other, so batches on different streams may overlap freely.  Every
+precision runs on the library's own kernels (the BF16 one on sbi_contract_bf16_k8), none on a
+library GEMM whose workgroups may wait on each other.
 
 With `graphs="auto"` the server replays graphs only while requests are queued (at least
 `n_streams` pending): a captured launch costs the host less, so it raises throughput under a
@@ -24,6 +23,13 @@ static input of the smallest power-of-two bucket that holds it, the graph is rep
 stream, and the batch's rows of the static output are copied out.  The captured work is the basis
 kernel and the contraction (no memset nodes); rows past the batch hold earlier inputs and are
 ignored, and every point is computed independently, so results equal eager evaluation bit for bit.
+
+Stream ordering: submit() records an event on the caller's current stream, and the batch that
+takes the request waits for it on its own stream before reading the points, so points produced by
+kernels still pending on the caller's stream are read finished.  The points are marked as used on
+the server's stream (record_stream), so the caller may drop them right after submit().  A result
+`r.y` is written on the server's stream: it is final once `r.done` is set; a caller that hands it
+to work on another stream without waiting for that, or frees it there, records that stream on it.
 
 Latency = completion (event observed) - arrival; throughput = point evaluations per second.
 """
@@ -46,6 +52,7 @@ class Request:
     done: float = None
     batch_points: int = 0
     taken: bool = False
+    ready: torch.cuda.Event = None  # recorded on the submitting stream after x was produced
 
 
 @dataclass
@@ -65,8 +72,6 @@ MIN_BUCKET = 64
 class Server:
     def __init__(self, interpolants, precision="mixed", n_streams=4, hbm_budget_bytes=8 << 30,
                  max_batch_points=8192, graphs=False):
-        if precision == "bf16" and n_streams > 1:
-            raise ValueError("precision 'bf16' uses the library GEMM: serve it on one stream")
         if graphs not in (False, True, "auto"):
             raise ValueError("graphs must be False, True or 'auto'")
         self.graph_mode = graphs
@@ -138,7 +143,7 @@ class Server:
         ip = self.ips[name]
         rows = ip.layout.k_raw if ip._nlayout is None or self.precision == "bary-fp32" else \
             ip._nlayout.basis_n
-        item = 8 if self.precision in ("fp64", "newton-fp64") else 4
+        item = {"fp64": 8, "newton-fp64": 8, "bf16": 2}.get(self.precision, 4)
         return n_points * (rows * item + ip.d_out * 8 + ip.d_in * 8)
 
     def submit(self, name, x, arrival=None):
@@ -147,6 +152,8 @@ class Server:
         r = Request(self._next, name, x, time.perf_counter() if arrival is None else arrival)
         if self.footprint(name, x.shape[0]) > self.budget:
             raise ValueError("request larger than the HBM budget of the server")
+        r.ready = torch.cuda.Event()
+        r.ready.record(torch.cuda.current_stream(x.device))
         self._next += 1
         self.pending.append(r)
         self.by_name[name].append(r)
@@ -199,6 +206,9 @@ class Server:
             ip = self.ips[head.name]
             use_graph = self.graphs and (self.graph_mode is True or
                                          self.n_pending >= len(self.slots))
+            for r in batch:  # the points are read on this stream once their producer is done
+                s.stream.wait_event(r.ready)
+                r.x.record_stream(s.stream)
             if use_graph:
                 g, x_st, y_st = self._graph(s, head.name, self.bucket(pts))
                 with torch.cuda.stream(s.stream):

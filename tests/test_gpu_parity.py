@@ -269,3 +269,52 @@ def test_input_shapes_and_errors(gpu):
         fresh(x1)
     with pytest.raises(AssertionError):
         fresh.integral()
+
+
+@pytest.mark.parametrize("d,m", [(1, 22), (1, 31), (2, 90)])
+def test_gauss_hermite_high_degree(gpu, d, m):
+    """Gauss-Hermite interpolants of degree 20 to 30: eval, gradient and adjoint match the oracle
+    whichever form serves them (the proxy Newton form where it meets FP64 accuracy on sample
+    points, the barycentric kernels otherwise); the proxy's measured error is reported."""
+    A = np.random.default_rng(d + m).standard_normal((2, d)) * 0.3
+    f = lambda x: np.sin(A @ np.asarray(x) + 0.4)
+    ref, ip, ong = build_pair(d, 2, m, f, kind="gh")
+    x = sample(ong, 150, seed=m)
+    print(f"GH d={d} m={m}: proxy error {ip.proxy_error}, mixed path {ip.mixed_path}", flush=True)
+    assert rel(ip.evaluate(x, "fp64").cpu().numpy(), ref(x)) < TOL
+    assert rel(ip.gradient(x).cpu().numpy(), ref.gradient(x)) < 1e-11
+    v = np.random.default_rng(1).standard_normal((len(x), 2))
+    g = ip.vjp(x, v, precision="fp64").cpu().numpy()
+    assert rel(g, np.einsum("pm,pmj->pj", v, ref.gradient(x))) < 1e-11
+    assert rel(ip.evaluate(x, "mixed").cpu().numpy(), ref(x)) < 1e-6 or ip.mixed_path != "newton"
+    if ip.proxy_error is not None and ip.proxy_error > 1e-12:
+        with pytest.raises(ValueError):
+            ip.evaluate(x, "newton-fp64")
+
+
+def test_mixed_without_newton_form_warns(gpu):
+    """An index set past the Newton layout's limits (nodes with 8 factors: the adjoint records
+    hold 6 other factors) has no Newton form: 'mixed' is served by 'bary-fp32' with a
+    SmolyaxPrecisionWarning, and the Newton-only precisions raise."""
+    from smolyax_amd import nodes as SN
+    from smolyax_amd.interpolation import SmolyakBarycentricInterpolator, SmolyaxPrecisionWarning
+
+    d = 8
+    k = np.ones(d)
+    t = 8.5
+    f = lambda x: np.exp(-np.sum(np.asarray(x) / (1.0 + np.arange(d))))
+    ip = SmolyakBarycentricInterpolator(d_out=1, node_gen=SN.Leja(dim=d), k=k, t=t, f=f,
+                                        precision="mixed")
+    assert ip.mixed_path == "bary-fp32"
+    x = np.random.default_rng(0).uniform(-1, 1, (64, d))
+    with pytest.warns(SmolyaxPrecisionWarning):
+        y = ip(x)
+    y32 = ip.evaluate(x, "bary-fp32")
+    assert np.array_equal(y.cpu().numpy(), y32.cpu().numpy())
+    y64 = ip.evaluate(x, "fp64").cpu().numpy()
+    e = rel(y.cpu().numpy(), y64)
+    print(f"bary-fp32 in place of mixed: {e:.2e} from fp64", flush=True)
+    assert e < 1e-3
+    for prec in ("newton-fp64", "bf16"):
+        with pytest.raises(ValueError):
+            ip.evaluate(x, prec)

@@ -64,13 +64,26 @@ def test_admission_by_hbm_budget(gpu):
         assert torch.equal(r.y, alone)
 
 
-def test_library_gemm_precision_needs_one_stream(gpu):
-    """BF16 runs on the library GEMM, whose kernels may wait on other workgroups of the same
-    launch: the server refuses to overlap such launches."""
+def test_bf16_served_on_several_streams(gpu):
+    """BF16 runs on the library's own MFMA kernel (no workgroup waits on another), so the server
+    overlaps BF16 batches on 4 streams; results are the same bits alone and in the mix."""
     from smolyax_amd import serving
+    from smolyax_amd import workloads as W
 
-    with pytest.raises(ValueError):
-        serving.Server({}, precision="bf16", n_streams=4)
+    ips = W.build_serving_interpolants(small_specs()[:4])
+    names = sorted(ips)
+    srv = serving.Server(ips, precision="bf16", n_streams=4, max_batch_points=2048)
+    reqs, _ = serving.run_poisson(srv, names, qps=30000.0, n_requests=80, batch_hi=512, seed=4,
+                                  time_limit_s=60.0)
+    torch.cuda.synchronize()
+    assert len(reqs) == 80 and all(r.done is not None for r in reqs)
+    assert any(r.batch_points > r.x.shape[0] for r in reqs)
+    for r in reqs:
+        alone = srv.ips[r.name].evaluate(r.x, precision="bf16")
+        ref64 = srv.ips[r.name].evaluate(r.x, precision="fp64")
+        assert torch.equal(r.y, alone), (r.rid, r.name, r.batch_points)
+        scale = float(ref64.abs().max()) + 1e-30
+        assert float((r.y - ref64).abs().max()) / scale < 5e-2
 
 
 def test_graph_replays_equal_eager(gpu):
@@ -94,3 +107,31 @@ def test_graph_replays_equal_eager(gpu):
             alone = srv.ips[r.name].evaluate(r.x, precision="mixed")
             assert torch.equal(r.y, alone), (rnd, r.rid, r.name, r.batch_points)
     assert any(len(sl.graphs) > 0 for sl in srv.slots)
+
+
+def test_points_from_pending_producer(gpu):
+    """Points written by kernels still pending on the submitting stream are read only after that
+    stream reaches them: a long spin kernel is queued before the copy that produces x, and x is
+    submitted before the copy has run."""
+    from smolyax_amd import serving
+    from smolyax_amd import workloads as W
+
+    ips = W.build_serving_interpolants(small_specs()[:2])
+    names = sorted(ips)
+    srv = serving.Server(ips, precision="mixed", n_streams=4)
+    torch.manual_seed(0)
+    bases = [torch.randn((300, ips[names[i % 2]].d_in), dtype=torch.float64, device="cuda")
+             for i in range(6)]
+    torch.cuda.synchronize()
+    reqs = []
+    for i, b in enumerate(bases):
+        x = torch.full_like(b, float("nan"))
+        torch.cuda._sleep(20_000_000)  # bounded spin (~10 ms) on the current stream
+        x.copy_(b)
+        reqs.append(srv.submit(names[i % 2], x))
+        del x
+    srv.drain()
+    torch.cuda.synchronize()
+    for i, r in enumerate(reqs):
+        alone = srv.ips[r.name].evaluate(bases[i], precision="mixed")
+        assert torch.equal(r.y, alone), i

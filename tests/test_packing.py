@@ -70,3 +70,41 @@ def test_vectorized_f_matches_pointwise():
     F2, o2, n2 = packing.assemble_values(L, blocks, SN.Leja(dim=d), fv, {}, vectorized=True)
     assert n1 == n2
     assert np.allclose(F1, F2, rtol=0, atol=1e-15) and np.allclose(o1, o2, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("kind", ["leja", "gh"])
+def test_f_evals_cross_compatible_with_oracle(kind):
+    """The f_evals dict of the reference's format (interpolation.py:115-163, :208-228; written here
+    by the oracle's set_f) is accepted as it is: every value is reused (no call of f), F and the
+    offset are the ones built from f, and the dict the product writes has the oracle's keys
+    (flat node keys for nested nodes, per-block node dicts for non-nested ones) and values."""
+    from smolyax_amd import nodes as SN
+    from smolyax_amd import packing
+
+    d, d_out = 5, 2
+    k = np.log((2 + np.arange(d)) / np.log(2))
+    ong, sng = (ON.leja(dim=d), SN.Leja(dim=d)) if kind == "leja" else \
+        (ON.gauss_hermite(dim=d), SN.GaussHermite(dim=d))
+    t = OI.find_approximate_threshold(k, 150, ong.is_nested)
+    A = np.random.default_rng(4).standard_normal((d_out, d))
+    f = lambda x: np.sin(A @ x + 0.3)
+    ref = OS.Interpolant(d_out=d_out, node_gen=ong, k=k, t=t)
+    fe_oracle = ref.set_f(f=f)
+
+    def no_calls(x):
+        raise AssertionError("f called although f_evals holds every node")
+
+    blocks = packing.nonzero_blocks(k, t)
+    L = packing.build_layout(blocks, sng, d, d_out)
+    F0, off0, _ = packing.assemble_values(L, blocks, sng, f, fe_own := {})
+    F1, off1, n1 = packing.assemble_values(L, blocks, sng, no_calls, fe_oracle)
+    assert n1 == 0
+    assert np.array_equal(F0, F1) and np.array_equal(off0, off1)
+    assert set(fe_own) == set(fe_oracle)
+    for key in fe_own:
+        a, b = fe_own[key], fe_oracle[key]
+        if kind == "leja":
+            assert np.array_equal(np.asarray(a), np.asarray(b))
+        else:
+            assert set(a) == set(b)
+            assert all(np.array_equal(np.asarray(a[q]), np.asarray(b[q])) for q in a)
