@@ -341,7 +341,7 @@ __global__ __launch_bounds__(kTile* kWaves) void bary_eval_kernel(
 // that contain j; every (p, o, j) entry is written by exactly one lane.
 // ------------------------------------------------------------------------------------------
 constexpr int kJMaxN = 8;    // max active dims per block supported by the jacobian kernel
-constexpr int kJDm = 16;     // max (deg + 1) per factor in the jacobian kernel
+constexpr int kJDm = 32;     // max (deg + 1) per factor in the jacobian kernel
 
 // Basis (first form) L_mu = w_mu prod_{nu != mu} (x - xi_nu) and, for the differentiated factor,
 // its derivative by the product rule (running pair (p, p') over the factors, O(deg^2) per point)
@@ -644,11 +644,12 @@ extern "C" int sbi_bary_jacobian_f64(const sbi_plan* plan, const double* x, int6
     return SBI_ERR_INVALID;
   }
   dim3 grid((unsigned)tiles, (unsigned)d.d_in, (unsigned)gz);
-  if (d.max_n > kJMaxN || d.dmax1 > kJDm) {
-    sbi::set_error("sbi_bary_jacobian_f64: supports at most 8 active dims and degree 15");
+  const size_t lds = (size_t)(d.max_n + 1) * d.dmax1 * kTile * sizeof(double);
+  if (d.max_n > kJMaxN || d.dmax1 > kJDm || lds > (size_t)kMaxLds) {
+    sbi::set_error("sbi_bary_jacobian_f64: supports at most 8 active dims, degree 31, and "
+                   "(active dims + 1) * (max degree + 1) <= 320");
     return SBI_ERR_UNSUPPORTED;
   }
-  const size_t lds = (size_t)(d.max_n + 1) * d.dmax1 * kTile * sizeof(double);
   sbi::allow_lds((const void*)bary_jacobian_kernel);
   hipLaunchKernelGGL(bary_jacobian_kernel, grid, dim3(64), lds, (hipStream_t)stream, d, x, P, ldx,
                      J);

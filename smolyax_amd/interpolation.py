@@ -461,13 +461,19 @@ class SmolyakBarycentricInterpolator:
         chunk = int(self._memory_limit * 1024**3 // per_point) // 64 * 64 or 64
         return max(64, min(P, chunk, MAX_CALL_POINTS))
 
+    @staticmethod
+    def _ld(t):
+        """Row stride of a row-major matrix (a single row may carry any stride in torch)."""
+        return t.stride(0) if t.shape[0] > 1 else max(t.shape[1], 1)
+
     def _contract(self, A, B, offset, y):
         """y[P, n] = offset + A^T B on the matrix cores (sbi_contract: FP64 or FP32 MFMA with
         FP64 accumulation; rows independent of the batch)."""
         code = {torch.float64: 0, torch.float32: 1}[A.dtype]
-        N.check(N.fn["sbi_contract"](A.data_ptr(), A.stride(0), B.data_ptr(), B.stride(0),
+        ld = self._ld
+        N.check(N.fn["sbi_contract"](A.data_ptr(), ld(A), B.data_ptr(), ld(B),
                                      None if offset is None else offset.data_ptr(), y.data_ptr(),
-                                     y.stride(0), y.shape[0], y.shape[1], A.shape[0], code,
+                                     ld(y), y.shape[0], y.shape[1], A.shape[0], code,
                                      self._stream()), "contract")
 
     def _eval_bary_gemm(self, x, fp32=False, out=None):
