@@ -149,6 +149,17 @@ int sbi_bary_jacobian_f64(const sbi_plan* plan, const double* x, int64_t P, int6
 int64_t sbi_bary_integral_workspace(const sbi_plan* plan);
 int sbi_bary_integral_f64(const sbi_plan* plan, double* Q, double* partial, void* stream);
 
+/* sbi_bary_integral_f64 without a workspace argument and without any allocation: the partial sums
+ * live in a device array of the library (SBI_INTEGRAL_SCRATCH_DOUBLES doubles = 4.5 MiB, one copy
+ * per device, held until the process ends).  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  Same kernels and summation order as sbi_bary_integral_f64, so the same Q bit
+ * for bit.  SBI_ERR_CAPACITY when the plan's partial sums exceed the array (never for d_out up to
+ * 65536).  Calls that overlap on one device (several streams or threads) share the array and
+ * race: serialise them. */
+#define SBI_INTEGRAL_SCRATCH_DOUBLES 589824
+int sbi_bary_integral_f64_noalloc(const sbi_plan* plan, double* Q, void* stream);
+
 
 /* ----------------------------------------------------------------------------------------------
  * Hierarchical (Newton-surplus) form, nested nodes only (Leja).  Same interpolant as the

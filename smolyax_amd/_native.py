@@ -87,6 +87,7 @@ SIGNATURES = {
     "sbi_bary_jacobian_f64": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp]),
     "sbi_bary_integral_workspace": (c_i64, [c_vp]),
     "sbi_bary_integral_f64": (c_i32, [c_vp, c_vp, c_vp, c_vp]),
+    "sbi_bary_integral_f64_noalloc": (c_i32, [c_vp, c_vp, c_vp]),
     "sbi_newton_plan_create": (c_i32, [ctypes.POINTER(NewtonDesc), ctypes.POINTER(c_vp)]),
     "sbi_newton_plan_destroy": (c_i32, [c_vp]),
     "sbi_newton_eval_mixed": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp]),

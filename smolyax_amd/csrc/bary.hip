@@ -531,6 +531,13 @@ __global__ __launch_bounds__(kIntThreads) void bary_integral_final_kernel(
 
 }  // namespace
 
+// Device scratch of sbi_bary_integral_f64_noalloc: the chunks' partial sums (at most
+// kIntScratchDoubles doubles = 4.5 MiB on every device the library runs on, held until the process
+// ends).  Host code never passes `scratch` itself to a kernel: in host code the name is a host-side
+// stand-in; the device address of each device's copy comes from hipGetSymbolAddress.
+constexpr int64_t kIntScratchDoubles = SBI_INTEGRAL_SCRATCH_DOUBLES;
+static __device__ __align__(256) double scratch[kIntScratchDoubles];
+
 // ------------------------------------------------------------------------------------------
 // C ABI
 // ------------------------------------------------------------------------------------------
@@ -673,6 +680,59 @@ static void integral_launch(const sbi_bary_desc& d, int nc, int64_t chunk, doubl
                      d.k_raw, d.d_out, chunk, partial);
 }
 
+// Chunks of the integral's partial-sum pass for a plan: nc chunks of `chunk` entries, output tiles
+// of w (the partial sums take nc * d_out doubles).
+struct IntSplit {
+  int w;
+  int64_t nc, chunk, otiles;
+};
+
+static IntSplit integral_split(const sbi_bary_desc& d) {
+  // output tile width: the smallest power of two >= d_out, at most 64 (4 entry lanes), or 256
+  // (one entry lane: a workgroup reads 2 KB runs of each F row) from d_out = 1024 on, where F is
+  // large enough to keep ~2048 such workgroups busy (config 3: 75 % against 65 % of HBM; config
+  // 4, d_out 256: 33 against 25 us)
+  IntSplit sp;
+  const int wmax = d.d_out >= 1024 ? 256 : 64;
+  int w = 1;
+  while (w < d.d_out && w < wmax) w <<= 1;
+  sp.w = w;
+  sp.otiles = (d.d_out + w - 1) / w;
+  // entry chunks: ~2048 workgroups in all, each chunk at least 4 entries per entry lane
+  const int64_t lanes = kIntThreads / w;
+  int64_t nc = std::max<int64_t>(1, std::min<int64_t>(kIntMaxChunks, (2048 + sp.otiles - 1) / sp.otiles));
+  nc = std::max<int64_t>(1, std::min<int64_t>(nc, (d.k_raw + 4 * lanes - 1) / (4 * lanes)));
+  sp.chunk = (d.k_raw + nc - 1) / nc;
+  sp.nc = std::max<int64_t>(1, (d.k_raw + sp.chunk - 1) / std::max<int64_t>(sp.chunk, 1));
+  return sp;
+}
+
+static int integral_run(const sbi_bary_desc& d, const IntSplit& sp, double* Q, double* partial,
+                        hipStream_t st) {
+  if (sp.otiles > 65535) {
+    sbi::set_error("sbi_bary_integral_f64: d_out too large for one launch");
+    return SBI_ERR_INVALID;
+  }
+  const int nc = (int)sp.nc;
+  switch (sp.w) {
+    case 1: integral_launch<1>(d, nc, sp.chunk, partial, st); break;
+    case 2: integral_launch<2>(d, nc, sp.chunk, partial, st); break;
+    case 4: integral_launch<4>(d, nc, sp.chunk, partial, st); break;
+    case 8: integral_launch<8>(d, nc, sp.chunk, partial, st); break;
+    case 16: integral_launch<16>(d, nc, sp.chunk, partial, st); break;
+    case 32: integral_launch<32>(d, nc, sp.chunk, partial, st); break;
+    case 64: integral_launch<64>(d, nc, sp.chunk, partial, st); break;
+    case 128: integral_launch<128>(d, nc, sp.chunk, partial, st); break;
+    default: integral_launch<256>(d, nc, sp.chunk, partial, st); break;
+  }
+  SBI_HIP_CHECK(hipGetLastError());
+  dim3 g2((unsigned)((d.d_out + kFinO - 1) / kFinO));
+  hipLaunchKernelGGL(bary_integral_final_kernel, g2, dim3(kIntThreads), 0, st, d.d_out, nc,
+                     d.offset, partial, Q);
+  SBI_HIP_CHECK(hipGetLastError());
+  return SBI_OK;
+}
+
 extern "C" int sbi_bary_integral_f64(const sbi_plan* plan, double* Q, double* partial,
                                      void* stream) {
   if (!plan || plan->kind != 1 || !Q || !partial || !plan->bary.ent_q) {
@@ -680,40 +740,51 @@ extern "C" int sbi_bary_integral_f64(const sbi_plan* plan, double* Q, double* pa
     return SBI_ERR_INVALID;
   }
   const sbi_bary_desc& d = plan->bary;
-  hipStream_t st = (hipStream_t)stream;
-  // output tile width: the smallest power of two >= d_out, at most 64 (4 entry lanes), or 256
-  // (one entry lane: a workgroup reads 2 KB runs of each F row) from d_out = 1024 on, where F is
-  // large enough to keep ~2048 such workgroups busy (config 3: 75 % against 65 % of HBM; config
-  // 4, d_out 256: 33 against 25 us)
-  const int wmax = d.d_out >= 1024 ? 256 : 64;
-  int w = 1;
-  while (w < d.d_out && w < wmax) w <<= 1;
-  const int64_t otiles = (d.d_out + w - 1) / w;
-  // entry chunks: ~2048 workgroups in all, each chunk at least 4 entries per entry lane
-  const int64_t lanes = kIntThreads / w;
-  int64_t nc = std::max<int64_t>(1, std::min<int64_t>(kIntMaxChunks, (2048 + otiles - 1) / otiles));
-  nc = std::max<int64_t>(1, std::min<int64_t>(nc, (d.k_raw + 4 * lanes - 1) / (4 * lanes)));
-  const int64_t chunk = (d.k_raw + nc - 1) / nc;
-  nc = std::max<int64_t>(1, (d.k_raw + chunk - 1) / std::max<int64_t>(chunk, 1));
-  if (otiles > 65535) {
-    sbi::set_error("sbi_bary_integral_f64: d_out too large for one launch");
+  return integral_run(d, integral_split(d), Q, partial, (hipStream_t)stream);
+}
+
+// Device address of this device's copy of `scratch`, looked up once per device (with that device
+// current) and kept for the process.
+static int scratch_address(double** out) {
+  static std::mutex mu;
+  static double* addr[64] = {};
+  int dev = 0;
+  SBI_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) {
+    sbi::set_error("sbi_bary_integral_f64_noalloc: device index beyond 63");
+    return SBI_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* ptr = nullptr;
+    SBI_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) {
+      sbi::set_error("sbi_bary_integral_f64_noalloc: no device address for the scratch array");
+      return SBI_ERR_HIP;
+    }
+    addr[dev] = (double*)ptr;
+  }
+  *out = addr[dev];
+  return SBI_OK;
+}
+
+extern "C" int sbi_bary_integral_f64_noalloc(const sbi_plan* plan, double* Q, void* stream) {
+  if (!plan || plan->kind != 1 || !Q || !plan->bary.ent_q) {
+    sbi::set_error("sbi_bary_integral_f64_noalloc: invalid argument (or the plan has no ent_q)");
     return SBI_ERR_INVALID;
   }
-  switch (w) {
-    case 1: integral_launch<1>(d, (int)nc, chunk, partial, st); break;
-    case 2: integral_launch<2>(d, (int)nc, chunk, partial, st); break;
-    case 4: integral_launch<4>(d, (int)nc, chunk, partial, st); break;
-    case 8: integral_launch<8>(d, (int)nc, chunk, partial, st); break;
-    case 16: integral_launch<16>(d, (int)nc, chunk, partial, st); break;
-    case 32: integral_launch<32>(d, (int)nc, chunk, partial, st); break;
-    case 64: integral_launch<64>(d, (int)nc, chunk, partial, st); break;
-    case 128: integral_launch<128>(d, (int)nc, chunk, partial, st); break;
-    default: integral_launch<256>(d, (int)nc, chunk, partial, st); break;
+  const sbi_bary_desc& d = plan->bary;
+  const IntSplit sp = integral_split(d);
+  if (sp.nc * d.d_out > kIntScratchDoubles) {  // checked before any HIP call
+    sbi::set_error("sbi_bary_integral_f64_noalloc: the partial sums (" +
+                   std::to_string(sp.nc * d.d_out) + " doubles) exceed the " +
+                   std::to_string(kIntScratchDoubles) +
+                   "-double device scratch; use sbi_bary_integral_f64 with a workspace");
+    return SBI_ERR_CAPACITY;
   }
-  SBI_HIP_CHECK(hipGetLastError());
-  dim3 g2((unsigned)((d.d_out + kFinO - 1) / kFinO));
-  hipLaunchKernelGGL(bary_integral_final_kernel, g2, dim3(kIntThreads), 0, st, d.d_out, (int)nc,
-                     d.offset, partial, Q);
-  SBI_HIP_CHECK(hipGetLastError());
-  return SBI_OK;
+  double* partial = nullptr;
+  const int rc = scratch_address(&partial);
+  if (rc != SBI_OK) return rc;
+  // every partial sum the final pass reads is written by this call's first pass: nothing to clear
+  return integral_run(d, sp, Q, partial, (hipStream_t)stream);
 }
