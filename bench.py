@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--cpu-points", type=int, default=8192)
     ap.add_argument("--oracle-points", type=int, default=256)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="skip the config 3 and config 4 legs of the default run")
     ap.add_argument("--serve-requests", type=int, default=600)
     return ap.parse_args()
 
@@ -79,14 +81,68 @@ def timed_loop(fn, steps, warmup, torch):
     return (t1 - t0), e0.elapsed_time(e1) / steps
 
 
-def bench_gemm_config(args, torch, W, cfg):
-    """Large-d_out workloads: every path is a basis kernel + one dense GEMM (MFMA)."""
+PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0
+
+
+def contraction_times(ip, x, torch, steps):
+    """Average device time of the contraction kernel alone (HIP events on the launch stream over
+    `steps` back-to-back launches) for each GEMM path, on basis matrices computed once:
+    mixed = sbi_contract_split_k4 (FP64 head + FP32 rows), bf16 = sbi_contract_bf16_k8 (FP64 head
+    + BF16 rows), newton-fp64 = sbi_contract (FP64)."""
+    import ctypes
+
+    from smolyax_amd import _native as N
+
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    NL = ip._nlayout
+    P, n, Nn, nh = x.shape[0], ip.d_out, NL.basis_n, NL.n_head
+    y = torch.empty((P, n), dtype=torch.float64, device="cuda")
+    H64 = torch.empty((max(nh, 1), P), dtype=torch.float64, device="cuda")
+    N.check(N.fn["sbi_newton_rows_f64"](ip._nplan, x.data_ptr(), P, x.stride(0),
+                                        ip._head_fac.data_ptr(), nh, H64.data_ptr(), P, st), "rows")
+    S64 = ip._surpluses("f64", gemm=True)
+    out = {}
+    for name, SQ, q, dt, code, fn in (
+            ("mixed", ip._surpluses_k4(nh), 4, torch.float32, 3, "sbi_contract_split_k4"),
+            ("bf16", ip._surpluses_k8(nh), 8, torch.bfloat16, 4, "sbi_contract_bf16_k8")):
+        KQ = SQ.shape[0] * q
+        HT = torch.empty((KQ // q, P, q), dtype=dt, device="cuda")
+        N.check(N.fn["sbi_newton_basis_rows"](ip._nplan, x.data_ptr(), P, x.stride(0), nh, Nn,
+                                              HT.data_ptr(), P, code, st), "basis")
+
+        def run(fn=fn, HT=HT, SQ=SQ, KQ=KQ):
+            N.check(N.fn[fn](H64.data_ptr(), P, S64.data_ptr(), n, nh, HT.data_ptr(), P,
+                             SQ.data_ptr(), n, KQ, None, y.data_ptr(), n, P, n, st), fn)
+        _, ms = timed_loop(run, steps, 2, torch)
+        out[name] = {"kernel": fn, "ms": ms, "k_rows": KQ + nh}
+        del HT
+    HT = torch.empty((Nn, P), dtype=torch.float64, device="cuda")
+    N.check(N.fn["sbi_newton_basis_rows"](ip._nplan, x.data_ptr(), P, x.stride(0), 0, Nn,
+                                          HT.data_ptr(), P, 0, st), "basis")
+
+    def run64():
+        ip._contract(HT, S64, None, y)
+    _, ms = timed_loop(run64, max(2, steps // 2), 1, torch)
+    out["newton-fp64"] = {"kernel": "sbi_contract (FP64)", "ms": ms, "k_rows": Nn}
+    return out
+
+
+def bench_gemm_leg(args, torch, W, name, cpu=True, log=None):
+    """A large-d_out workload (config 3 or 4): every path is a basis kernel + one contraction on
+    the matrix cores.  Reports each path's throughput (inputs resident, whole step), the
+    contraction kernel's roofline on the matrix cores, accuracy against the FP64 oracle and the
+    exact interpolant (the oracle's extended precision) on a sample of points, the CPU baseline
+    on a sample, and for config 4 the adjoint (vjp) in mixed and FP64 with its own accuracy."""
+    from oracle import cpu_baseline as CB
     from oracle import nodes as ON
     from oracle import smolyak as OS
     from smolyax_amd import indices as SI
     from smolyax_amd import nodes as SN
     from smolyax_amd.interpolation import SmolyakBarycentricInterpolator
 
+    log = log or _log
+    cfg = W.CONFIGS[name]
     d, d_out, m, P = cfg["d_in"], cfg["d_out"], cfg["m"], cfg["P"]
     f = cfg["f"]()
     k = W.k_rule(d)
@@ -97,79 +153,113 @@ def bench_gemm_config(args, torch, W, cfg):
     ip.set_f(f=lambda X: f(X), vectorized=True)
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - ts
+    log(f"[bench] {name}: set up in {setup_s:.1f} s")
     x_np = W.points_uniform(P, d, seed=0)
     x = torch.as_tensor(x_np, device="cuda")
     results, outputs = {}, {}
-    for name in ("bf16", "mixed", "newton-fp64", "fp64"):
-        steps = args.steps if name != "fp64" else max(2, args.steps // 10)
+    for pname in ("bf16", "mixed", "newton-fp64", "fp64"):
+        steps = args.steps if pname != "fp64" else max(2, args.steps // 4)
         box = {}
 
         def step():
-            box["y"] = ip.evaluate(x, precision=name)
+            box["y"] = ip.evaluate(x, precision=pname)
 
-        wall, ev_mean_ms = timed_loop(step, steps, min(args.warmup, 2), torch)
-        outputs[name] = box["y"].cpu().numpy()
-        results[name] = {"value": P * steps / wall, "ms_per_step": wall / steps * 1e3,
-                         "steps": steps}
-    accuracy = {}
+        wall, ev_ms = timed_loop(step, steps, min(args.warmup, 2), torch)
+        outputs[pname] = box["y"]
+        results[pname] = {"value": P * steps / wall, "ms_per_step": wall / steps * 1e3,
+                          "device_ms_per_step": ev_ms}
+        log(f"[bench] {name} {pname}: {results[pname]['ms_per_step']:.3f} ms per step")
+    L, NL = ip.layout, ip._nlayout
+    # contraction kernels alone, on the matrix cores: FLOP on the Newton terms (2 d_out |Lambda|
+    # per point, the product's own work) and on the reference's terms (2 d_out K_raw per point)
+    ct = contraction_times(ip, x, torch, args.steps)
+    flop_newton = 2.0 * d_out * NL.basis_n * P
+    flop_kraw = 2.0 * d_out * L.k_raw * P
+    peaks = {"mixed": PEAK_FP32_TFLOPS, "bf16": PEAK_BF16_TFLOPS, "newton-fp64": PEAK_FP64_TFLOPS}
+    roof = {}
+    for pname, c in ct.items():
+        ach = flop_newton / (c["ms"] * 1e-3) / 1e12
+        roof[pname] = {"bound": "mfma", "kernel": c["kernel"], "avg_launch_ms": c["ms"],
+                       "achieved": ach, "peak": peaks[pname], "unit": "TFLOP/s",
+                       "frac": ach / peaks[pname], "flop_per_launch": flop_newton,
+                       "flop_basis": "2 d_out |Lambda| per point (Newton terms)"}
+    # accuracy on a sample: FP64 oracle (the reference's algorithm) and the exact interpolant
+    no = 16
+    cols = np.arange(d_out) if d_out <= 256 else \
+        np.sort(np.random.default_rng(5).choice(d_out, 64, replace=False))
+    fc = f if len(cols) == d_out else (lambda X: np.asarray(f(X))[..., cols])
+    ref = OS.Interpolant(d_out=len(cols), node_gen=ON.leja(dim=d), k=k, t=t, f=fc)
+    y64 = ref(x_np[:no])
+    yex = np.asarray(ref(x_np[:no], dtype=np.longdouble), dtype=np.float64)
 
     def errs(a, b):
         diff = a - b
         return {"max_rel": float(np.max(np.abs(diff)) / np.max(np.abs(b))),
                 "rel_rmse": float(np.sqrt(np.mean(diff ** 2) / np.mean(b ** 2)))}
 
-    for name in results:
-        accuracy[name] = {"vs_gpu_fp64_all_points": errs(outputs[name], outputs["fp64"])}
-    out_extra = {}
+    accuracy = {"points": no, "outputs": int(len(cols)),
+                "oracle_fp64_vs_exact": errs(y64, yex)}
+    for pname, yv in outputs.items():
+        a = yv[:no].cpu().numpy()[:, cols]
+        accuracy[pname] = {"vs_oracle": errs(a, y64), "vs_exact": errs(a, yex),
+                           "vs_gpu_fp64_all_points": errs(yv.cpu().numpy(),
+                                                          outputs["fp64"].cpu().numpy())}
+    leg = {"config": {"workload": name, "d_in": d, "d_out": d_out, "points": P,
+                      "nodes": int(ip.n_f_evals), "k_raw": int(L.k_raw), "t": t},
+           "setup_s": setup_s, "paths": results,
+           "speedup_mixed_vs_fp64": results["mixed"]["value"] / results["fp64"]["value"],
+        "speedup_mixed_vs_newton_fp64": results["mixed"]["value"] / results["newton-fp64"]["value"],
+           "speedup_mixed_vs_newton_fp64": results["mixed"]["value"]
+           / results["newton-fp64"]["value"],
+           "contraction_roofline": roof, "accuracy": accuracy,
+           "path_tflops_kraw_basis": {pn: flop_kraw / (r["ms_per_step"] * 1e-3) / 1e12
+                                      for pn, r in results.items()}}
     if cfg.get("vjp"):
-        rng = np.random.default_rng(2)
-        v = torch.as_tensor(rng.standard_normal((P, d_out)), device="cuda")
-        for name in ("mixed", "fp64"):
+        w = np.random.default_rng(2).standard_normal(d_out)
+        v = torch.as_tensor(np.tile(w, (P, 1)), device="cuda")
+        nov = 8
+        Jex = np.asarray(ref.gradient(x_np[:nov], dtype=np.longdouble), dtype=np.float64)
+        gex = np.einsum("m,pmj->pj", w, Jex)
+        g64 = np.einsum("m,pmj->pj", w, ref.gradient(x_np[:nov]))
+        vj = {}
+        for pname in ("mixed", "newton-fp64"):
             box = {}
 
             def vstep():
-                box["g"] = ip.vjp(x, v, precision=name)
+                box["g"] = ip.vjp(x, v, precision=pname)
 
-            wall, _ = timed_loop(vstep, args.steps, min(args.warmup, 2), torch)
-            results["vjp-" + name] = {"value": P * args.steps / wall,
-                                      "ms_per_step": wall / args.steps * 1e3}
-            outputs["vjp-" + name] = box["g"].cpu().numpy()
-        accuracy["vjp-mixed"] = {"vs_gpu_fp64_all_points": errs(outputs["vjp-mixed"],
-                                                                outputs["vjp-fp64"])}
-        if cfg.get("oracle_points"):
-            no = cfg["oracle_points"]
-            ref = OS.Interpolant(d_out=d_out, node_gen=ON.leja(dim=d), k=k, t=t, f=f)
-            yr = ref(x_np[:no])
-            gr = np.einsum("pm,pmj->pj", v.cpu().numpy()[:no], ref.gradient(x_np[:no]))
-            for name in ("bf16", "mixed", "newton-fp64", "fp64"):
-                accuracy[name]["vs_oracle"] = errs(outputs[name][:no], yr)
-            accuracy["vjp-fp64"] = {"vs_oracle": errs(outputs["vjp-fp64"][:no], gr)}
-            accuracy["vjp-mixed"]["vs_oracle"] = errs(outputs["vjp-mixed"][:no], gr)
-            accuracy["oracle_points"] = no
-    rows = bench_gradient_integral(ip, x, torch)
-    L, NL = ip.layout, ip._nlayout
-    flops_newton = 2.0 * d_out * NL.basis_n * P
-    flops_bary = 2.0 * d_out * L.k_raw * P
-    t_mixed = results["mixed"]["ms_per_step"] * 1e-3
-    roofline = {"bound": "mfma", "achieved": flops_newton / t_mixed / 1e12, "peak": 157.3,
-                "unit": "TFLOP/s", "frac": flops_newton / t_mixed / 1e12 / 157.3, "traffic": None,
-                "kernel": "newton basis + FP32 GEMM (whole step)"}
-    extra = {name: {"tflops_alg": (flops_bary if name == "fp64" else flops_newton)
-                    / (r["ms_per_step"] * 1e-3) / 1e12}
-             for name, r in results.items() if not name.startswith("vjp")}
-    out = {
-        "metric": "point-evals/s, d_in=1e3 n=1e4 (mixed vs FP64, <=1e-6 rel); serve p50/p99 ms",
-        "value": results["mixed"]["value"], "unit": "point-evals/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": results["mixed"]["ms_per_step"],
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 (mixed path)", "data": "synthetic",
-        "config": {"workload": args.config, "d_in": d, "d_out": d_out, "points": P,
-                   "nodes": int(ip.n_f_evals), "k_raw": int(L.k_raw), "t": t},
-        "roofline": roofline, "paths": results, "path_tflops": extra, "accuracy": accuracy,
-        "gradient_integral": rows,
-        "cpu_baseline": None, "setup_s": setup_s, **out_extra,
-    }
-    print(json.dumps(out), flush=True)
+            wall, ev_ms = timed_loop(vstep, args.steps, min(args.warmup, 2), torch)
+            g = box["g"][:nov].cpu().numpy()
+            vj[pname] = {"value": P * args.steps / wall, "ms_per_step": wall / args.steps * 1e3,
+                         "vs_exact": errs(g, gex), "vs_oracle": errs(g, g64)}
+            log(f"[bench] {name} vjp {pname}: {vj[pname]['ms_per_step']:.3f} ms per step")
+        vj["oracle_fp64_vs_exact"] = errs(g64, gex)
+        vj["points_checked"] = nov
+        leg["vjp"] = vj
+    leg["gradient_integral"] = bench_gradient_integral(ip, x, torch)
+    if cpu:
+        import torch as _t
+
+        _t.set_num_threads(CB.threads())
+        full = ref if len(cols) == d_out else OS.Interpolant(
+            d_out=d_out, node_gen=ON.leja(dim=d), k=k, t=t, f=lambda X: f(X))
+        tc = CB.TorchCPUInterpolant(full)
+        nc = 16 if d_out > 1024 else 512
+        tc(x_np[:2])
+        c0 = time.perf_counter()
+        tc(x_np[:nc])
+        c1 = time.perf_counter()
+        leg["cpu_baseline"] = {
+            "value": nc / (c1 - c0), "unit": "point-evals/s", "cores": CB.threads(),
+            "kind": "port",
+            "sample": f"{nc} of the {P} points, one call ({c1 - c0:.1f} s), FP64 torch-CPU "
+                      f"restatement of the reference's padded-group einsum evaluation"}
+        log(f"[bench] {name} cpu baseline: {leg['cpu_baseline']['value']:.3g} point-evals/s")
+    else:
+        leg["cpu_baseline"] = None
+    del ip
+    torch.cuda.empty_cache()
+    return leg
 
 
 def bench_gradient_integral(ip, x, torch, steps=10):
@@ -365,8 +455,17 @@ def main():
         print(json.dumps(out), flush=True)
         return
     cfg = W.CONFIGS[args.config]
-    if cfg["d_out"] > 1:
-        return bench_gemm_config(args, torch, W, cfg)
+    if cfg["d_out"] > 1:  # one large-d_out workload on its own
+        leg = bench_gemm_leg(args, torch, W, args.config, cpu=not args.no_cpu)
+        out = {"metric": "point-evals/s (mixed path) of the named workload", "unit": "point-evals/s",
+               "value": leg["paths"]["mixed"]["value"], "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": leg["paths"]["mixed"]["ms_per_step"],
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "f32 products / f64 accumulation", "data": "synthetic",
+               "config": leg["config"], "roofline": leg["contraction_roofline"]["mixed"],
+               "cpu_baseline": leg["cpu_baseline"], "leg": leg}
+        print(json.dumps(out), flush=True)
+        return
     d, d_out, m, P = cfg["d_in"], cfg["d_out"], cfg["m"], cfg["P"]
     f = cfg["f"]()
     k = W.k_rule(d)
@@ -406,6 +505,16 @@ def main():
         outputs[name] = y.cpu().numpy().copy()
         results[name] = {"value": P * args.steps / wall, "ms_per_step": wall / args.steps * 1e3,
                          "kernel_ms": ev_mean_ms}
+    # the drop-in API (validation, output allocation, dispatch) on the same points
+    box = {}
+
+    def api_step():
+        box["y"] = ip.evaluate(x, precision="mixed")
+
+    wall, ev_mean_ms = timed_loop(api_step, args.steps, args.warmup, torch)
+    assert np.array_equal(box["y"].cpu().numpy(), outputs["mixed"])
+    results["mixed-api"] = {"value": P * args.steps / wall, "ms_per_step": wall / args.steps * 1e3,
+                            "device_ms_per_step": ev_mean_ms, "call": "ip.evaluate(x, 'mixed')"}
 
     # accuracy against the FP64 CPU oracle on a sample of the same points, and of every path
     # against the FP64 GPU path on all points
@@ -440,7 +549,18 @@ def main():
     flops = 2.0 * d_out * L.k_raw * P
     roof_fp64 = {"bound": "valu", "kernel": "bary_eval_kernel<1>", "achieved": flops / t_b / 1e12,
                  "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                 "frac": flops / t_b / 1e12 / PEAK_FP64_TFLOPS, "avg_launch_ms": results["fp64"]["kernel_ms"]}
+                 "frac": flops / t_b / 1e12 / PEAK_FP64_TFLOPS, "avg_launch_ms": results["fp64"]["kernel_ms"],
+                 "flop_basis": "2 d_out K_raw per point (the reference's terms)"}
+    # the FP64 Newton kernel on the same algorithmic basis, and on its own terms (one FMA per
+    # node and point plus the per-group FP64 work: 2 |Lambda| + 4 groups per point)
+    t_n = results["newton-fp64"]["kernel_ms"] * 1e-3
+    flop_nt = (2.0 * NL.n_nodes + 4.0 * NL.n_groups) * P
+    roof_newton = {"bound": "valu", "kernel": "newton_eval_kernel<double>",
+                   "achieved": flops / t_n / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                   "frac": flops / t_n / 1e12 / PEAK_FP64_TFLOPS,
+                   "flop_basis": "2 d_out K_raw per point (the reference's terms)",
+                   "own_terms_tflops": flop_nt / t_n / 1e12,
+                   "avg_launch_ms": results["newton-fp64"]["kernel_ms"]}
 
     rows = bench_gradient_integral(ip, x, torch)
 
@@ -462,6 +582,13 @@ def main():
                "sample": f"{nc} of the {P} points x {reps} calls ({c1 - c0:.1f} s), FP64 torch-CPU "
                          f"restatement of the reference's padded-group einsum evaluation"}
 
+    n_nodes = int(ip.n_f_evals)
+    legs = {}
+    if not args.no_legs:
+        del ip, x, y
+        torch.cuda.empty_cache()
+        for lname in ("c3_sin_d100_o4096", "c4_tanh_d200_o256"):
+            legs[lname] = bench_gemm_leg(args, torch, W, lname, cpu=not args.no_cpu)
     serve = None if args.no_serve else bench_serving(torch, W, n_requests=args.serve_requests)
     value = results["mixed"]["value"]
     out = {
@@ -478,16 +605,19 @@ def main():
         "dtype": "f32 products / f64 accumulation",
         "data": "synthetic",
         "config": {"workload": args.config, "d_in": d, "d_out": d_out, "points": P,
-                   "nodes": int(ip.n_f_evals), "t": t, "k_raw": int(L.k_raw),
+                   "nodes": n_nodes, "t": t, "k_raw": int(L.k_raw),
                    "blocks": int(len(L.blk_n)), "internal_nodes": int(NL.n_internal),
                    "path": "mixed (hierarchical Newton form)"},
         "roofline": roofline,
         "roofline_fp64": roof_fp64,
+        "roofline_newton_fp64": roof_newton,
         "cpu_baseline": cpu,
         "paths": results,
         "speedup_mixed_vs_fp64": results["mixed"]["value"] / results["fp64"]["value"],
+        "speedup_mixed_vs_newton_fp64": results["mixed"]["value"] / results["newton-fp64"]["value"],
         "accuracy": accuracy,
         "gradient_integral": rows,
+        "legs": legs,
         "serve": serve,
         "setup_s": setup_s,
     }

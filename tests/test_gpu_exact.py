@@ -37,7 +37,7 @@ WORKLOADS = ["c1_rational_d10", "c2_exp_d1000", "c3_sin_d100_o4096", "c4_tanh_d2
 NEWTON_FP64_TOL = 1e-13
 FP64_FACTOR = 4.0
 MIXED_TOL = 1e-6
-BF16_TOL = 5e-2
+BF16_TOL = 3e-3
 GRAD_TOL = 1e-12
 
 

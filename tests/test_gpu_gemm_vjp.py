@@ -5,7 +5,7 @@ tests/test_oracle_extended.py).  Tolerances (max |diff| / max |ref|): the Newton
 1e-13 (no cancellation between its terms); the barycentric combination technique in FP64 within
 4x the FP64 oracle's own distance from the exact values (the rounding that the Smolyak
 coefficients amplify, the reference's as well), at least 1e-14; mixed (FP64 head rows, FP32 MFMA
-with FP64 accumulation) 1e-6; bf16 (FP64 head rows, BF16 MFMA) 2e-2; gradient and adjoint in
+with FP64 accumulation) 1e-6; bf16 (FP64 head rows, BF16 MFMA) 3e-3 (measured 2e-4 to 8e-4); gradient and adjoint in
 FP64 1e-12, mixed adjoint 1e-6; central finite differences with h = 1e-6 in FP64: 1e-6.
 """
 
@@ -22,7 +22,7 @@ pytestmark = pytest.mark.gpu
 
 NEWTON_TOL = 1e-13
 MIXED_TOL = 1e-6
-BF16_TOL = 2e-2
+BF16_TOL = 3e-3
 GRAD_TOL = 1e-12
 
 
