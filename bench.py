@@ -539,8 +539,8 @@ def main():
     NL = ip._nlayout
     t_mixed = results["mixed"]["kernel_ms"] * 1e-3
     bytes_mixed = 8.0 * P * d + 4.0 * d_out * L.k_raw + 8.0 * P * d_out
-    staged = N.fn["sbi_newton_mixed_staged"](ip._nplan) == 1
-    kname = "newton_mixed_staged_kernel" if staged else "newton_eval_kernel<float>"
+    kname = {2: "newton_mixed_mfma_kernel", 1: "newton_mixed_staged_kernel"}.get(
+        int(N.fn["sbi_newton_mixed_staged"](ip._nplan)), "newton_eval_kernel<float>")
     roofline = {"bound": "hbm", "achieved": bytes_mixed / t_mixed / 1e9, "peak": PEAK_HBM_GBS,
                 "unit": "GB/s", "frac": bytes_mixed / t_mixed / 1e9 / PEAK_HBM_GBS,
                 "traffic": measured_traffic(kname, args.config), "kernel": kname,

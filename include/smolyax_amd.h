@@ -233,6 +233,29 @@ typedef struct sbi_newton_desc {
   const int32_t* jac_red;     /* [3 * n_jac_red] {dim, first partial slot, slot count} */
   const int32_t* vjp_ent_row; /* [n_vjp_ent] factor-table row of every adjoint entry */
   const int32_t* row_tdim;    /* [n_tab_rows] table dim index of every factor-table row */
+  /* matrix-core mixed kernel (d_out = 1; optional: NULL mt_a selects another kernel).  The
+   * internal nodes and the root are numbered by increasing cost sum_j k_j nu_j (rows of W in that
+   * order); each group (j, k) keeps a dense vector over the rows 0 .. (its last parent) of
+   * c_k * surplus (FP32, zero where a row is not a parent). */
+  int32_t rw_n;               /* rows of W (internal nodes + root) */
+  int32_t _pad6;
+  const int32_t* rint_rec;    /* [n_internal] W row | parent W row << 16, in depth order (the
+                                 depth levels of level_off) */
+  const int32_t* rint_h;      /* [n_internal] h-table row of the node's last factor */
+  /* Groups in tiles of 16; a tile's dense surplus vectors form one operand of
+   * v_mfma_f32_16x16x4 against 4 rows of W for 16 points, one 64-float step per 4 rows (lane l:
+   * group l % 16, row l / 16 of the step). */
+  int32_t mt_n;               /* tiles */
+  int32_t mt_steps;           /* A steps */
+  int32_t mi_n;               /* work items: step ranges of one tile */
+  int32_t _pad5;
+  const float* mt_a;          /* [64 * mt_steps] */
+  const void* mt_grp;         /* [16 * mt_n] 16-byte group descriptors {int32 dim, int32 degree;
+                                 double xi_0 of the dim} (dummies: dim 0, degree 1, zero A) */
+  const int32_t* mi_rec;      /* [4 * mi_n] {first A step, first step within the tile (W rows
+                                 4 s ..), step count, tile} */
+  const int32_t* mwave_item;  /* [5] work-item ranges of the 4 wave groups of a point tile (a
+                                 group of 4 waves takes 16 of the 64 points each) */
 } sbi_newton_desc;
 
 typedef struct sbi_newton_plan sbi_newton_plan;
@@ -246,8 +269,9 @@ int sbi_newton_plan_destroy(sbi_newton_plan* plan);
  * sbi_contract gives the same polynomial. */
 int sbi_newton_eval_mixed(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
                           double* y, void* stream);
-/* 1 if sbi_newton_eval_mixed runs the LDS-staged kernel for this plan (node tables fit in LDS),
- * 0 if it runs the streaming kernel that reads them from global memory. */
+/* Which kernel sbi_newton_eval_mixed runs for this plan: 2 the matrix-core kernel (the groups'
+ * sums on the FP32 matrix cores), 1 the LDS-staged kernel (node tables fit in LDS), 0 the
+ * streaming kernel that reads them from global memory. */
 int sbi_newton_mixed_staged(const sbi_newton_plan* plan);
 /* Same in FP64 throughout. */
 int sbi_newton_eval_f64(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,

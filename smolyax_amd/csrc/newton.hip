@@ -454,11 +454,178 @@ __global__ __launch_bounds__(kTile* kWaves) void newton_mixed_staged_kernel(
   }
 }
 
+// ------------------------------------------------------------------------------------------
+// Matrix-core mixed-precision kernel (d_out = 1).  The rows of W (internal nodes and the root)
+// are numbered by increasing cost, so a group (j, k) sums t = sum_q S_g[q] W[q] over a dense
+// vector S_g of its parents' rows 0 .. len_g - 1 (zeros in the gaps) and needs no parent index.
+// The groups' dense vectors, in tiles of 16 groups, are contracted with W on the FP32 matrix cores: per 4 rows, v_mfma_f32_16x16x4 takes A = the tile's surpluses (lane l: group
+// l % 16, row l / 16) and B = W for 16 points (lane l: row l / 16, point l % 16), four times for
+// the tile's 64 points.  The result T[group, point] (lane l, register r: group 4 (l / 16) + r,
+// point l % 16 of the subset) is multiplied in FP64 by the Newton factor
+// h(x) = prod_{r < k} (x_j - xi_r) of the group (c_k is folded into the surpluses) and summed.
+// Every LDS read of the loop is a 256-byte lane-ordered vector: no broadcast reads.
+//   LDS: [W: rw_pad rows x 64 floats, element (q, p) at ((q / 4) * 64 + p) * 4 + q % 4][region R]
+//   region R, phase 1: h table of the internal nodes' factors
+//   region R, phase 2: A steps (256 bytes each) | group descriptors (16 bytes) | work items
+// 12 waves per 64-point tile take work items (step ranges of one tile; y is linear in the partial
+// sums) balanced on the host.  (The operand roles are swapped in the loop: see step 3.)
+// ------------------------------------------------------------------------------------------
+constexpr int kMWaves = 16;   // waves per tile: kMGroups wave groups x 4 point subsets
+constexpr int kMGroups = 4;   // wave groups (the host balances the work items over them)
+
+__device__ __forceinline__ int widx(int q, int p) { return ((q >> 2) * kTile + p) * 4 + (q & 3); }
+
+__global__ __launch_bounds__(kTile* kMWaves) void newton_mixed_mfma_kernel(
+    sbi_newton_desc d, const double* __restrict__ x, int64_t P, int64_t ldx, double* __restrict__ y) {
+  extern __shared__ double lds_raw[];
+  lds_f32* W = (lds_f32*)lds_raw;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t tile0 = (int64_t)blockIdx.x * kTile;
+  const int64_t p_raw = tile0 + lane;
+  const bool valid = p_raw < P;
+  const int64_t p = valid ? p_raw : (P - 1);
+  const double* __restrict__ xp = x + p * ldx;
+  const int rw_pad = (d.rw_n + 3) & ~3;
+  lds_f32* R = W + rw_pad * kTile;  // region R (16-byte aligned: rw_pad * 256 bytes)
+
+  cptr<int32_t> hs_dim = C(d.hs_dim), hs_kmax = C(d.hs_kmax), hs_row = C(d.hs_row),
+                rint_rec = C(d.rint_rec), rint_h = C(d.rint_h), level_off = C(d.level_off),
+                wave_item = C(d.mwave_item), mi_rec = C(d.mi_rec);
+  cptr<double> dim_xi = C(d.dim_xi), dim_c = C(d.dim_c);
+
+  // Work items (step 3): wave w = 4 g + s takes the items of wave group g for the 16 points
+  // 16 s .. 16 s + 15 of the tile.  Lane l owns group l % 16 of an item's tile and the points
+  // 16 s + 4 (l / 16) + r (r < 4).  Coordinates are read four items ahead of their use; the first
+  // four items' are requested here, before the tables, from the descriptors in global memory.
+  const int li = lane & 15, qd = lane >> 4;
+  const int sub = wave & 3, wgrp = wave >> 2;
+  const double* xr[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) xr[r] = x + min(tile0 + 16 * sub + 4 * qd + r, P - 1) * ldx;
+  const int i_begin = wave_item[wgrp], i_end = wave_item[wgrp + 1];
+  const v4i* __restrict__ gdesc = (const v4i*)d.mt_grp;
+  struct Pre {
+    int j, k;
+    double a, xv[4];
+  };
+  auto fetch = [&](const v4i gdsc) {
+    Pre q;
+    q.j = gdsc.x;
+    q.k = gdsc.y;
+    q.a = __hiloint2double(gdsc.w, gdsc.z);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) q.xv[r] = xr[r][q.j];
+    return q;
+  };
+  const v4i dummy = v4i{0, 1, 0, 0};
+  auto first = [&](int i) { return i < i_end ? gdesc[mi_rec[4 * i + 3] * 16 + li] : dummy; };
+  Pre n1 = fetch(first(i_begin)), n2 = fetch(first(i_begin + 1));
+  Pre n3 = fetch(first(i_begin + 2)), n4 = fetch(first(i_begin + 3));
+
+  // 1a. h values of the internal nodes' factors (lane = point), FP64 -> FP32
+  for (int i = wave; i < d.n_hs_dims; i += kMWaves) {
+    const int j = hs_dim[i], km = hs_kmax[i], row = hs_row[i];
+    const double xj = xp[j];
+    cptr<double> xi = dim_xi + (int64_t)j * d.kmax1;
+    cptr<double> cj = dim_c + (int64_t)j * d.kmax1;
+    double pr = 1.0;
+    for (int k = 1; k <= km; ++k) {
+      pr *= (xj - xi[k - 1]);
+      R[(row + k - 1) * kTile + lane] = (float)(pr * cj[k]);
+    }
+  }
+  for (int q = wave; q < rw_pad; q += kMWaves)
+    if (q == 0 || q >= d.rw_n) W[widx(q, lane)] = (q == 0) ? 1.0f : 0.0f;
+  __syncthreads();
+  // 1b. products of the internal nodes, depth by depth: W[row] = W[parent] * h
+  for (int lv = 0; lv < d.n_levels; ++lv) {
+    const int a = level_off[lv], b = level_off[lv + 1];
+    for (int q = a + wave; q < b; q += kMWaves) {
+      const uint32_t rec = (uint32_t)rint_rec[q];
+      const int row = rec & 0xffff, par = rec >> 16;
+      W[widx(row, lane)] = W[widx(par, lane)] * R[rint_h[q] * kTile + lane];
+    }
+    __syncthreads();
+  }
+
+  // 2. stage the A steps, the group descriptors and the work items (region R)
+  typedef __attribute__((address_space(3))) char lds_char;
+  lds_char* AS = (lds_char*)R;
+  lds_char* GD = AS + (size_t)d.mt_steps * 256;
+  lds_char* IT = GD + (size_t)d.mt_n * 256;
+  auto stage = [&](const void* src, lds_char* dst, int nbytes) {
+    for (int c = wave; c * 1024 < nbytes; c += kMWaves) {
+      const int off = c * 1024 + lane * 16;
+      if (off < nbytes)
+        __builtin_amdgcn_global_load_lds((const char*)src + off, (lds_void*)(dst + c * 1024), 16, 0,
+                                         0);
+    }
+  };
+  stage(d.mt_a, AS, d.mt_steps * 256);
+  stage(d.mt_grp, GD, d.mt_n * 256);
+  stage(d.mi_rec, IT, d.mi_n * 16);
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed
+  __syncthreads();
+  const lds_f32* A = (const lds_f32*)AS;
+  const __attribute__((address_space(3))) v4i* GI = (const __attribute__((address_space(3))) v4i*)GD;
+  const __attribute__((address_space(3))) v4i* II = (const __attribute__((address_space(3))) v4i*)IT;
+
+  // 3. work items.  v_mfma_f32_16x16x4 with A = W (lane l: point 16 s + l % 16 of subset s,
+  // row l / 16 of the step) and B = the tile's surpluses (lane l: row l / 16, group l % 16): the
+  // result T[point, group] sits in lane l, register r at point 16 s + 4 (l / 16) + r, group l % 16.
+  // So each lane owns one group of the tile (its dim j) and 16 points: its coordinates are x_j of
+  // 16 rows, and the 16 lanes of a lane row read 16 dims of one row (for the consecutive dims of
+  // the prefix tiles, one 128-byte line).
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int it = i_begin; it < i_end; ++it) {
+    const v4i ir = II[it];
+    const int a0 = __builtin_amdgcn_readfirstlane(ir.x), w0 = __builtin_amdgcn_readfirstlane(ir.y);
+    const int ns = __builtin_amdgcn_readfirstlane(ir.z);
+    const Pre cur = n1;
+    n1 = n2;
+    n2 = n3;
+    n3 = n4;
+    if (it + 4 < i_end) n4 = fetch(GI[__builtin_amdgcn_readfirstlane(II[it + 4].w) * 16 + li]);
+    v4f D = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    const int wb = sub * 64 + li * 4 + qd;
+    for (int st = 0; st < ns; ++st)
+      D = __builtin_amdgcn_mfma_f32_16x16x4f32(W[(w0 + st) * kTile * 4 + wb],
+                                               A[(a0 + st) * kTile + lane], D, 0, 0, 0);
+    cptr<double> xi = dim_xi + (int64_t)cur.j * d.kmax1;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double pr = cur.xv[r] - cur.a;
+      for (int k = 1; k < cur.k; ++k) pr *= (cur.xv[r] - xi[k]);
+      acc[r] = fma(pr, (double)D[r], acc[r]);
+    }
+  }
+
+  // 4. deterministic reduction in FP64: each point's sums over the 16 lanes of its lane row (one
+  // group each) by a fixed butterfly, then over the wave groups in order
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int m = 8; m > 0; m >>= 1) acc[r] += __shfl_xor(acc[r], m, 16);
+  __syncthreads();
+  lds_f64* Rd = (lds_f64*)lds_raw;
+  if (li == 0) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Rd[wgrp * kTile + 16 * sub + 4 * qd + r] = acc[r];
+  }
+  __syncthreads();
+  if (wave == 0 && valid) {
+    double sacc = d.s_root[0];
+    for (int w = 0; w < kMGroups; ++w) sacc += Rd[w * kTile + lane];
+    y[p] = sacc;
+  }
+}
+
 }  // namespace
 
 struct sbi_newton_plan {
   sbi_newton_desc d;
-  size_t lds_f32, lds_f64, lds_staged;
+  size_t lds_f32, lds_f64, lds_staged, lds_mfma;
 };
 
 extern "C" int sbi_newton_plan_create(const sbi_newton_desc* desc, sbi_newton_plan** out) {
@@ -495,6 +662,20 @@ extern "C" int sbi_newton_plan_create(const sbi_newton_desc* desc, sbi_newton_pl
                         ? tot
                         : (size_t)-1;
   }
+  // matrix-core kernel: W table + max(h table, A steps + descriptors + work items), at least the
+  // reduction's 16 x 4 x 64 doubles
+  {
+    const size_t w = (size_t)((d.rw_n + 3) & ~3) * kTile * 4;
+    const size_t hs = (size_t)d.n_hs_rows * kTile * 4;
+    const size_t st = (size_t)d.mt_steps * 256 + (size_t)d.mt_n * 256 + (size_t)d.mi_n * 16;
+    size_t tot = w + (hs > st ? hs : st);
+    const size_t red = (size_t)kMWaves * 4 * kTile * sizeof(double);
+    if (tot < red) tot = red;
+    p->lds_mfma = (d.mt_a && d.mt_grp && d.mi_rec && d.mwave_item && d.rint_rec && d.rint_h &&
+                   d.hs_dim && d.rw_n == d.n_internal + 1 && d.mt_n > 0 && d.mi_n > 0)
+                      ? tot
+                      : (size_t)-1;
+  }
   // a node table beyond LDS leaves the plan to the basis, Jacobian and adjoint entry points: the
   // streaming kernels then return SBI_ERR_CAPACITY (the caller takes the GEMM form)
   *out = p;
@@ -506,6 +687,7 @@ extern "C" int sbi_newton_mixed_staged(const sbi_newton_plan* plan) {
     sbi::set_error("sbi_newton_mixed_staged: null plan");
     return SBI_ERR_INVALID;
   }
+  if (plan->lds_mfma <= (size_t)kMaxLds) return 2;
   return plan->lds_staged <= (size_t)kMaxLds ? 1 : 0;
 }
 
@@ -553,15 +735,24 @@ extern "C" int sbi_newton_eval_mixed(const sbi_newton_plan* plan, const double* 
     sbi::set_error("sbi_newton_eval_mixed: invalid argument (the streaming kernel takes d_out == 1)");
     return SBI_ERR_INVALID;
   }
+  if (plan->lds_mfma <= (size_t)kMaxLds) {
+    if (P == 0) return SBI_OK;
+    sbi::allow_lds((const void*)newton_mixed_mfma_kernel);
+    hipLaunchKernelGGL(newton_mixed_mfma_kernel, dim3((unsigned)((P + kTile - 1) / kTile)),
+                       dim3(kTile * kMWaves), plan->lds_mfma, (hipStream_t)stream, plan->d, x, P,
+                       ldx, y);
+    SBI_HIP_CHECK(hipGetLastError());
+    return SBI_OK;
+  }
   if (plan->lds_staged > (size_t)kMaxLds) return newton_launch<float>(plan, x, P, ldx, y, stream);
   if (P == 0) return SBI_OK;
-  sbi::allow_lds((const void*)newton_mixed_staged_kernel<true>);
-  sbi::allow_lds((const void*)newton_mixed_staged_kernel<false>);
   dim3 grid((unsigned)((P + kTile - 1) / kTile));
   // 16-byte coordinate loads need an even row stride, a 16-byte aligned x and an even d_in >= 2
   // (every pair {2i, 2i+1} with 2i < d_in then lies inside the row)
   const bool pairs = (ldx % 2 == 0) && ((uintptr_t)x % 16 == 0) && plan->d.d_in >= 2 &&
                      plan->d.d_in % 2 == 0;
+  sbi::allow_lds((const void*)newton_mixed_staged_kernel<true>);
+  sbi::allow_lds((const void*)newton_mixed_staged_kernel<false>);
   if (pairs)
     hipLaunchKernelGGL(newton_mixed_staged_kernel<true>, grid, dim3(kTile * kWaves),
                        plan->lds_staged, (hipStream_t)stream, plan->d, x, P, ldx, y);

@@ -327,6 +327,13 @@ class SmolyakBarycentricInterpolator:
         t["sing_off"], t["sing_x"] = _dev_tensor(sing_off, i32, dev), _dev_tensor(sing_x, f64, dev)
         if NL.node_sq is not None:  # staged kernel tables (else the unstaged kernel runs)
             t["node_sq"] = _dev_tensor(NL.node_sq, i32, dev)
+        if getattr(NL, "mt_a", None) is not None:  # matrix-core kernel tables
+            t["rint_rec"] = _dev_tensor(NL.rint_rec, i32, dev)
+            t["rint_h"] = _dev_tensor(NL.rint_h, i32, dev)
+            t["mt_a"] = _dev_tensor(NL.mt_a, f32, dev)
+            t["mt_grp"] = torch.as_tensor(NL.mt_grp.view(np.uint8).copy()).to(dev)
+            t["mi_rec"] = _dev_tensor(NL.mi_rec, i32, dev)
+            t["mwave_item"] = _dev_tensor(NL.mwave_item, i32, dev)
         # surpluses in nf.nus order (adjoint), and in the basis-row order of the GEMM paths (the
         # FP64 head rows first)
         self._S = {"f64": _dev_tensor(self._nform.s, f64, dev)}
@@ -341,6 +348,9 @@ class SmolyakBarycentricInterpolator:
         d.basis_n, d.n_tab_dims, d.n_tab_rows = NL.basis_n, NL.n_tab_dims, NL.n_tab_rows
         d.n_vjp_ent = NL.n_vjp_ent
         d.n_jac_units, d.n_jac_partials, d.n_jac_red = NL.n_jac_units, NL.n_jac_partials, NL.n_jac_red
+        if "mt_a" in t:
+            d.rw_n = NL.rw_n
+            d.mt_n, d.mt_steps, d.mi_n = NL.mt_n, NL.mt_steps, NL.mi_n
         for name, ten in t.items():
             setattr(d, name, ten.data_ptr())
         self._ndesc = d

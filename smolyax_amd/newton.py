@@ -92,6 +92,7 @@ class NewtonForm:
         if not node_gen.is_nested:
             raise ValueError("the Newton (hierarchical) form needs nested nodes (Leja)")
         self.nus = indices.indexset(k, t)
+        self.k = np.asarray(k, dtype=np.float64)
         self.d_in = len(k)
         self.d_out = d_out
         self.node_gen = node_gen
@@ -268,12 +269,145 @@ def build_newton_layout(nf: NewtonForm, n_waves: int = N_WAVES) -> NewtonLayout:
     else:  # 16-bit record fields overflow: no staged kernel (its tables are left out)
         irec, L.node_sq = [], None
     L.int_rec = np.array(irec or [0], dtype=np.int32)
+    build_cost_order_tables(nf, L, rows)
     build_basis_tables(nf, L)
     # basis rows for the GEMM paths: the heaviest rows first (FP64 head of the mixed contraction)
     L.basis_order, L.n_head, L.head_fac = head_order(nf, L)
     desc = L.basis_desc.view(np.uint16).reshape(L.basis_n, 8)
     L.basis_desc = np.ascontiguousarray(desc[L.basis_order]).reshape(-1).view(np.uint8)
     return L
+
+
+LINE_DIMS = 16    # dims per 128-byte line of coordinates
+
+
+def build_cost_order_tables(nf: NewtonForm, L: NewtonLayout, hs_rows: dict) -> None:
+    """Cost-ordered product table and dense surplus vectors (matrix-core mixed kernel, d_out = 1).
+
+    The internal nodes (and the root) are numbered by increasing cost sum_j k_j nu_j: the parents
+    of a group (j, k) are the internal nodes q with every dim below j and cost(q) + k k_j within
+    the threshold, so for dims past every internal node's dims they are exactly the first rows of
+    that order, and for the others a short prefix with gaps.  Each group keeps a dense vector of
+    c_k * s (FP32) over the rows 0 .. (its last parent), zeros in the gaps: t = sum_q S_g[q] W[q]
+    needs no parent indices.
+
+    Sets: rw_n (rows of W), rint_rec [n_internal] (row | parent row << 16, depth order),
+    rint_h [n_internal] (h-table row), and the matrix-core tables (build_mfma_tables); mt_a is
+    None when a limit is exceeded (then the staged kernel serves the plan).
+    """
+    nus = nf.nus
+    k = nf.k
+    cost = lambda nu: float(sum(k[j] * v for j, v in nu))
+    parent_of = {nu: nu[:-1] for nu in nus if nu}
+    internal = sorted({p for p in parent_of.values() if p})
+    order = sorted([()] + internal, key=lambda v: (cost(v), len(v), v))
+    rrow = {v: i for i, v in enumerate(order)}
+    L.rw_n = len(order)
+    lvl = sorted(internal, key=lambda v: (len(v), rrow[v]))
+    L.rint_rec = np.array([rrow[v] | (rrow[v[:-1]] << 16) for v in lvl] or [0], dtype=np.int64)
+    L.rint_rec = L.rint_rec.astype(np.uint32).view(np.int32)
+    L.rint_h = np.array([hs_rows[v[-1][0]] + v[-1][1] - 1 for v in lvl] or [0], dtype=np.int32)
+    g4 = L.grp.reshape(-1, 4)
+    node_by_grp = sorted((nu for nu in nus if nu), key=lambda v: (v[-1][0], v[-1][1]))
+    idx = {nu: i for i, nu in enumerate(nus)}
+    s0 = np.asarray(nf.s, dtype=np.float64).reshape(len(nus), -1)[:, 0]
+    vecs = []
+    for g in range(L.n_groups):
+        n0, n1, j, kk = (int(v) for v in g4[g])
+        members = node_by_grp[n0:n1]
+        vec = np.zeros(max(rrow[v[:-1]] for v in members) + 1)
+        for v in members:
+            vec[rrow[v[:-1]]] = s0[idx[v]] * L.dim_c[j, kk]
+        vecs.append(vec)
+    L.mt_a = None
+    if not vecs or L.rw_n >= 65536 or not all(np.all(np.abs(v) < 1e37) for v in vecs):
+        return
+    build_mfma_tables(L, vecs)
+
+
+MFMA_WAVES = 4       # wave groups per point tile of the matrix-core mixed kernel (kMGroups)
+MFMA_ITEM_STEPS = 16  # k steps per work item at most (longer tiles are split over items)
+MFMA_ITEM_COST = 12.0  # per-item work besides its k steps (coordinates, Newton factors, sums)
+
+
+def build_mfma_tables(L: NewtonLayout, vecs, n_waves: int = MFMA_WAVES) -> None:
+    """Tables of the matrix-core mixed kernel (d_out = 1): the groups' dense surplus vectors
+    (cost-ordered rows, see build_cost_order_tables) contracted with the product table W on the
+    FP32 matrix cores, 16 groups x 4 rows x 16 points per v_mfma_f32_16x16x4.
+
+    Groups are packed 16 to a tile: those whose vectors are prefixes (every dim past the internal
+    nodes' dims) in their order (consecutive dims), the others sorted by length; dummy groups
+    (dim 0, degree 1, zero surpluses) fill the last tile of each kind.  A tile's A operand is
+    stored step by step: for rows 4s .. 4s + 3, 64 floats in MFMA lane order (lane l holds group
+    l % 16, row 4s + l / 16), over as many steps as its longest vector needs.  A work item is a
+    range of at most MFMA_ITEM_STEPS steps of one tile (y is linear in the partial sums, so a
+    tile's steps may be split over items).
+
+    Sets: mt_n (tiles), mt_grp (16-byte group descriptors [16 * mt_n]: {int32 dim, int32 degree,
+    double xi_0 of the dim}), mt_a (FP32 A steps, 64 per step), mt_steps, mi_n (work items),
+    mi_rec (int32 [4 * mi_n]: first A step, first step within the tile, step count, tile),
+    mwave_item [n_waves + 1] (item ranges per wave group, balanced by steps + MFMA_ITEM_COST).
+    """
+    g4 = L.grp.reshape(-1, 4)
+    ng = L.n_groups
+    lens = np.array([len(v) for v in vecs])
+    last_int_dim = int(L.int_dim[: L.n_internal].max()) if L.n_internal else -1
+    pre = [g for g in range(ng) if g4[g, 2] > last_int_dim]
+    other = sorted((g for g in range(ng) if g4[g, 2] <= last_int_dim), key=lambda g: -lens[g])
+    tiles = []
+    for i in range(0, len(other), 16):
+        tiles.append(other[i:i + 16])
+    # prefix groups: a new tile also where the dims enter a new 128-byte line of coordinates, so
+    # that a tile's coordinate reads never share a line with another tile's
+    cur = []
+    for g in pre:
+        if cur and (len(cur) == 16 or g4[g, 2] // LINE_DIMS != g4[cur[-1], 2] // LINE_DIMS):
+            tiles.append(cur)
+            cur = []
+        cur.append(g)
+    if cur:
+        tiles.append(cur)
+    A, desc, tile_a0, tile_ns = [], [], [], []
+    s0 = 0
+    for t in tiles:
+        ns = int(-(-max(lens[g] for g in t) // 4))
+        blk = np.zeros((ns, 4, 16), dtype=np.float32)  # [step, row in step, group]
+        for i, g in enumerate(t):
+            v = np.zeros(4 * ns)
+            v[: len(vecs[g])] = vecs[g]
+            blk[:, :, i] = v.reshape(ns, 4)
+        A.append(blk.reshape(ns, 64))  # lane l = 16 * (row in step) + group
+        tile_a0.append(s0)
+        tile_ns.append(ns)
+        s0 += ns
+        for i in range(16):
+            if i < len(t):
+                j, kk = int(g4[t[i], 2]), int(g4[t[i], 3])
+                desc.append((j, kk, L.dim_xi[j, 0]))
+            else:
+                desc.append((0, 1, 0.0))
+    L.mt_n = len(tiles)
+    L.mt_a = np.concatenate(A).reshape(-1).astype(np.float32) if A else np.zeros(64, np.float32)
+    L.mt_steps = s0
+    descdt = np.dtype([("j", "<i4"), ("k", "<i4"), ("a", "<f8")])
+    L.mt_grp = np.array(desc or [(0, 1, 0.0)] * 16, dtype=descdt)
+    items = []
+    for ti, (a0, ns) in enumerate(zip(tile_a0, tile_ns)):
+        cut = int(-(-ns // MFMA_ITEM_STEPS))
+        per = int(-(-ns // cut))
+        for c in range(0, ns, per):
+            items.append((a0 + c, c, min(per, ns - c), ti))
+    cost = [it[2] + MFMA_ITEM_COST for it in items]
+    load = [0.0] * n_waves
+    owner = [[] for _ in range(n_waves)]
+    for ii in sorted(range(len(items)), key=lambda i: -cost[i]):
+        w = int(np.argmin(load))
+        owner[w].append(ii)
+        load[w] += cost[ii]
+    L.mwave_item = np.array(np.cumsum([0] + [len(o) for o in owner]), dtype=np.int32)
+    items = [items[i] for w in range(n_waves) for i in sorted(owner[w])]
+    L.mi_n = len(items)
+    L.mi_rec = np.array(items or [(0, 0, 0, 0)], dtype=np.int32).reshape(-1)
 
 
 HEAD_ROWS = 64  # rows of the Newton basis that the mixed GEMM path contracts in FP64
@@ -420,4 +554,41 @@ def eval_newton_layout(L: NewtonLayout, x, fp32=False):
         for i in range(n0, n1):
             t = (t + s[i] * W[L.node_q[i]]).astype(T)
         acc += h * t.astype(np.float64)
+    return acc + L.s_root[0]
+
+
+def eval_mfma_layout(L: NewtonLayout, x):
+    """Numpy model of the matrix-core mixed kernel (test helper): per tile, T = A^T W over its
+    steps in FP32, then y += h_k(x_j) T in FP64 with h_k(x) = prod_{r<k} (x - xi_r) (the c_k of
+    the Newton basis is folded into the surpluses)."""
+    x = np.atleast_2d(np.asarray(x, dtype=np.float64))
+    P = x.shape[0]
+    h = np.zeros((max(L.n_hs_rows, 1), P), dtype=np.float32)
+    for i in range(L.n_hs_dims):
+        j, km, r0 = int(L.hs_dim[i]), int(L.hs_kmax[i]), int(L.hs_row[i])
+        pr = np.ones(P)
+        for kk in range(1, km + 1):
+            pr = pr * (x[:, j] - L.dim_xi[j, kk - 1])
+            h[r0 + kk - 1] = (pr * L.dim_c[j, kk]).astype(np.float32)
+    W = np.zeros((-(-L.rw_n // 4) * 4, P), dtype=np.float32)
+    W[0] = 1.0
+    rec = L.rint_rec.view(np.uint32)
+    for i in range(L.n_internal):
+        r, par = int(rec[i] & 0xFFFF), int(rec[i] >> 16)
+        W[r] = W[par] * h[int(L.rint_h[i])]
+    A = L.mt_a.reshape(-1, 4, 16)
+    acc = np.zeros(P)
+    for it in range(L.mi_n):
+        a0, w0, ns, t = (int(v) for v in L.mi_rec[4 * it: 4 * it + 4])
+        T = np.zeros((16, P), dtype=np.float32)
+        for s_ in range(ns):
+            rows = W[4 * (w0 + s_): 4 * (w0 + s_) + 4]
+            T = (T + A[a0 + s_].T.astype(np.float32) @ rows).astype(np.float32)
+        for i in range(16):
+            dsc = L.mt_grp[16 * t + i]
+            j, kk = int(dsc["j"]), int(dsc["k"])
+            pr = np.ones(P)
+            for r in range(kk):
+                pr = pr * (x[:, j] - L.dim_xi[j, r])
+            acc += pr * T[i].astype(np.float64)
     return acc + L.s_root[0]

@@ -106,3 +106,51 @@ def test_gemm_row_order_puts_heavy_rows_first():
     for r in range(L.n_head):
         got = sorted((int(j), int(kk)) for j, kk in fac[r] if j >= 0)
         assert got == sorted(nf.nus[order[r]])
+
+
+@pytest.mark.parametrize("d,m", [(1, 9), (3, 40), (10, 1000), (200, 5000), (1000, 10000)])
+def test_matrix_core_layout_matches_oracle(d, m):
+    """Tables of the matrix-core mixed kernel: cost-ordered rows of W, dense surplus vectors in
+    tiles of 16 groups (A steps in MFMA lane order), work items and their wave groups.  The numpy
+    model of its arithmetic matches the oracle (FP64 Newton layout at large d) to 1e-6; every
+    node's folded surplus appears exactly once in the A steps; every tile's steps are covered by
+    its items exactly once; the wave groups cover the items in order."""
+    from smolyax_amd import newton as NW
+    from smolyax_amd import nodes as SN
+
+    rng = np.random.default_rng(d)
+    k = np.log((2 + np.arange(d)) / np.log(2))
+    t = OI.find_approximate_threshold(k, m, True)
+    a = rng.standard_normal(d) / (np.arange(d) + 1.0) ** 2
+    f = lambda x: np.exp(-(np.asarray(x) @ a))
+    sng = SN.Leja(dim=d)
+    nf = NW.NewtonForm(k, t, sng, 1)
+    X = np.stack([nf.grid_point(nu, np.zeros(d)) for nu in nf.nus])
+    nf.set_values(f(X).reshape(-1, 1))
+    L = NW.build_newton_layout(nf)
+    assert L.mt_a is not None
+    x = rng.uniform(-1, 1, (32, d))
+    y = NW.eval_mfma_layout(L, x)
+    if d <= 10:
+        ref = OS.Interpolant(d_out=1, node_gen=ON.leja(dim=d), k=k, t=t, f=f)
+        yr = ref(x)[:, 0]
+    else:
+        yr = NW.eval_newton_layout(L, x, fp32=False)
+    assert np.max(np.abs(y - yr)) / np.max(np.abs(yr)) < 1e-6
+    # each node's folded surplus once: the nonzero A entries are the nodes' c_k * s (nonzero ones)
+    g4 = L.grp.reshape(-1, 4)
+    cnt = np.diff(g4[:, :2], axis=1)[:, 0]
+    folded = L.node_s32.astype(np.float64) * L.dim_c[g4[:, 2].repeat(cnt), g4[:, 3].repeat(cnt)]
+    assert np.count_nonzero(L.mt_a) == np.count_nonzero(folded)
+    # items cover each tile's steps once, in A-step order
+    rec = L.mi_rec.reshape(-1, 4)
+    cover = np.zeros(L.mt_steps, dtype=int)
+    for a0, w0, ns, tt in rec[: L.mi_n]:
+        cover[a0: a0 + ns] += 1
+        assert 0 <= tt < L.mt_n and w0 >= 0
+    assert np.all(cover == 1)
+    assert L.mwave_item[0] == 0 and L.mwave_item[-1] == L.mi_n
+    assert np.all(np.diff(L.mwave_item) >= 0) and len(L.mwave_item) == NW.MFMA_WAVES + 1
+    # the depth levels of the cost-ordered records are those of level_off
+    depth = [len(v) for v in sorted({nu[:-1] for nu in nf.nus if len(nu) > 1}, key=len)]
+    assert list(np.diff(L.level_off)) == [depth.count(dd) for dd in sorted(set(depth))]
