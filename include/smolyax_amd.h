@@ -248,7 +248,8 @@ typedef struct sbi_newton_desc {
   int32_t mt_n;               /* tiles */
   int32_t mt_steps;           /* A steps */
   int32_t mi_n;               /* work items: step ranges of one tile */
-  int32_t _pad5;
+  int32_t n_xi_dims;          /* dims 0 .. n_xi_dims - 1 hold every group of degree >= 2: their
+                                 rows of dim_xi are staged on chip */
   const float* mt_a;          /* [64 * mt_steps] */
   const void* mt_grp;         /* [16 * mt_n] 16-byte group descriptors {int32 dim, int32 degree;
                                  double xi_0 of the dim} (dummies: dim 0, degree 1, zero A) */

@@ -61,7 +61,7 @@ class NewtonDesc(ctypes.Structure):
         ("n_jac_units", c_i32), ("n_jac_partials", c_i32), ("n_jac_red", c_i32), ("_pad4", c_i32),
         ("jac_unit", c_vp), ("jac_red", c_vp), ("vjp_ent_row", c_vp), ("row_tdim", c_vp),
         ("rw_n", c_i32), ("_pad6", c_i32), ("rint_rec", c_vp), ("rint_h", c_vp),
-        ("mt_n", c_i32), ("mt_steps", c_i32), ("mi_n", c_i32), ("_pad5", c_i32),
+        ("mt_n", c_i32), ("mt_steps", c_i32), ("mi_n", c_i32), ("n_xi_dims", c_i32),
         ("mt_a", c_vp), ("mt_grp", c_vp), ("mi_rec", c_vp), ("mwave_item", c_vp),
     ]
 

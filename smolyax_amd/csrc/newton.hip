@@ -518,10 +518,12 @@ __global__ __launch_bounds__(kTile* kMWaves) void newton_mixed_mfma_kernel(
     for (int r = 0; r < 4; ++r) q.xv[r] = xr[r][q.j];
     return q;
   };
-  const v4i dummy = v4i{0, 1, 0, 0};
-  auto first = [&](int i) { return i < i_end ? gdesc[mi_rec[4 * i + 3] * 16 + li] : dummy; };
-  Pre n1 = fetch(first(i_begin)), n2 = fetch(first(i_begin + 1));
-  Pre n3 = fetch(first(i_begin + 2)), n4 = fetch(first(i_begin + 3));
+  // (past the group's items: item 0's descriptor, loaded but never used; mi_n >= 1)
+  auto first = [&](int i) { return gdesc[mi_rec[4 * (i < i_end ? i : 0) + 3] * 16 + li]; };
+  // the four descriptors first, then the coordinates (a load waited for waits for all before it)
+  const v4i f1 = first(i_begin), f2 = first(i_begin + 1), f3 = first(i_begin + 2),
+            f4 = first(i_begin + 3);
+  Pre n1 = fetch(f1), n2 = fetch(f2), n3 = fetch(f3), n4 = fetch(f4);
 
   // 1a. h values of the internal nodes' factors (lane = point), FP64 -> FP32
   for (int i = wave; i < d.n_hs_dims; i += kMWaves) {
@@ -554,6 +556,7 @@ __global__ __launch_bounds__(kTile* kMWaves) void newton_mixed_mfma_kernel(
   lds_char* AS = (lds_char*)R;
   lds_char* GD = AS + (size_t)d.mt_steps * 256;
   lds_char* IT = GD + (size_t)d.mt_n * 256;
+  lds_char* XI = IT + (size_t)d.mi_n * 16;  // dim_xi rows of dims < n_xi_dims
   auto stage = [&](const void* src, lds_char* dst, int nbytes) {
     for (int c = wave; c * 1024 < nbytes; c += kMWaves) {
       const int off = c * 1024 + lane * 16;
@@ -565,6 +568,7 @@ __global__ __launch_bounds__(kTile* kMWaves) void newton_mixed_mfma_kernel(
   stage(d.mt_a, AS, d.mt_steps * 256);
   stage(d.mt_grp, GD, d.mt_n * 256);
   stage(d.mi_rec, IT, d.mi_n * 16);
+  stage(d.dim_xi, XI, d.n_xi_dims * d.kmax1 * 8);
   __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed
   __syncthreads();
   const lds_f32* A = (const lds_f32*)AS;
@@ -578,27 +582,40 @@ __global__ __launch_bounds__(kTile* kMWaves) void newton_mixed_mfma_kernel(
   // 16 rows, and the 16 lanes of a lane row read 16 dims of one row (for the consecutive dims of
   // the prefix tiles, one 128-byte line).
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int it = i_begin; it < i_end; ++it) {
+  // one work item; its coordinates are in `slot`, which is refilled with those of item it + 4
+  // once they are used.  The loop below is unrolled over the four slots so that each keeps its
+  // registers: a refill lands where it is read, never copied (a copy would wait for its loads).
+  auto process = [&](int it, Pre& slot) {
     const v4i ir = II[it];
     const int a0 = __builtin_amdgcn_readfirstlane(ir.x), w0 = __builtin_amdgcn_readfirstlane(ir.y);
     const int ns = __builtin_amdgcn_readfirstlane(ir.z);
-    const Pre cur = n1;
-    n1 = n2;
-    n2 = n3;
-    n3 = n4;
-    if (it + 4 < i_end) n4 = fetch(GI[__builtin_amdgcn_readfirstlane(II[it + 4].w) * 16 + li]);
+    const v4i gnext = GI[__builtin_amdgcn_readfirstlane(II[it + 4 < i_end ? it + 4 : it].w) * 16 + li];
+    const Pre& cur = slot;
     v4f D = v4f{0.0f, 0.0f, 0.0f, 0.0f};
     const int wb = sub * 64 + li * 4 + qd;
     for (int st = 0; st < ns; ++st)
       D = __builtin_amdgcn_mfma_f32_16x16x4f32(W[(w0 + st) * kTile * 4 + wb],
                                                A[(a0 + st) * kTile + lane], D, 0, 0, 0);
-    cptr<double> xi = dim_xi + (int64_t)cur.j * d.kmax1;
+    // node values xi_1 .. of the group's dim from the on-chip copy (a global load here would
+    // wait for every coordinate load in flight)
+    const lds_f64* xi = (const lds_f64*)XI + cur.j * d.kmax1;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       double pr = cur.xv[r] - cur.a;
       for (int k = 1; k < cur.k; ++k) pr *= (cur.xv[r] - xi[k]);
       acc[r] = fma(pr, (double)D[r], acc[r]);
     }
+    slot = fetch(gnext);  // unconditional (past the end: this item's dims again, never used), so
+                          // that no branch hides the loads from the wait-count bookkeeping
+  };
+  for (int it = i_begin; it < i_end; it += 4) {
+    process(it, n1);
+    if (it + 1 >= i_end) break;
+    process(it + 1, n2);
+    if (it + 2 >= i_end) break;
+    process(it + 2, n3);
+    if (it + 3 >= i_end) break;
+    process(it + 3, n4);
   }
 
   // 4. deterministic reduction in FP64: each point's sums over the 16 lanes of its lane row (one
@@ -667,12 +684,14 @@ extern "C" int sbi_newton_plan_create(const sbi_newton_desc* desc, sbi_newton_pl
   {
     const size_t w = (size_t)((d.rw_n + 3) & ~3) * kTile * 4;
     const size_t hs = (size_t)d.n_hs_rows * kTile * 4;
-    const size_t st = (size_t)d.mt_steps * 256 + (size_t)d.mt_n * 256 + (size_t)d.mi_n * 16;
+    const size_t st = (size_t)d.mt_steps * 256 + (size_t)d.mt_n * 256 + (size_t)d.mi_n * 16 +
+                      (((size_t)d.n_xi_dims * d.kmax1 * 8 + 15) & ~(size_t)15);
     size_t tot = w + (hs > st ? hs : st);
     const size_t red = (size_t)kMWaves * 4 * kTile * sizeof(double);
     if (tot < red) tot = red;
     p->lds_mfma = (d.mt_a && d.mt_grp && d.mi_rec && d.mwave_item && d.rint_rec && d.rint_h &&
-                   d.hs_dim && d.rw_n == d.n_internal + 1 && d.mt_n > 0 && d.mi_n > 0)
+                   d.hs_dim && d.rw_n == d.n_internal + 1 && d.mt_n > 0 && d.mi_n > 0 &&
+                   d.n_xi_dims >= 0 && d.n_xi_dims <= d.d_in)
                       ? tot
                       : (size_t)-1;
   }

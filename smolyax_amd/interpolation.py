@@ -292,7 +292,8 @@ class SmolyakBarycentricInterpolator:
             "int_dim": _dev_tensor(NL.int_dim, i32, dev),
             "int_deg": _dev_tensor(NL.int_deg, i32, dev),
             "level_off": _dev_tensor(NL.level_off, i32, dev),
-            "dim_xi": _dev_tensor(NL.dim_xi, f64, dev),
+            # one spare element so that 16-byte copies of its rows never pass its end
+            "dim_xi": _dev_tensor(np.append(NL.dim_xi.reshape(-1), 0.0), f64, dev),
             "dim_c": _dev_tensor(NL.dim_c, f64, dev),
             "grp": _dev_tensor(NL.grp, i32, dev),
             "grp_next_dim": _dev_tensor(NL.grp_next_dim, i32, dev),
@@ -351,6 +352,7 @@ class SmolyakBarycentricInterpolator:
         if "mt_a" in t:
             d.rw_n = NL.rw_n
             d.mt_n, d.mt_steps, d.mi_n = NL.mt_n, NL.mt_steps, NL.mi_n
+            d.n_xi_dims = NL.n_xi_dims
         for name, ten in t.items():
             setattr(d, name, ten.data_ptr())
         self._ndesc = d

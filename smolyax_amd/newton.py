@@ -346,7 +346,9 @@ def build_mfma_tables(L: NewtonLayout, vecs, n_waves: int = MFMA_WAVES) -> None:
     Sets: mt_n (tiles), mt_grp (16-byte group descriptors [16 * mt_n]: {int32 dim, int32 degree,
     double xi_0 of the dim}), mt_a (FP32 A steps, 64 per step), mt_steps, mi_n (work items),
     mi_rec (int32 [4 * mi_n]: first A step, first step within the tile, step count, tile),
-    mwave_item [n_waves + 1] (item ranges per wave group, balanced by steps + MFMA_ITEM_COST).
+    mwave_item [n_waves + 1] (item ranges per wave group, balanced by steps + MFMA_ITEM_COST),
+    n_xi_dims (1 + the largest dim of a group of degree >= 2: the kernel stages those dims' node
+    rows on chip).
     """
     g4 = L.grp.reshape(-1, 4)
     ng = L.n_groups
@@ -408,6 +410,8 @@ def build_mfma_tables(L: NewtonLayout, vecs, n_waves: int = MFMA_WAVES) -> None:
     items = [items[i] for w in range(n_waves) for i in sorted(owner[w])]
     L.mi_n = len(items)
     L.mi_rec = np.array(items or [(0, 0, 0, 0)], dtype=np.int32).reshape(-1)
+    high = g4[g4[:, 3] >= 2, 2]
+    L.n_xi_dims = int(high.max()) + 1 if len(high) else 0
 
 
 HEAD_ROWS = 64  # rows of the Newton basis that the mixed GEMM path contracts in FP64
