@@ -547,7 +547,8 @@ def main():
                 "bytes_per_launch": bytes_mixed, "avg_launch_ms": results["mixed"]["kernel_ms"]}
     t_b = results["fp64"]["kernel_ms"] * 1e-3
     flops = 2.0 * d_out * L.k_raw * P
-    roof_fp64 = {"bound": "valu", "kernel": "bary_eval_kernel<1>", "achieved": flops / t_b / 1e12,
+    kname_b = "bary_eval_prog_kernel" if ip._prog is not None else "bary_eval_kernel<1>"
+    roof_fp64 = {"bound": "valu", "kernel": kname_b, "achieved": flops / t_b / 1e12,
                  "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                  "frac": flops / t_b / 1e12 / PEAK_FP64_TFLOPS, "avg_launch_ms": results["fp64"]["kernel_ms"],
                  "flop_basis": "2 d_out K_raw per point (the reference's terms)"}

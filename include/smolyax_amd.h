@@ -120,6 +120,22 @@ typedef struct sbi_bary_desc {
   const int32_t* dim_blk_pos;    /*   (0..n-1, n-1 = last factor) */
   const double* ent_q;           /* [k_raw] quadrature weight of every entry: the product of its
                                     factors' slot_q values (integral; optional) */
+  /* Record program of the d_out = 1 eval kernel (optional; pseg NULL: per-block records).  A
+   * record holds 1..3 table rows (one value of each prefix factor, the last one's values r and
+   * r + 1) and the 2 * DL values of F they multiply; a program segment is a run of records of one
+   * streamed slot grouped by their number of rows, then blocks left to the per-block path. */
+  const int32_t* pseg_off;       /* [n_split + 1] per wave: range of its program segments */
+  const int32_t* pseg;           /* [n_pseg * 8] segment headers: slot, first F value, first row,
+                                    n1 | n2 << 16, n3 | DL << 16 | n_gen << 24, first per-block
+                                    block, streamed dim of the wave's next segment (-1: none),
+                                    streamed dim of this segment (n_k = records with k rows;
+                                    DL = 0: none) */
+  const double* pbasis;          /* [n_pseg * 8] nodes[4], first-form weights[4] of each
+                                    segment's streamed slot (DL <= 4) */
+  const int32_t* prow;           /* table rows of the records */
+  const double* pF;              /* F values of the records, 2 * DL per record */
+  int32_t n_pseg;
+  int32_t ones_row;              /* first of two table rows of 1.0 after the prefix rows */
 } sbi_bary_desc;
 
 typedef struct sbi_plan sbi_plan;

@@ -39,6 +39,9 @@ class BaryDesc(ctypes.Structure):
         ("F", c_vp), ("offset", c_vp), ("blk_desc", c_vp), ("seg_off", c_vp), ("seg_slot", c_vp),
         ("seg_b0", c_vp), ("seg_b1", c_vp), ("n_split", c_i32), ("n_seg", c_i32),
         ("dim_blk", c_vp), ("dim_blk_idx", c_vp), ("dim_blk_pos", c_vp), ("ent_q", c_vp),
+        ("pseg_off", c_vp), ("pseg", c_vp), ("pbasis", c_vp), ("prow", c_vp), ("pF", c_vp),
+        ("n_pseg", c_i32),
+        ("ones_row", c_i32),
     ]
 
 

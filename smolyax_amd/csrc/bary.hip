@@ -337,6 +337,309 @@ __global__ __launch_bounds__(kTile* kWaves) void bary_eval_kernel(
 }
 
 // ------------------------------------------------------------------------------------------
+// Record-program evaluation kernel (d_out = 1)
+// ------------------------------------------------------------------------------------------
+// A program segment covers blocks of one streamed slot, so the streamed basis Ll is fixed over
+// it.  Its blocks are cut into records: a record fixes one value of every prefix factor but the
+// last and two consecutive values of the last one, so it reads its table values with one
+// ds_read_b64 per leading factor and one ds_read2st64_b64 for the pair, and multiplies 2 DL
+// consecutive values of F.  The kernel accumulates V[m] += P * F[., m] over the records and adds
+// V . Ll once per segment: one FMA per entry.  Records with the same number of rows are read U
+// at a time, every scalar load of the chunk issued before the first is used, so a memory round
+// trip serves U records (scalar loads complete out of order, so any wait drains them all).
+// Blocks the records do not cover (more than three prefix factors, streamed sizes outside
+// kProgDL) take the per-block path of bary_eval_kernel.
+
+constexpr int kProgHdr = 8;     // int32 per program segment header
+constexpr int kProgBasis = 8;   // doubles per segment: nodes[4], weights[4] of the streamed slot
+constexpr int kProgSgpr = 48;   // SGPRs per record chunk for rows and F values
+constexpr int kProgTab = 8;     // prefix slots per wave whose coordinates are requested together
+constexpr int kXGroup = 4;      // segments per group of streamed coordinates requested together
+
+template <int DL, int NP>
+struct RecChunk {
+  static constexpr int kPer = 4 * DL + NP;  // SGPRs per record
+  static constexpr int U = kProgSgpr / kPer < 1 ? 1 : (kProgSgpr / kPer > 12 ? 12 : kProgSgpr / kPer);
+  static constexpr int U2 = (U + 1) / 2;
+};
+
+// U records from record i on; TAIL: only records < n count (loads clamped to record n - 1).
+template <int DL, int NP, int U, bool TAIL>
+__device__ __forceinline__ void rec_chunk(int i, int n, cptr<int32_t> rows, cptr<double> Fs,
+                                          const lds_f64* tab, int lane, double (&V)[DL]) {
+  constexpr int FS = 2 * DL;
+  int r[U][NP];
+  double f[U][FS];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int k = TAIL ? min(i + u, n - 1) : i + u;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) r[u][q] = rows[k * NP + q];
+#pragma unroll
+    for (int e = 0; e < FS; ++e) f[u][e] = Fs[(int64_t)k * FS + e];
+  }
+  double t[U][NP + 1];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+#pragma unroll
+    for (int q = 0; q + 1 < NP; ++q) t[u][q] = tab[r[u][q] * kTile + lane];
+    t[u][NP - 1] = tab[r[u][NP - 1] * kTile + lane];
+    t[u][NP] = tab[(r[u][NP - 1] + 1) * kTile + lane];
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    if (!TAIL || i + u < n) {
+      double p0 = t[u][NP - 1], p1 = t[u][NP];
+      if constexpr (NP >= 2) {
+        double pre = t[u][0];
+        if constexpr (NP == 3) pre *= t[u][1];
+        p0 *= pre;
+        p1 *= pre;
+      }
+#pragma unroll
+      for (int m = 0; m < DL; ++m) V[m] = fma(p0, f[u][m], V[m]);
+#pragma unroll
+      for (int m = 0; m < DL; ++m) V[m] = fma(p1, f[u][DL + m], V[m]);
+    }
+  }
+}
+
+template <int DL, int NP>
+__device__ __forceinline__ void rec_run(int n, cptr<int32_t>& rows, cptr<double>& Fs,
+                                        const lds_f64* tab, int lane, double (&V)[DL]) {
+  using S = RecChunk<DL, NP>;
+  int i = 0;
+  for (; i + S::U <= n; i += S::U) rec_chunk<DL, NP, S::U, false>(i, n, rows, Fs, tab, lane, V);
+  if (i < n) {
+    if (n - i <= S::U2)
+      rec_chunk<DL, NP, S::U2, true>(i, n, rows, Fs, tab, lane, V);
+    else
+      rec_chunk<DL, NP, S::U, true>(i, n, rows, Fs, tab, lane, V);
+  }
+  rows += n * NP;
+  Fs += (int64_t)n * 2 * DL;
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const int64_t b = __builtin_bit_cast(int64_t, v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, ((int64_t)hi << 32) | (uint32_t)lo);
+}
+
+// Streamed basis of DL entries from the segment's nodes and weights, held in lanes
+// l0 + m (nodes) and l0 + 4 + m (weights) of bv: the arithmetic of basis_regs (same products in
+// the same order), exact one-hot at a node.
+template <int DL>
+__device__ __forceinline__ void basis_small(double x, double bv, int l0, double (&L)[DL]) {
+  double xi[DL], w[DL];
+#pragma unroll
+  for (int m = 0; m < DL; ++m) {
+    xi[m] = readlane_f64(bv, l0 + m);
+    w[m] = readlane_f64(bv, l0 + 4 + m);
+  }
+  double dlt[DL];
+#pragma unroll
+  for (int m = 0; m < DL; ++m) dlt[m] = x - xi[m];
+  double suf[DL];
+  double s = 1.0;
+#pragma unroll
+  for (int m = DL - 1; m >= 0; --m) {
+    suf[m] = s;
+    s *= dlt[m];
+  }
+  double pre = 1.0;
+#pragma unroll
+  for (int m = 0; m < DL; ++m) {
+    double v = w[m] * pre * suf[m];
+    if (dlt[m] == 0.0) v = 1.0;
+    L[m] = v;
+    pre *= dlt[m];
+  }
+}
+
+// The records of one program segment (streamed size DL; counts n1 | n2 << 16 and n3 in the low
+// half of h4): V over the records, then V . Ll with the basis from the segment's own nodes and
+// weights (lanes l0.. of bv).  rows / Fs run on over the wave's segments.
+template <int DL>
+__device__ __forceinline__ double prog_records(uint32_t h3, uint32_t h4, double bv, int l0,
+                                               double x_cur, cptr<int32_t>& rows,
+                                               cptr<double>& Fs, const lds_f64* tab, int lane,
+                                               double acc) {
+  double V[DL];
+#pragma unroll
+  for (int m = 0; m < DL; ++m) V[m] = 0.0;
+  rec_run<DL, 1>((int)(h3 & 0xffff), rows, Fs, tab, lane, V);
+  rec_run<DL, 2>((int)(h3 >> 16), rows, Fs, tab, lane, V);
+  rec_run<DL, 3>((int)(h4 & 0xffff), rows, Fs, tab, lane, V);
+  double Lb[DL];
+  basis_small<DL>(x_cur, bv, l0, Lb);
+#pragma unroll
+  for (int m = 0; m < DL; ++m) acc = fma(V[m], Lb[m], acc);
+  return acc;
+}
+
+__global__ __launch_bounds__(kTile* kWaves) void bary_eval_prog_kernel(
+    sbi_bary_desc d, const double* __restrict__ x, int64_t P, int64_t ldx,
+    double* __restrict__ y) {
+  extern __shared__ double lds_raw[];
+  lds_f64* lds = (lds_f64*)lds_raw;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t p_raw = (int64_t)blockIdx.x * kTile + lane;
+  const bool valid = p_raw < P;
+  const int64_t p = valid ? p_raw : (P - 1);
+  const double* __restrict__ xp = x + p * ldx;
+  cptr<int32_t> slot_dim = C(d.slot_dim), slot_deg = C(d.slot_deg),
+                prefix_slot = C(d.prefix_slot), prefix_off = C(d.prefix_off);
+  cptr<double> slot_nodes = C(d.slot_nodes), slot_w = C(d.slot_w), F = C(d.F);
+  cptr<int32_t> desc = C(d.blk_desc), pseg = C(d.pseg), prow = C(d.prow);
+  cptr<double> pF = C(d.pF);
+
+  const int g_begin = C(d.pseg_off)[wave];
+  const int g_end = C(d.pseg_off)[wave + 1];
+  // Segments are processed in groups of kXGroup.  Per-segment data arrive in vector registers,
+  // lane-distributed (lane l: segment l / 8 of a group, word or double l % 8), through vector
+  // loads, whose in-order counter lets every wait cover exactly what it needs: the headers of
+  // the group after next, the basis nodes and weights of the next group and the next group's
+  // streamed coordinates are requested at the start of a group, so each has a whole group of
+  // segments to arrive.  Every such load is unconditional with a clamped index; a segment without
+  // a streamed factor has dim -1 and reads dim 0, unused.  The wave's record streams start at
+  // its first segment's offsets and run on from there.
+  const int g_last = g_begin < g_end ? g_end - 1 : 0;
+  const int* __restrict__ pseg_v = d.pseg;
+  const double* __restrict__ pbasis_v = d.pbasis;
+  const int lseg = (lane >> 3) & (kXGroup - 1), lword = lane & 7;
+  int hv_cur = pseg_v[min(g_begin + lseg, g_last) * kProgHdr + lword];
+  int hv_next = pseg_v[min(g_begin + kXGroup + lseg, g_last) * kProgHdr + lword];
+  double bv_cur = pbasis_v[min(g_begin + lseg, g_last) * kProgBasis + lword];
+  double xg[kXGroup];
+#pragma unroll
+  for (int k = 0; k < kXGroup; ++k)
+    xg[k] = xp[max(pseg[min(g_begin + k, g_last) * kProgHdr + 7], 0)];
+  cptr<int32_t> rows = prow + pseg[min(g_begin, g_last) * kProgHdr + 2];
+  cptr<double> Fs = pF + pseg[min(g_begin, g_last) * kProgHdr + 1];
+
+  // 1. prefix table (all waves cooperate; each wave requests the coordinates of up to kProgTab
+  //    slots before computing any) and the two rows of ones after it
+  const int nps = d.n_prefix_slots;
+  for (int s0 = wave; s0 < nps; s0 += kProgTab * kWaves) {
+    int sl[kProgTab];
+    double xv[kProgTab];
+#pragma unroll
+    for (int u = 0; u < kProgTab; ++u) sl[u] = prefix_slot[min(s0 + u * kWaves, nps - 1)];
+#pragma unroll
+    for (int u = 0; u < kProgTab; ++u) xv[u] = xp[slot_dim[sl[u]]];
+#pragma unroll
+    for (int u = 0; u < kProgTab; ++u) {
+      const int s = s0 + u * kWaves;
+      if (s < nps)
+        basis_lds(xv[u], slot_nodes + (int64_t)sl[u] * d.dmax1, slot_w + (int64_t)sl[u] * d.dmax1,
+                  slot_deg[sl[u]], lds, prefix_off[s], lane);
+    }
+  }
+  if (wave == kWaves - 1) {
+    lds[d.ones_row * kTile + lane] = 1.0;
+    lds[(d.ones_row + 1) * kTile + lane] = 1.0;
+  }
+  __syncthreads();
+
+  double acc = 0.0;
+  for (int g0 = g_begin; g0 < g_end; g0 += kXGroup) {
+    const int hv_next2 = pseg_v[min(g0 + 2 * kXGroup + lseg, g_last) * kProgHdr + lword];
+    const double bv_next = pbasis_v[min(g0 + kXGroup + lseg, g_last) * kProgBasis + lword];
+    double xn[kXGroup];
+#pragma unroll
+    for (int k = 0; k < kXGroup; ++k)
+      xn[k] = xp[max(__builtin_amdgcn_readlane(hv_next, 8 * k + 7), 0)];
+    const int g1 = min(g0 + kXGroup, g_end);
+    if (__builtin_amdgcn_readlane(hv_cur, 6) & 1) {
+      // a group of kXGroup segments with one record each (DL = 2, one prefix row pair, nothing
+      // on the per-block path): their records are consecutive, so one chunk serves the group;
+      // the arithmetic is that of the per-segment path
+      int r[kXGroup];
+      double f[kXGroup][4];
+#pragma unroll
+      for (int k = 0; k < kXGroup; ++k) {
+        r[k] = rows[k];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) f[k][e] = Fs[4 * k + e];
+      }
+      double t0[kXGroup], t1[kXGroup];
+#pragma unroll
+      for (int k = 0; k < kXGroup; ++k) {
+        t0[k] = lds[r[k] * kTile + lane];
+        t1[k] = lds[(r[k] + 1) * kTile + lane];
+      }
+#pragma unroll
+      for (int k = 0; k < kXGroup; ++k) {
+        double V[2];
+#pragma unroll
+        for (int m = 0; m < 2; ++m) V[m] = fma(t0[k], f[k][m], 0.0);
+#pragma unroll
+        for (int m = 0; m < 2; ++m) V[m] = fma(t1[k], f[k][2 + m], V[m]);
+        double Lb[2];
+        basis_small<2>(xg[k], bv_cur, 8 * k, Lb);
+#pragma unroll
+        for (int m = 0; m < 2; ++m) acc = fma(V[m], Lb[m], acc);
+      }
+      rows += kXGroup;
+      Fs += 4 * kXGroup;
+    } else {
+    for (int g = g0; g < g1; ++g) {
+      const int k = g - g0;
+      double x_cur = xg[0];
+#pragma unroll
+      for (int q = 1; q < kXGroup; ++q) x_cur = (k == q) ? xg[q] : x_cur;
+      const uint32_t h3 = (uint32_t)__builtin_amdgcn_readlane(hv_cur, 8 * k + 3);
+      const uint32_t h4 = (uint32_t)__builtin_amdgcn_readlane(hv_cur, 8 * k + 4);
+      switch ((h4 >> 16) & 0xff) {
+        case 2: acc = prog_records<2>(h3, h4, bv_cur, 8 * k, x_cur, rows, Fs, lds, lane, acc); break;
+        case 3: acc = prog_records<3>(h3, h4, bv_cur, 8 * k, x_cur, rows, Fs, lds, lane, acc); break;
+        case 4: acc = prog_records<4>(h3, h4, bv_cur, 8 * k, x_cur, rows, Fs, lds, lane, acc); break;
+        default: break;
+      }
+      // blocks on the per-block path (the streamed basis again in the general form: same values)
+      const int n_gen = (int)(h4 >> 24);
+      if (n_gen > 0) {
+        const int ls = __builtin_amdgcn_readlane(hv_cur, 8 * k);
+        const int gb0 = __builtin_amdgcn_readlane(hv_cur, 8 * k + 5);
+        double Ll[kLMax];
+        if (ls >= 0) {
+          basis_regs(x_cur, slot_nodes + (int64_t)ls * d.dmax1, slot_w + (int64_t)ls * d.dmax1,
+                     slot_deg[ls], Ll);
+        } else {
+#pragma unroll
+          for (int m = 0; m < kLMax; ++m) Ll[m] = (m == 0) ? 1.0 : 0.0;
+        }
+        double a1[1] = {acc};
+        for (int b = gb0; b < gb0 + n_gen; ++b)
+          contract_dispatch<1>(d, b, F + desc[4 * b], 1, (uint32_t)desc[4 * b + 1],
+                               (uint32_t)desc[4 * b + 2], (uint32_t)desc[4 * b + 3], Ll, lds, lane,
+                               a1);
+        acc = a1[0];
+      }
+    }
+    }
+#pragma unroll
+    for (int k = 0; k < kXGroup; ++k) xg[k] = xn[k];
+    hv_cur = hv_next;
+    hv_next = hv_next2;
+    bv_cur = bv_next;
+  }
+
+  // 2. deterministic cross-wave reduction (table area is reused)
+  __syncthreads();
+  lds[wave * kTile + lane] = acc;
+  __syncthreads();
+  if (wave == 0 && valid) {
+    double s = d.offset[0];
+    for (int w = 0; w < kWaves; ++w) s += lds[w * kTile + lane];
+    y[p] = s;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
 // Jacobian kernel (parity path of `gradient`): one wave per (tile, dim j); loops over the blocks
 // that contain j; every (p, o, j) entry is written by exactly one lane.
 // ------------------------------------------------------------------------------------------
@@ -545,6 +848,7 @@ struct sbi_plan {
   int kind;  // 1 = barycentric
   sbi_bary_desc bary;
   size_t eval_lds_bytes;
+  size_t prog_lds_bytes;  // record-program kernel (0: no program)
 };
 
 extern "C" {
@@ -577,10 +881,27 @@ int sbi_bary_plan_create(const sbi_bary_desc* desc, sbi_plan** out) {
   const size_t lds = table > red ? table : red;
   // a plan whose prefix table exceeds LDS is still valid for the basis / jacobian / integral
   // entry points; sbi_bary_eval_f64 then reports SBI_ERR_CAPACITY
+  size_t prog_lds = 0;
+  if (d.pseg) {
+    if (d.d_out != 1 || !d.pseg_off || !d.prow || !d.pF || !d.pbasis || d.n_pseg <= 0 ||
+        d.ones_row != d.n_prefix_vals) {
+      sbi::set_error("sbi_bary_plan_create: invalid record program (d_out must be 1 and the rows "
+                     "of ones follow the prefix rows)");
+      return SBI_ERR_INVALID;
+    }
+    const size_t ptab = (size_t)(d.ones_row + 2) * kTile * sizeof(double);
+    const size_t pred = (size_t)kWaves * kTile * sizeof(double);
+    prog_lds = ptab > pred ? ptab : pred;
+    if (prog_lds > (size_t)kMaxLds) {
+      sbi::set_error("sbi_bary_plan_create: record program table exceeds LDS (318 prefix rows)");
+      return SBI_ERR_CAPACITY;
+    }
+  }
   sbi_plan* p = new sbi_plan();
   p->kind = 1;
   p->bary = d;
   p->eval_lds_bytes = lds;
+  p->prog_lds_bytes = prog_lds;
   *out = p;
   return SBI_OK;
 }
@@ -627,6 +948,14 @@ extern "C" int sbi_bary_eval_f64(const sbi_plan* plan, const double* x, int64_t 
   if (P == 0) return SBI_OK;
   hipStream_t st = (hipStream_t)stream;
   const int64_t dout = plan->bary.d_out;
+  if (plan->prog_lds_bytes) {  // record program (d_out == 1)
+    const sbi_bary_desc& d = plan->bary;
+    sbi::allow_lds((const void*)bary_eval_prog_kernel);
+    hipLaunchKernelGGL(bary_eval_prog_kernel, dim3((unsigned)((P + kTile - 1) / kTile)),
+                       dim3(kTile * kWaves), plan->prog_lds_bytes, st, d, x, P, ldx, y);
+    SBI_HIP_CHECK(hipGetLastError());
+    return SBI_OK;
+  }
   if (dout == 1) return launch_eval<1>(plan, x, P, ldx, y, st);  // unit-stride F rows
   if (dout == 2) return launch_eval<2>(plan, x, P, ldx, y, st);
   return launch_eval<4>(plan, x, P, ldx, y, st);

@@ -98,6 +98,7 @@ class SmolyakBarycentricInterpolator:
         self._newton_fp64_ok = False
         self._warned_mixed = False
         self._layout = None
+        self._prog = None           # record program of the d_out = 1 FP64 eval kernel
         if f is not None:
             self.set_f(f=f)
 
@@ -116,6 +117,7 @@ class SmolyakBarycentricInterpolator:
         self._n_f_evals_new += n_new
         L.slot_q = packing.quadrature_table(L, self._node_gen)
         self._layout = L
+        self._prog = None
         self._F_host = F
         self._offset_host = offset
         self._nlayout = None
@@ -258,6 +260,15 @@ class SmolyakBarycentricInterpolator:
         t["dim_blk"] = _dev_tensor(L.dim_blk, i32, dev)
         t["dim_blk_idx"] = _dev_tensor(L.dim_blk_idx, i32, dev)
         t["dim_blk_pos"] = _dev_tensor(L.dim_blk_pos, i32, dev)
+        # record program of the d_out = 1 eval kernel
+        prog = packing.build_eval_program(L) if self._d_out == 1 and self._prog is None else self._prog
+        self._prog = prog
+        if prog is not None:
+            t["pseg_off"] = _dev_tensor(prog.seg_off, i32, dev)
+            t["pseg"] = _dev_tensor(prog.seg, i32, dev)
+            t["pbasis"] = _dev_tensor(prog.basis, f64, dev)
+            t["prow"] = _dev_tensor(prog.rows, i32, dev)
+            t["pF"] = _dev_tensor(packing.program_values(prog, self._F_host), f64, dev)
         self._dev = t
         d = N.BaryDesc()
         d.d_in, d.d_out = self._d_in, self._d_out
@@ -273,6 +284,9 @@ class SmolyakBarycentricInterpolator:
             setattr(d, name, ten.data_ptr())
         d.n_split = packing.N_WAVES
         d.n_seg = int(L.seg_off[-1])
+        if prog is not None:
+            d.n_pseg = int(prog.seg_off[-1])
+            d.ones_row = prog.ones_row
         self._desc = d
         if self._plan is not None:
             N.fn["sbi_plan_destroy"](self._plan)

@@ -26,6 +26,32 @@ N_WAVES = 16         # waves per point tile (kWaves in bary.hip)
 # with 490 runs and twice the run time.
 BLOCK_COST = 4.0
 SEG_COST = 35.0
+# Record program of the d_out = 1 eval kernel: streamed sizes (deg + 1) with record code
+# (kProgDL in bary.hip), the record types in program order (prefix factors per record), the
+# header size of a program segment in int32, and the table rows the kernel holds in LDS.
+PROG_DL = (2, 3, 4)
+PROG_MAX_PRE = 3
+PROG_TYPES = (1, 2, 3)
+PROG_HDR = 8
+PROG_BASIS = 8           # doubles per segment: nodes and weights of its streamed slot (DL <= 4)
+PROG_GROUP = 4           # segments per group of the eval kernel (kXGroup in bary.hip)
+PROG_MAX_GEN = 255       # per-block (generic) blocks per program segment
+PROG_MAX_COUNT = 65535   # records of one type per program segment
+PROG_MAX_ROWS = 320      # prefix table rows + 2 rows of ones, 64 points x 8 B each, in 160 KiB
+PROG_SGPR = 48           # SGPRs per record chunk (kProgSgpr in bary.hip)
+# Wave split of the program: per-wave cost in cycles as measured on MI355X at d_in = 1000 (one
+# scalar memory round trip per record chunk): a full chunk, a partial chunk, a segment start
+# beyond its chunks, a block on the per-block path per entry (plus a segment-like start).
+PROG_CYC_FULL = 790.0
+PROG_CYC_TAIL = 690.0
+PROG_CYC_SEG = 1390.0
+PROG_CYC_GEN_ENTRY = 100.0
+PROG_CYC_GROUP1 = 2400.0   # a group of PROG_GROUP one-record segments (one chunk for all)
+
+
+def prog_chunk(dl: int, npre: int) -> int:
+    """Records per chunk of the eval kernel for a streamed size and record type."""
+    return max(1, min(12, PROG_SGPR // (4 * dl + npre)))
 
 
 @dataclass
@@ -159,8 +185,12 @@ def build_layout(blocks: Blocks, node_gen, d_in: int, d_out: int) -> Layout:
 
     # device order: runs of equal streamed slot, streamed dims ascending (no-stream blocks first)
     def run_key(b):
+        # within a run, blocks with more than PROG_MAX_PRE prefix factors come last: the record
+        # program of the eval kernel leaves them to its per-block path, which takes a segment's
+        # trailing blocks
         s = last_slot[b]
-        return (-1, -1) if s < 0 else (int(L.slot_dim[s]), int(L.slot_deg[s]))
+        many = len(pre_slots[b]) > PROG_MAX_PRE
+        return (-1, -1, many) if s < 0 else (int(L.slot_dim[s]), int(L.slot_deg[s]), many)
 
     order = sorted(range(nb), key=run_key)
     L.order = np.array(order, dtype=np.int64)
@@ -358,3 +388,227 @@ def assemble_values(L: Layout, blocks: Blocks, node_gen, f, f_evals, vectorized=
     else:
         offset = np.zeros(d_out)
     return F, np.ascontiguousarray(offset, dtype=np.float64), n_new
+
+
+@dataclass
+class EvalProgram:
+    """Record program of the d_out = 1 barycentric eval kernel (see build_eval_program)."""
+    seg_off: np.ndarray     # [N_WAVES + 1] per wave: range of its program segments
+    seg: np.ndarray         # [n_seg * PROG_HDR] int32 segment headers
+    basis: np.ndarray       # [n_seg * PROG_BASIS] nodes and weights of each segment's streamed slot
+    rows: np.ndarray        # int32 table rows, record order
+    f_index: np.ndarray     # int64: position in the flat device F of every program F value (-1: 0)
+    n_records: int
+    ones_row: int           # first of the two table rows that hold 1.0
+
+
+def _block_records(L: Layout, pos: int, dl: int, ones: int):
+    """Records of one block with at most PROG_MAX_PRE prefix factors: yields (prefix factors,
+    rows, F positions).  A record fixes every prefix factor but the last and two consecutive values
+    c0, c0 + 1 of the last one (rows r + c0 and r + c0 + 1 of the table); the block's F is C order
+    over (prefix factors..., streamed factor), so the record's 2 DL values are contiguous.  An odd
+    last value gets zeros as its second half (the row after it is still a table row: the next
+    slot's, or a row of ones).  A block without prefix factors is one record on the rows of ones
+    with zeros as its second half."""
+    f0, f1 = int(L.blk_fac[pos]), int(L.blk_fac[pos + 1])
+    npre = f1 - f0
+    base = int(L.blk_foff[pos])
+    zeros = [-1] * dl
+    if npre == 0:
+        yield 1, [ones], list(range(base, base + dl)) + zeros
+        return
+    rows = [int(r) for r in L.fac_row[f0:f1]]
+    degs = [int(g) for g in L.fac_deg[f0:f1]]
+    strides = [0] * npre
+    st = dl
+    for i in range(npre - 1, -1, -1):
+        strides[i] = st
+        st *= degs[i] + 1
+    for outer in it.product(*[range(g + 1) for g in degs[:-1]]):
+        o_off = base + sum(m * strides[i] for i, m in enumerate(outer))
+        o_rows = [rows[i] + m for i, m in enumerate(outer)]
+        for c0 in range(0, degs[-1] + 1, 2):
+            a = o_off + c0 * dl
+            second = list(range(a + dl, a + 2 * dl)) if c0 + 1 <= degs[-1] else zeros
+            yield npre, o_rows + [rows[-1] + c0], list(range(a, a + dl)) + second
+
+
+def build_eval_program(L: Layout):
+    """Record program of the d_out = 1 eval kernel; None when the table with its rows of ones
+    exceeds PROG_MAX_ROWS.
+
+    The blocks (device order) are split over the N_WAVES waves of a tile by cost (PROG_* costs);
+    a program segment is the part of one run (one streamed slot) inside one wave, cut further to
+    the header's count limits.  Its records come grouped by their number of prefix factors
+    (PROG_TYPES order), then (per-block path) its trailing blocks with more than PROG_MAX_PRE
+    prefix factors, or all of its blocks when the streamed size is not in PROG_DL or there is no
+    streamed factor.  Header (int32):
+      [0] streamed slot (-1: none)   [1] first F value (program F)   [2] first row
+      [3] n(1) | n(2) << 16          [4] n(3) | DL << 16 | n_gen << 24 (DL = 0: no records)
+      [5] first per-block block      [6] 1 on the first segment of a group of PROG_GROUP
+      segments (counted from the wave's first) that have one record each, DL = 2 and nothing
+      on the per-block path, else 0   [7] streamed dim of this segment (-1: none)
+    Basis (double, PROG_BASIS per segment): nodes then first-form weights of the streamed slot.
+    """
+    ones = int(L.n_prefix_vals)
+    if ones + 2 > PROG_MAX_ROWS:
+        return None
+    nt = len(PROG_TYPES)
+    nb = len(L.blk_n)
+    recs_of = []
+    cnt = np.zeros((nb, nt), dtype=np.int64)      # records of each type per block
+    gen_e = np.zeros(nb)                          # entries of a per-block-path block
+    run_of = np.zeros(nb, dtype=np.int64)
+    for r in range(len(L.run_start) - 1):
+        run_of[int(L.run_start[r]):int(L.run_start[r + 1])] = r
+    for pos in range(nb):
+        ls = int(L.blk_last_slot[pos])
+        dl = int(L.slot_deg[ls]) + 1 if ls >= 0 else 0
+        npre = int(L.blk_fac[pos + 1] - L.blk_fac[pos])
+        if dl in PROG_DL and npre <= PROG_MAX_PRE:
+            r = list(_block_records(L, pos, dl, ones))
+            for tp, _, _ in r:
+                cnt[pos, PROG_TYPES.index(tp)] += 1
+        else:
+            r = None
+            gen_e[pos] = float(L.blk_foff[pos + 1] - L.blk_foff[pos])
+        recs_of.append(r)
+    units = []
+    for pos in range(nb):
+        ls = int(L.blk_last_slot[pos])
+        dl = int(L.slot_deg[ls]) + 1 if ls >= 0 else 0
+        units.append([prog_chunk(dl, tp) if dl in PROG_DL else 1 for tp in PROG_TYPES])
+
+    def seg_cost(c, u, ge):
+        cyc = PROG_CYC_SEG + ge * PROG_CYC_GEN_ENTRY
+        for i in range(nt):
+            if c[i]:
+                cyc += (c[i] // u[i]) * PROG_CYC_FULL + (PROG_CYC_TAIL if c[i] % u[i] else 0.0)
+        return cyc
+
+    dl_of = [int(L.slot_deg[int(L.blk_last_slot[pos])]) + 1 if L.blk_last_slot[pos] >= 0 else 0
+             for pos in range(nb)]
+
+    def group_cost(grp):
+        if len(grp) == PROG_GROUP and all(one for _, one in grp):
+            return PROG_CYC_GROUP1
+        return sum(cyc for cyc, _ in grp)
+
+    def split(bound):
+        """Greedy cut of the block sequence into waves of cost <= bound (segments in groups of
+        PROG_GROUP from each wave's first, as the kernel walks them); the cut list or None."""
+        cuts = [0]
+        done, grp = 0.0, []                # cost of the wave's complete groups; the open group
+        seg_c, seg_ge, run = [0] * nt, 0.0, -1
+        for pos in range(nb):
+            new_seg = run_of[pos] != run
+            c = [int(v) for v in cnt[pos]] if new_seg else [seg_c[i] + int(cnt[pos, i])
+                                                             for i in range(nt)]
+            ge = (0.0 if new_seg else seg_ge) + gen_e[pos]
+            info = (seg_cost(c, units[pos], ge), dl_of[pos] == 2 and c == [1, 0, 0] and ge == 0)
+            if new_seg:
+                d2, g2 = (done + group_cost(grp), [info]) if len(grp) == PROG_GROUP else \
+                    (done, grp + [info])
+            else:
+                d2, g2 = done, grp[:-1] + [info]
+            if d2 + group_cost(g2) > bound and pos > cuts[-1]:
+                cuts.append(pos)
+                if len(cuts) > N_WAVES:
+                    return None
+                c, ge = [int(v) for v in cnt[pos]], gen_e[pos]
+                info = (seg_cost(c, units[pos], ge), dl_of[pos] == 2 and c == [1, 0, 0] and ge == 0)
+                d2, g2 = 0.0, [info]
+            done, grp = d2, g2
+            seg_c, seg_ge, run = c, ge, run_of[pos]
+        return cuts + [nb] * (N_WAVES + 1 - len(cuts))
+
+    lo, hi = 0.0, 1.0
+    while split(hi) is None:
+        hi *= 2.0
+    for _ in range(30):
+        mid = 0.5 * (lo + hi)
+        if split(mid) is None:
+            lo = mid
+        else:
+            hi = mid
+    cuts = np.array(split(hi))
+    seg_off, hdrs, basis, rows, fidx = [0], [], [], [], []
+    nrec = 0
+    for w in range(N_WAVES):
+        w_hdrs = []
+        b, b_end = int(cuts[w]), int(cuts[w + 1])
+        while b < b_end:
+            r = int(np.searchsorted(L.run_start, b, side="right")) - 1
+            e = min(int(L.run_start[r + 1]), b_end)
+            ls = int(L.blk_last_slot[b])
+            dl = int(L.slot_deg[ls]) + 1 if ls >= 0 else 0
+            rec_ok = dl in PROG_DL
+            # cut the run's part into pieces within the header's count limits
+            pieces, cur, cnt = [], [], [0] * (nt + 1)
+            for pos in range(b, e):
+                recs = recs_of[pos]
+                add = [0] * nt
+                for tp, _, _ in recs or []:
+                    add[PROG_TYPES.index(tp)] += 1
+                gen_here = recs is None
+                over = any(cnt[i] + add[i] > PROG_MAX_COUNT for i in range(nt)) or \
+                    (gen_here and cnt[nt] + 1 > PROG_MAX_GEN) or (not gen_here and cnt[nt] > 0)
+                if over and cur:
+                    pieces.append(cur)
+                    cur, cnt = [], [0] * (nt + 1)
+                cur.append((pos, recs))
+                for i in range(nt):
+                    cnt[i] += add[i]
+                cnt[nt] += int(gen_here)
+            pieces.append(cur)
+            for piece in pieces:
+                by_type = [[] for _ in range(nt)]
+                gen = []
+                for pos, recs in piece:
+                    if recs is None:
+                        gen.append(pos)
+                    else:
+                        for tp, rr, ff in recs:
+                            by_type[PROG_TYPES.index(tp)].append((rr, ff))
+                assert gen == list(range(gen[0], gen[0] + len(gen))) if gen else True
+                if gen:
+                    assert gen[-1] == piece[-1][0], "per-block blocks must end the segment"
+                f_off, r_off = len(fidx), len(rows)
+                for recs in by_type:
+                    for rr, ff in recs:
+                        rows.extend(rr)
+                        fidx.extend(ff)
+                    nrec += len(recs)
+                n = [len(rr) for rr in by_type]
+                gen_b0 = gen[0] if gen else piece[-1][0] + 1
+                dim = int(L.slot_dim[ls]) if ls >= 0 else -1
+                w_hdrs.append([ls, f_off, r_off, n[0] | n[1] << 16,
+                               n[2] | (dl if rec_ok else 0) << 16 | len(gen) << 24, gen_b0, 0,
+                               dim])
+                bv = np.zeros(PROG_BASIS)
+                if rec_ok:
+                    bv[:dl] = L.slot_nodes[ls, :dl]
+                    bv[4:4 + dl] = L.slot_w[ls, :dl]
+                basis.append(bv)
+            b = e
+        # groups of PROG_GROUP segments with one record each (DL = 2, one row pair, nothing on
+        # the per-block path): flag 1 in word 6 of the group's first header
+        for i0 in range(0, len(w_hdrs) - PROG_GROUP + 1, PROG_GROUP):
+            if all(h[3] == 1 and (h[4] & 0xffff) == 0 and (h[4] >> 16) == 2
+                   for h in w_hdrs[i0:i0 + PROG_GROUP]):
+                w_hdrs[i0][6] = 1
+        hdrs.extend(w_hdrs)
+        seg_off.append(len(hdrs))
+    assert len(fidx) < 2**31 and len(rows) < 2**31
+    return EvalProgram(seg_off=np.array(seg_off, dtype=np.int32),
+                       seg=np.array(hdrs, dtype=np.int64).astype(np.uint32).view(np.int32).reshape(-1)
+                       if hdrs else np.zeros(PROG_HDR, dtype=np.int32),
+                       basis=np.array(basis).reshape(-1) if basis else np.zeros(PROG_BASIS),
+                       rows=np.array(rows or [0], dtype=np.int32),
+                       f_index=np.array(fidx or [0], dtype=np.int64), n_records=nrec, ones_row=ones)
+
+
+def program_values(prog: EvalProgram, F: np.ndarray) -> np.ndarray:
+    """The program's F stream: the device F (d_out = 1) gathered in record order, zeros at -1."""
+    Ff = np.append(np.asarray(F, dtype=np.float64).reshape(-1), 0.0)
+    return Ff[prog.f_index]

@@ -318,3 +318,61 @@ def test_mixed_without_newton_form_warns(gpu):
     for prec in ("newton-fp64", "bf16"):
         with pytest.raises(ValueError):
             ip.evaluate(x, prec)
+
+
+@pytest.mark.parametrize("d,m,kind,kscale", [(5, 300, "gh", 1.0), (8, 3000, "leja", 0.3),
+                                             (2, 40, "leja", 1.0), (12, 600, "leja", 0.2),
+                                             (1, 22, "gh", 1.0), (6, 200, "leja", 1.0)])
+def test_eval_record_program(gpu, d, m, kind, kscale):
+    """d_out = 1 FP64 evaluation runs the record-program kernel: against the oracle, and against
+    the per-block kernel (the same plan without its program).  The index sets cover blocks with
+    four or more active dims and streamed sizes outside 2..4 (the kernel's per-block path), odd
+    prefix degrees (records padded with zeros), points on grid nodes and a partial last tile."""
+    import ctypes
+
+    import torch
+
+    from smolyax_amd import _native as N
+    from smolyax_amd import nodes as SN
+    from smolyax_amd.interpolation import SmolyakBarycentricInterpolator
+
+    rng = np.random.default_rng(d + m)
+    k = kscale * np.log((2 + np.arange(d)) / np.log(2))
+    ong, sng = (ON.leja(dim=d), SN.Leja(dim=d)) if kind == "leja" else \
+        (ON.gauss_hermite(dim=d), SN.GaussHermite(dim=d))
+    t = OI.find_approximate_threshold(k, m, ong.is_nested)
+    A = rng.standard_normal((1, d)) / (np.arange(d) + 1.0)
+    f = lambda x: np.cos(A @ x + 0.2) + 0.1 * (A @ x) ** 2
+    ref = OS.Interpolant(d_out=1, node_gen=ong, k=k, t=t, f=f)
+    ip = SmolyakBarycentricInterpolator(d_out=1, node_gen=sng, k=k, t=t, f=f)
+    assert ip._prog is not None and ip._desc.n_pseg > 0
+    x = sample(ong, 203, seed=m)
+    x[:3, 0] = ong[0](4)[:3]                 # points on nodes of dim 0
+    print(f"record program {kind} d={d} m={m}: {ip._prog.n_records} records", flush=True)
+    y = ip.evaluate(x, "fp64").cpu().numpy()
+    # against the exact interpolant of the FP64 data (the oracle in extended precision): the
+    # larger index sets carry sum |zeta| |F| of 1e2..1e3 max |y|, so every FP64 evaluation of the
+    # combination technique, the oracle's included, rounds at 1e-13..1e-12 there
+    y_ex = ref(x, dtype=np.longdouble).astype(np.float64)
+    e_ora = rel(ref(x), y_ex)
+    e_prog = rel(y, y_ex)
+    print(f"  record program {e_prog:.2e}, FP64 oracle {e_ora:.2e} from the exact values",
+          flush=True)
+    assert e_prog <= max(4.0 * e_ora, 1e-14)
+    # the per-block kernel on the same device tables
+    dA = N.BaryDesc()
+    ctypes.pointer(dA)[0] = ip._desc
+    dA.pseg = None
+    plan = ctypes.c_void_p()
+    N.check(N.fn["sbi_bary_plan_create"](ctypes.byref(dA), ctypes.byref(plan)), "plan")
+    try:
+        xt = torch.as_tensor(x, device="cuda")
+        y0 = torch.empty((len(x), 1), dtype=torch.float64, device="cuda")
+        N.check(N.fn["sbi_bary_eval_f64"](plan, xt.data_ptr(), len(x), d, y0.data_ptr(), None),
+                "eval")
+        torch.cuda.synchronize()
+    finally:
+        N.fn["sbi_plan_destroy"](plan)
+    e_blk = rel(y0.cpu().numpy(), y_ex)
+    print(f"  per-block kernel {e_blk:.2e}", flush=True)
+    assert e_blk <= max(4.0 * e_ora, 1e-14)

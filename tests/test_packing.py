@@ -108,3 +108,55 @@ def test_f_evals_cross_compatible_with_oracle(kind):
         else:
             assert set(a) == set(b)
             assert all(np.array_equal(np.asarray(a[q]), np.asarray(b[q])) for q in a)
+
+
+@pytest.mark.parametrize("kind,d,m,kscale", [("leja", 6, 200, 1.0), ("leja", 8, 3000, 0.3),
+                                             ("gh", 5, 300, 1.0), ("leja", 2, 40, 1.0),
+                                             ("leja", 12, 600, 0.2)])
+def test_eval_program_matches_layout(kind, d, m, kscale):
+    """The record program of the d_out = 1 eval kernel (records of up to three prefix factors,
+    pairs of consecutive table rows, per-block path for the rest) uses every F value exactly once
+    and evaluates the same interpolant as the block-by-block layout.  The index sets include
+    blocks with four or more active dims (per-block path), streamed sizes outside 2..4 (whole
+    segments on the per-block path) and odd prefix degrees (single records)."""
+    from smolyax_amd import nodes as SN
+    from smolyax_amd import packing
+
+    rng = np.random.default_rng(d + m)
+    k = kscale * np.log((2 + np.arange(d)) / np.log(2))
+    sng = SN.Leja(dim=d) if kind == "leja" else SN.GaussHermite(dim=d)
+    t = OI.find_approximate_threshold(k, m, sng.is_nested)
+    blocks = packing.nonzero_blocks(k, t)
+    L = packing.build_layout(blocks, sng, d, 1)
+    A = rng.standard_normal((1, d))
+    F, off, _ = packing.assemble_values(L, blocks, sng, lambda x: np.sin(A @ x + 0.3), {})
+    prog = packing.build_eval_program(L)
+    x = rng.uniform(-0.9, 0.9, (30, d)) * (1.0 if kind == "leja" else 2.0)
+    x[0, 0] = L.slot_nodes[0, 0] if L.slot_dim[0] == 0 else x[0, 0]   # a point on a node
+    y_ref = layout_eval.eval_layout(L, F, off, x)[:, 0]
+    y, used = layout_eval.eval_program(L, prog, F, off, x)
+    assert np.all(used == 1)
+    assert np.max(np.abs(y - y_ref)) <= 1e-12 * np.max(np.abs(y_ref))
+    hdr = prog.seg.reshape(-1, packing.PROG_HDR)
+    # per wave: each header names the next segment's streamed dim; basis rows are the slot's
+    for w in range(packing.N_WAVES):
+        g0, g1 = prog.seg_off[w], prog.seg_off[w + 1]
+        for g in range(g0, g1):
+            ls = hdr[g, 0]
+            assert hdr[g, 7] == (L.slot_dim[ls] if ls >= 0 else -1)
+            if (g - g0) % packing.PROG_GROUP == 0:
+                grp = hdr[g:g + packing.PROG_GROUP]
+                simple = len(grp) == packing.PROG_GROUP and all(
+                    h[3] == 1 and (h[4] & 0xffff) == 0 and (h[4] >> 16) == 2 for h in grp)
+                assert hdr[g, 6] == int(simple)
+            else:
+                assert hdr[g, 6] == 0
+            dl = (hdr[g, 4].view(np.uint32) >> 16) & 0xff
+            bv = prog.basis.reshape(-1, packing.PROG_BASIS)[g]
+            if dl:
+                assert np.array_equal(bv[:dl], L.slot_nodes[ls, :dl])
+                assert np.array_equal(bv[4:4 + dl], L.slot_w[ls, :dl])
+    n_gen = (hdr[:, 4].view(np.uint32) >> 24).sum()
+    npre = np.diff(L.blk_fac)
+    print(f"{kind} d={d}: {prog.n_records} records, {n_gen} per-block blocks "
+          f"(of {len(L.blk_n)}; max prefix factors {npre.max()})")
