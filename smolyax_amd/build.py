@@ -53,6 +53,34 @@ def build(verbose=True):
     return OUT
 
 
+# Host-side check of the C ABI under AddressSanitizer and UndefinedBehaviorSanitizer (CPU only).
+# The sanitizer flags apply to the host compilation only (each follows -Xarch_host); the device
+# code of the .hip sources is built as usual and never runs in the check.
+SANITIZE = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined"]
+CHECK_SOURCES = [("indices.cpp", True), ("host_check.cpp", True), ("bary.hip", False),
+                 ("newton.hip", False)]
+
+
+def build_host_check(verbose=True):
+    """Build smolyax_amd/_build/asan/host_check (indices.cpp, the plan entry points of bary.hip
+    and newton.hip, and the driver csrc/host_check.cpp) with ASan + UBSan on the host side."""
+    objdir = os.path.join(HERE, "_build", "asan")
+    os.makedirs(objdir, exist_ok=True)
+    base = ["-O1", "-g", "-fno-omit-frame-pointer", "-std=c++17", "-Wall", "-Wno-unused-function"]
+    objs = []
+    for src, host_only in CHECK_SOURCES:
+        obj = os.path.join(objdir, src + ".o")
+        if host_only:
+            cmd = [HIPCC, "-x", "c++", "-c", *base, "-ffp-contract=off", *SANITIZE]
+        else:
+            cmd = [HIPCC, "-c", *base, *HIP_FLAGS, *SANITIZE]
+        _run(cmd + [os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    exe = os.path.join(objdir, "host_check")
+    _run([HIPCC, f"--offload-arch={ARCH}", *SANITIZE, *objs, "-o", exe])
+    return exe
+
+
 if __name__ == "__main__":
     build()
     sys.exit(0)
