@@ -20,10 +20,11 @@
 // clamped loads (rows beyond P or n are computed and discarded) and, in the last chunk, by zeroing
 // terms with k >= K.
 //
-// Accumulation: FP64 operands use the FP64 MFMA (v_mfma_f64_16x16x4).  FP32 operands use the FP32
-// MFMA (v_mfma_f32_16x16x4) into FP32 partial sums that are added into FP64 accumulators (in LDS) every
-// kFlushK terms of k, so the FP32 accumulation error stays that of a 256-term sum and the result's
-// error is dominated by the FP32 rounding of the operands.
+// Accumulation: FP64 operands use the FP64 MFMA (v_mfma_f64_16x16x4), FP32 operands the FP32 MFMA
+// (v_mfma_f32_16x16x4); either way the MFMA accumulators hold partial sums of kFlushK terms of k
+// that are added into FP64 accumulators (in LDS).  FP32: the accumulation error stays that of a
+// 256-term sum and the result's error is dominated by the FP32 rounding of the operands.  FP64:
+// the rounding of a K-term sum grows with kFlushK + K / kFlushK instead of K.
 
 #include <hip/hip_runtime.h>
 
@@ -87,21 +88,20 @@ __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
   for (int fn = 0; fn < FN; ++fn) jcol[fn] = min(n_base + fn * 16 + li, n - 1);
 
   AccT acc[2][FN];
-  // FP64 sums of the FP32 partial tiles: in LDS (one column per thread, no sharing), which keeps
-  // the register count low enough for two waves per SIMD
-  __shared__ double acc64s[kF32 ? 2 * FN * 4 : 1][64 * kWaves];
+  // FP64 sums of the partial tiles of kFlushK terms (FP32 or FP64 MFMA accumulators): in LDS (one
+  // column per thread, no sharing), which keeps the register count low enough for two waves per
+  // SIMD.  Two levels keep the FP64 rounding of a long K at that of kFlushK + K / kFlushK terms.
+  __shared__ double acc64s[2 * FN * 4][64 * kWaves];
   auto acc64 = [&](int fm, int fn, int r) -> double& {
-    return acc64s[kF32 ? (fm * FN + fn) * 4 + r : 0][threadIdx.x];
+    return acc64s[(fm * FN + fn) * 4 + r][threadIdx.x];
   };
 #pragma unroll
   for (int fm = 0; fm < 2; ++fm)
 #pragma unroll
     for (int fn = 0; fn < FN; ++fn) {
       acc[fm][fn] = AccT{0, 0, 0, 0};
-      if constexpr (kF32) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc64(fm, fn, r) = 0.0;
-      }
+      for (int r = 0; r < 4; ++r) acc64(fm, fn, r) = 0.0;
     }
 
   if constexpr (kF32) {
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
 #pragma unroll
           for (int fn = 0; fn < FN; ++fn)
             acc[fm][fn] = mfma(ca[fm][s], cb[fn][s], acc[fm][fn]);
-      if (kF32 && ((k0 + kChunk) % kFlushK) == 0) flush();
+      if (((k0 + kChunk) % kFlushK) == 0) flush();
     }
   }
   if (!kK4 && K16 < K) {  // last partial chunk: rows k >= K contribute zero (K4: K % 16 == 0)
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
     }
     step();
   }
-  if (kF32) flush();
+  flush();
 
 #pragma unroll
   for (int fm = 0; fm < 2; ++fm)
@@ -254,7 +254,7 @@ __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
         // result layout: FP32 16x16x4 holds row 4g + r in register r of lane group g, FP64
         // 16x16x4 holds row g + 4r (column = lane % 16 in both)
         const int64_t row = m_base + fm * 16 + (kF32 ? 4 * g + r : g + 4 * r);
-        const double v = kF32 ? acc64(fm, fn, r) : (double)acc[fm][fn][r];
+        const double v = acc64(fm, fn, r);
         if (row < P) Y[row * ldy + col] = off + v;
       }
     }

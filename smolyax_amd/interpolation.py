@@ -25,6 +25,9 @@ MAX_CALL_POINTS = 65535 * 64  # points per call of sbi_contract / _jacobian_f64 
 PROXY_TOL_FP64 = 1e-12   # up to this: every path (newton-fp64, gradient, adjoint in FP64)
 PROXY_TOL_MIXED = 1e-7   # up to this: the FP32-accurate paths (mixed, bf16) only
 PROXY_SAMPLES = 256
+# "mixed" on the GEMM path (d_out > 1): while sample points are further than this from the FP64
+# path (relative to max |y|), 4x more of the heaviest Newton basis rows move into the FP64 head
+MIXED_TARGET = 2e-7
 
 
 class SmolyaxPrecisionWarning(UserWarning):
@@ -97,6 +100,8 @@ class SmolyakBarycentricInterpolator:
         self._proxy_err = None      # non-nested: proxy Newton form vs the FP64 barycentric path
         self._newton_fp64_ok = False
         self._warned_mixed = False
+        self._mixed_err = None      # mixed GEMM path vs the FP64 path on sample points (d_out > 1)
+        self._maxdeg = None         # per dim: highest degree in Lambda (sample points)
         self._layout = None
         self._prog = None           # record program of the d_out = 1 FP64 eval kernel
         if f is not None:
@@ -146,6 +151,7 @@ class SmolyakBarycentricInterpolator:
                 self._newton_off = str(e)
             if self._nplan is not None:
                 self._check_proxy()
+        self._calibrate_mixed()
         if self._newton_off and self._precision in ("newton-fp64", "bf16"):
             raise ValueError(f"precision {self._precision!r}: {self._newton_off}")
         if self._n_inputs is not None:
@@ -186,12 +192,43 @@ class SmolyakBarycentricInterpolator:
         nf.set_values(Y.cpu().numpy())
         return nf
 
+    def _calibrate_mixed(self):
+        """The mixed GEMM path (d_out > 1) contracts the Newton basis rows with the largest terms
+        in FP64 (the head) and the others with FP32 operands.  Its error is measured against the
+        FP64 path on sample points; while it exceeds MIXED_TARGET of max |y|, 4x more rows move
+        into the head (at most all of them, which is the FP64 contraction)."""
+        self._mixed_err = None
+        if self._nplan is None or self._d_out == 1:
+            return
+        # points from the nodes' measure only: far out, the polynomial's own size sets the scale
+        X = torch.as_tensor(self._proxy_samples()[: PROXY_SAMPLES // 2], device=self._device)
+        keep, warned = self._n_inputs, self._warned_mixed
+        ref = self.evaluate(X, precision="newton-fp64" if self._newton_fp64_ok else "fp64")
+        scale = float(ref.abs().max()) or 1.0
+        NL = self._nlayout
+        while True:
+            err = float((self.evaluate(X, precision="mixed") - ref).abs().max()) / scale
+            self._mixed_err = err
+            if err <= MIXED_TARGET or NL.n_head >= NL.basis_n:
+                break
+            newton.set_head(self._nform, NL, min(NL.basis_n, 4 * NL.n_head))
+            self._upload_newton()
+        self._n_inputs, self._warned_mixed = keep, warned
+
     def _proxy_samples(self):
-        """Sample points of a non-nested interpolant: half from its nodes' measure (Gauss-Hermite:
-        mean + scaling * N(0, 1/2) per dim), half uniform over the span of each dim's nodes."""
+        """Sample points: half from the nodes' measure (Gauss-Hermite: mean + scaling * N(0, 1/2)
+        per dim; Leja: uniform on each dim's domain), half uniform over the span of each dim's
+        nodes."""
         rng = np.random.default_rng(12345)
         d, n = self._d_in, PROXY_SAMPLES // 2
-        Z = rng.standard_normal((n, d)) / np.sqrt(2.0)
+        if getattr(self, "_maxdeg", None) is None or len(self._maxdeg) != d:
+            self._maxdeg = np.zeros(d, dtype=np.int64)
+            for nu in indices.indexset(self._k, self._t):
+                for j, v in nu:
+                    self._maxdeg[j] = max(self._maxdeg[j], v)
+        # Leja: uniform on the reference interval [-1, 1], the domain's measure
+        Z = rng.uniform(-1.0, 1.0, (n, d)) if self._is_nested else \
+            rng.standard_normal((n, d)) / np.sqrt(2.0)
         A = np.stack([np.asarray(g.scale(Z[:, j]), dtype=np.float64)
                       for j, g in enumerate(self._node_gen)], axis=1)
         B = np.empty((n, d))
@@ -734,6 +771,17 @@ class SmolyakBarycentricInterpolator:
         """The path that serves precision 'mixed': "newton" or "bary-fp32" (Newton form
         unavailable, see the constructor)."""
         return "newton" if self._nplan is not None else "bary-fp32"
+
+    @property
+    def mixed_error(self):
+        """d_out > 1 with a Newton form: the mixed GEMM path's largest relative distance from the
+        FP64 path on sample points, after its FP64 head was sized (None otherwise)."""
+        return self._mixed_err
+
+    @property
+    def mixed_head_rows(self):
+        """Newton basis rows the mixed GEMM path contracts in FP64 (None without a Newton form)."""
+        return None if self._nlayout is None else int(self._nlayout.n_head)
 
     @property
     def proxy_error(self):

@@ -272,10 +272,17 @@ def build_newton_layout(nf: NewtonForm, n_waves: int = N_WAVES) -> NewtonLayout:
     build_cost_order_tables(nf, L, rows)
     build_basis_tables(nf, L)
     # basis rows for the GEMM paths: the heaviest rows first (FP64 head of the mixed contraction)
-    L.basis_order, L.n_head, L.head_fac = head_order(nf, L)
-    desc = L.basis_desc.view(np.uint16).reshape(L.basis_n, 8)
-    L.basis_desc = np.ascontiguousarray(desc[L.basis_order]).reshape(-1).view(np.uint8)
+    L.basis_desc_nus = L.basis_desc
+    set_head(nf, L, HEAD_ROWS)
     return L
+
+
+def set_head(nf: NewtonForm, L: NewtonLayout, n_head: int):
+    """Put the n_head heaviest rows first in the GEMM paths' basis row order (head_order) and
+    permute the basis row descriptors to that order."""
+    L.basis_order, L.n_head, L.head_fac = head_order(nf, L, n_head)
+    desc = L.basis_desc_nus.view(np.uint16).reshape(L.basis_n, 8)
+    L.basis_desc = np.ascontiguousarray(desc[L.basis_order]).reshape(-1).view(np.uint8)
 
 
 LINE_DIMS = 16    # dims per 128-byte line of coordinates

@@ -135,3 +135,63 @@ def test_points_from_pending_producer(gpu):
     for i, r in enumerate(reqs):
         alone = srv.ips[r.name].evaluate(bases[i], precision="mixed")
         assert torch.equal(r.y, alone), i
+
+
+def test_serving_workload_at_size(gpu):
+    """The serving workload's extremes at their full size: d_in 500 and d_in 8 at 2e4 nodes with
+    d_out 1024, d_in 500 at 1e2 nodes with d_out 1, and the largest interpolant of the 32-spec set
+    (by nodes x d_out), beside two small ones.  Served results (FP32 operands, FP64 accumulation)
+    equal each request evaluated alone bit for bit; on a sample of 8 points per interpolant the
+    FP64 path is held to the exact interpolant of the FP64 data (the oracle in extended precision)
+    within 4x the FP64 oracle's own distance from it, and the served values to 1e-6 of max |y|
+    where the Newton form serves 'mixed' (1e-4 on the FP32 barycentric form)."""
+    from oracle import nodes as ON
+    from oracle import smolyak as OS
+    from smolyax_amd import serving
+    from smolyax_amd import workloads as W
+
+    base = W.serving_specs(32, seed=0)
+    largest = max(base, key=lambda sp: sp["nodes"] * sp["d_out"])
+    specs = [dict(name="xl500", d_in=500, nodes=20000, d_out=1024, fseed=7),
+             dict(name="xl8", d_in=8, nodes=20000, d_out=1024, fseed=8),
+             dict(name="xs500", d_in=500, nodes=100, d_out=1, fseed=9),
+             dict(largest, name="big")] + small_specs()[:2]
+    ips = W.build_serving_interpolants(specs)
+    names = sorted(ips)
+    for n in names:
+        ip = ips[n]
+        print(f"serving at size: {n} d_in={ip.d_in} d_out={ip.d_out} k_raw={ip.layout.k_raw} "
+              f"mixed via {ip.mixed_path} ({ip.mixed_head_rows} FP64 head rows, "
+              f"{ip.mixed_error} from FP64 on sample points)", flush=True)
+    srv = serving.Server(ips, precision="mixed", n_streams=4, max_batch_points=4096)
+    reqs, _ = serving.run_poisson(srv, names, qps=5000.0, n_requests=90, batch_hi=2048, seed=5,
+                                  time_limit_s=120.0)
+    torch.cuda.synchronize()
+    assert len(reqs) == 90 and all(r.done is not None for r in reqs)
+    for r in reqs:
+        alone = srv.ips[r.name].evaluate(r.x, precision="mixed")
+        assert torch.equal(r.y, alone), (r.rid, r.name, r.batch_points)
+    by_name = {}
+    for r in reqs:
+        by_name.setdefault(r.name, r)
+    sp_of = {sp["name"]: sp for sp in specs}
+    bad = []
+    for n, r in sorted(by_name.items()):
+        sp = sp_of[n]
+        ip = srv.ips[n]
+        d = sp["d_in"]
+        f = W.f_sin(d, sp["d_out"], seed=sp["fseed"])
+        ref = OS.Interpolant(d_out=sp["d_out"], node_gen=ON.gauss_hermite(
+            scaling=np.full(d, np.sqrt(2.0))), k=W.k_rule(d), t=ip._t,
+            f=lambda x, f=f: f(np.asarray(x)[None, :])[0])
+        xs = r.x[:8].cpu().numpy()
+        y_ex = ref(xs, dtype=np.longdouble).astype(np.float64)
+        scale = np.max(np.abs(y_ex))
+        e_ora = np.max(np.abs(ref(xs) - y_ex)) / scale
+        e64 = np.max(np.abs(ip.evaluate(r.x[:8], precision="fp64").cpu().numpy() - y_ex)) / scale
+        e_srv = np.max(np.abs(r.y[:8].cpu().numpy() - y_ex)) / scale
+        print(f"{n}: served {e_srv:.1e}, FP64 path {e64:.1e}, FP64 oracle {e_ora:.1e} from the "
+              f"exact values ({ip.mixed_path})", flush=True)
+        if e64 > max(4.0 * e_ora, 1e-14) or e_srv > (1e-6 if ip.mixed_path == "newton" else 1e-4):
+            bad.append(n)
+    assert not bad, bad
