@@ -42,8 +42,9 @@ def parse():
     return ap.parse_args()
 
 
-TRAFFIC_FILES = {("newton_mixed_staged_kernel", "c2_exp_d1000"):
-                 "profiles/newton_mixed_staged_c2_d1000_traffic.md"}
+TRAFFIC_FILES = {(k, "c2_exp_d1000"): "profiles/newton_mixed_mfma_c2_d1000_traffic.md"
+                 for k in ("newton_mixed_mfma_kernel", "bary_eval_prog_kernel",
+                           "newton_eval_kernel<double>")}
 
 
 def measured_traffic(kernel, config):
@@ -551,6 +552,7 @@ def main():
     roof_fp64 = {"bound": "valu", "kernel": kname_b, "achieved": flops / t_b / 1e12,
                  "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                  "frac": flops / t_b / 1e12 / PEAK_FP64_TFLOPS, "avg_launch_ms": results["fp64"]["kernel_ms"],
+                 "traffic": measured_traffic(kname_b, args.config),
                  "flop_basis": "2 d_out K_raw per point (the reference's terms)"}
     # the FP64 Newton kernel on the same algorithmic basis, and on its own terms (one FMA per
     # node and point plus the per-group FP64 work: 2 |Lambda| + 4 groups per point)
@@ -561,6 +563,7 @@ def main():
                    "frac": flops / t_n / 1e12 / PEAK_FP64_TFLOPS,
                    "flop_basis": "2 d_out K_raw per point (the reference's terms)",
                    "own_terms_tflops": flop_nt / t_n / 1e12,
+                   "traffic": measured_traffic("newton_eval_kernel<double>", args.config),
                    "avg_launch_ms": results["newton-fp64"]["kernel_ms"]}
 
     rows = bench_gradient_integral(ip, x, torch)
