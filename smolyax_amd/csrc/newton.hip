@@ -467,8 +467,9 @@ __global__ __launch_bounds__(kTile* kWaves) void newton_mixed_staged_kernel(
 //   LDS: [W: rw_pad rows x 64 floats, element (q, p) at ((q / 4) * 64 + p) * 4 + q % 4][region R]
 //   region R, phase 1: h table of the internal nodes' factors
 //   region R, phase 2: A steps (256 bytes each) | group descriptors (16 bytes) | work items
-// 12 waves per 64-point tile take work items (step ranges of one tile; y is linear in the partial
-// sums) balanced on the host.  (The operand roles are swapped in the loop: see step 3.)
+// 16 waves per 64-point tile (4 wave groups x 4 subsets of 16 points) take work items (step
+// ranges of one tile; y is linear in the partial sums) balanced on the host.  (The operand roles
+// are swapped in the loop: see step 3.)
 // ------------------------------------------------------------------------------------------
 constexpr int kMWaves = 16;   // waves per tile: kMGroups wave groups x 4 point subsets
 constexpr int kMGroups = 4;   // wave groups (the host balances the work items over them)
