@@ -472,6 +472,8 @@ __global__ __launch_bounds__(kTile* kWaves) void newton_mixed_staged_kernel(
 // are swapped in the loop: see step 3.)
 // ------------------------------------------------------------------------------------------
 constexpr int kMWaves = 16;   // waves per tile: kMGroups wave groups x 4 point subsets
+constexpr int kHsBatch = 4;   // table dims per wave whose coordinates are requested together
+constexpr int kLvBatch = 8;   // internal-node records per wave read together
 constexpr int kMGroups = 4;   // wave groups (the host balances the work items over them)
 
 __device__ __forceinline__ int widx(int q, int p) { return ((q >> 2) * kTile + p) * 4 + (q & 3); }
@@ -521,15 +523,30 @@ __global__ __launch_bounds__(kTile* kMWaves) void newton_mixed_mfma_kernel(
   };
   // (past the group's items: item 0's descriptor, loaded but never used; mi_n >= 1)
   auto first = [&](int i) { return gdesc[mi_rec[4 * (i < i_end ? i : 0) + 3] * 16 + li]; };
-  // the four descriptors first, then the coordinates (a load waited for waits for all before it)
+  // The internal-node records of this wave (record q = wave + 16 u in lane u) and the level
+  // offsets (lane l), by vector loads issued before anything else: the products of 1b then need
+  // no scalar round trip per level (records beyond 64 per wave take the scalar path).
+  const int nint = d.n_internal, nlev = d.n_levels;
+  const bool rec_v_ok = nint <= kTile * kMWaves && nlev < kTile;
+  const int qv = min(wave + kMWaves * lane, max(nint - 1, 0));
+  const int rec_v = d.rint_rec[qv], hrow_v = d.rint_h[qv];
+  const int lo_v = d.level_off[min(lane, nlev)];
+  // The four descriptors first, then the coordinates of the first kHsBatch table dims of this
+  // wave (1a), then the four items' coordinates: vector loads complete in order, so each wait
+  // below covers exactly the loads it needs (every such load is unconditional, indices clamped).
   const v4i f1 = first(i_begin), f2 = first(i_begin + 1), f3 = first(i_begin + 2),
             f4 = first(i_begin + 3);
-  Pre n1 = fetch(f1), n2 = fetch(f2), n3 = fetch(f3), n4 = fetch(f4);
+  const int nhs = d.n_hs_dims;
+  int hj[kHsBatch];
+  double hx[kHsBatch];
+#pragma unroll
+  for (int u = 0; u < kHsBatch; ++u) hj[u] = hs_dim[min(wave + u * kMWaves, max(nhs - 1, 0))];
+#pragma unroll
+  for (int u = 0; u < kHsBatch; ++u) hx[u] = xp[hj[u]];
 
   // 1a. h values of the internal nodes' factors (lane = point), FP64 -> FP32
-  for (int i = wave; i < d.n_hs_dims; i += kMWaves) {
-    const int j = hs_dim[i], km = hs_kmax[i], row = hs_row[i];
-    const double xj = xp[j];
+  auto h_rows = [&](int i, int j, double xj) {
+    const int km = hs_kmax[i], row = hs_row[i];
     cptr<double> xi = dim_xi + (int64_t)j * d.kmax1;
     cptr<double> cj = dim_c + (int64_t)j * d.kmax1;
     double pr = 1.0;
@@ -537,17 +554,43 @@ __global__ __launch_bounds__(kTile* kMWaves) void newton_mixed_mfma_kernel(
       pr *= (xj - xi[k - 1]);
       R[(row + k - 1) * kTile + lane] = (float)(pr * cj[k]);
     }
-  }
+  };
+#pragma unroll
+  for (int u = 0; u < kHsBatch; ++u)
+    if (wave + u * kMWaves < nhs) h_rows(wave + u * kMWaves, hj[u], hx[u]);
+  for (int i = wave + kHsBatch * kMWaves; i < nhs; i += kMWaves) h_rows(i, hs_dim[i], xp[hs_dim[i]]);
   for (int q = wave; q < rw_pad; q += kMWaves)
     if (q == 0 || q >= d.rw_n) W[widx(q, lane)] = (q == 0) ? 1.0f : 0.0f;
   __syncthreads();
-  // 1b. products of the internal nodes, depth by depth: W[row] = W[parent] * h
-  for (int lv = 0; lv < d.n_levels; ++lv) {
+  // 1b. products of the internal nodes, depth by depth: W[row] = W[parent] * h; a wave's records
+  // of a level are read kLvBatch at a time (one scalar round trip per batch)
+  for (int lv = 0; rec_v_ok && lv < nlev; ++lv) {
+    const int a = __builtin_amdgcn_readlane(lo_v, lv), b = __builtin_amdgcn_readlane(lo_v, lv + 1);
+    // this wave's records q in [a, b), q = wave (mod 16): lanes u0 .. u1 - 1
+    const int u0 = a <= wave ? 0 : (a - wave + kMWaves - 1) / kMWaves;
+    const int u1 = b <= wave ? 0 : (b - wave + kMWaves - 1) / kMWaves;
+    for (int u = u0; u < u1; ++u) {
+      const uint32_t rec = (uint32_t)__builtin_amdgcn_readlane(rec_v, u);
+      const int hr = __builtin_amdgcn_readlane(hrow_v, u);
+      W[widx(rec & 0xffff, lane)] = W[widx(rec >> 16, lane)] * R[hr * kTile + lane];
+    }
+    __syncthreads();
+  }
+  for (int lv = 0; !rec_v_ok && lv < nlev; ++lv) {
     const int a = level_off[lv], b = level_off[lv + 1];
-    for (int q = a + wave; q < b; q += kMWaves) {
-      const uint32_t rec = (uint32_t)rint_rec[q];
-      const int row = rec & 0xffff, par = rec >> 16;
-      W[widx(row, lane)] = W[widx(par, lane)] * R[rint_h[q] * kTile + lane];
+    for (int q0 = a + wave; q0 < b; q0 += kLvBatch * kMWaves) {
+      uint32_t rec[kLvBatch];
+      int hr[kLvBatch];
+#pragma unroll
+      for (int u = 0; u < kLvBatch; ++u) {
+        const int q = min(q0 + u * kMWaves, b - 1);
+        rec[u] = (uint32_t)rint_rec[q];
+        hr[u] = rint_h[q];
+      }
+#pragma unroll
+      for (int u = 0; u < kLvBatch; ++u)
+        if (q0 + u * kMWaves < b)
+          W[widx(rec[u] & 0xffff, lane)] = W[widx(rec[u] >> 16, lane)] * R[hr[u] * kTile + lane];
     }
     __syncthreads();
   }
@@ -570,6 +613,9 @@ __global__ __launch_bounds__(kTile* kMWaves) void newton_mixed_mfma_kernel(
   stage(d.mt_grp, GD, d.mt_n * 256);
   stage(d.mi_rec, IT, d.mi_n * 16);
   stage(d.dim_xi, XI, d.n_xi_dims * d.kmax1 * 8);
+  // the first four items' coordinates (after the tables: their burst of loads would hold up the
+  // tables' scalar loads)
+  Pre n1 = fetch(f1), n2 = fetch(f2), n3 = fetch(f3), n4 = fetch(f4);
   __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed
   __syncthreads();
   const lds_f32* A = (const lds_f32*)AS;

@@ -335,6 +335,10 @@ def build_cost_order_tables(nf: NewtonForm, L: NewtonLayout, hs_rows: dict) -> N
 MFMA_WAVES = 4       # wave groups per point tile of the matrix-core mixed kernel (kMGroups)
 MFMA_ITEM_STEPS = 16  # k steps per work item at most (longer tiles are split over items)
 MFMA_ITEM_COST = 12.0  # per-item work besides its k steps (coordinates, Newton factors, sums)
+# Relative rates of the kernel's wave groups 0..3 on equal work (in-kernel cycle counts at
+# d_in = 1000: the older waves are issued first, so the groups of higher waves finish later);
+# the work items are balanced on these rates.
+MFMA_GROUP_SPEED = (1.0, 0.85, 0.60, 0.53)
 
 
 def build_mfma_tables(L: NewtonLayout, vecs, n_waves: int = MFMA_WAVES) -> None:
@@ -407,10 +411,12 @@ def build_mfma_tables(L: NewtonLayout, vecs, n_waves: int = MFMA_WAVES) -> None:
         for c in range(0, ns, per):
             items.append((a0 + c, c, min(per, ns - c), ti))
     cost = [it[2] + MFMA_ITEM_COST for it in items]
-    load = [0.0] * n_waves
+    load = np.zeros(n_waves)
+    speed = np.array(MFMA_GROUP_SPEED[:n_waves]) if n_waves == len(MFMA_GROUP_SPEED) else \
+        np.ones(n_waves)
     owner = [[] for _ in range(n_waves)]
     for ii in sorted(range(len(items)), key=lambda i: -cost[i]):
-        w = int(np.argmin(load))
+        w = int(np.argmin((load + cost[ii]) / speed))
         owner[w].append(ii)
         load[w] += cost[ii]
     L.mwave_item = np.array(np.cumsum([0] + [len(o) for o in owner]), dtype=np.int32)
