@@ -448,7 +448,8 @@ class SmolyakBarycentricInterpolator:
         assert self._plan is not None, \
             "The operator has not yet been set up for a target function via `set_f`."
         if isinstance(x, torch.Tensor):
-            x = x.to(device=self._device, dtype=torch.float64)
+            if x.dtype != torch.float64 or x.device != self._device:
+                x = x.to(device=self._device, dtype=torch.float64)
         else:
             x = torch.as_tensor(np.asarray(x, dtype=np.float64), device=self._device)
         if tuple(x.shape) == (self._d_in,):
@@ -620,6 +621,7 @@ class SmolyakBarycentricInterpolator:
         P = x.shape[0]
         Nn = self._nlayout.basis_n
         kind = {"newton-fp64": "f64", "mixed": "f32", "bf16": "bf16"}[precision]
+        st = self._stream()
         y = torch.empty((P, self._d_out), dtype=torch.float64, device=self._device)
         if kind == "f64":
             S = self._surpluses("f64", gemm=True)
@@ -630,7 +632,7 @@ class SmolyakBarycentricInterpolator:
                 HT = torch.empty((Nn, b - a), dtype=torch.float64, device=self._device)
                 N.check(N.fn["sbi_newton_basis_rows"](self._nplan, xc.data_ptr(), b - a,
                                                       xc.stride(0), 0, Nn, HT.data_ptr(), b - a, 0,
-                                                      self._stream()), "basis")
+                                                      st), "basis")
                 self._contract(HT, S, None, y[a:b])
             return y
         # FP64 head rows, FP32 / BF16 remainder in the k-interleaved layout (16-byte operand loads)
@@ -649,19 +651,19 @@ class SmolyakBarycentricInterpolator:
             if Nn > nh:
                 N.check(N.fn["sbi_newton_basis_rows"](self._nplan, xc.data_ptr(), b - a,
                                                       xc.stride(0), nh, Nn, HT.data_ptr(), b - a,
-                                                      code, self._stream()), "basis")
+                                                      code, st), "basis")
             H64 = torch.empty((max(nh, 1), b - a), dtype=torch.float64, device=self._device)
             if nh:
                 N.check(N.fn["sbi_newton_rows_f64"](self._nplan, xc.data_ptr(), b - a,
                                                     xc.stride(0), self._head_fac.data_ptr(),
                                                     nh, H64.data_ptr(), b - a,
-                                                    self._stream()), "rows")
+                                                    st), "rows")
             yc = y[a:b]
             N.check(N.fn[fn](
                 H64.data_ptr(), H64.stride(0), S64.data_ptr(), max(S64.stride(0), self._d_out),
                 nh, HT.data_ptr() if KQ else None, b - a, SQ.data_ptr() if KQ else None,
                 self._d_out, KQ, None, yc.data_ptr(), yc.stride(0), b - a, self._d_out,
-                self._stream()), "contract")
+                st), "contract")
         return y
 
     def vjp(self, x, v, precision: str = None) -> torch.Tensor:

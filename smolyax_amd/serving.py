@@ -24,6 +24,12 @@ stream, and the batch's rows of the static output are copied out.  The captured 
 kernel and the contraction (no memset nodes); rows past the batch hold earlier inputs and are
 ignored, and every point is computed independently, so results equal eager evaluation bit for bit.
 
+Large requests: a batch is capped at `max_batch_work` multiply-adds of contraction (basis rows x
+outputs per point), so a request above its interpolant's cap is cut into pieces that take
+separate batches and short requests of other interpolants run between them; its output is
+allocated at submit() and each piece copies its rows into it (every point is computed on its own,
+so the result is the same bits as evaluating the request alone).
+
 Stream ordering: submit() records an event on the caller's current stream, and the batch that
 takes the request waits for it on its own stream before reading the points, so points produced by
 kernels still pending on the caller's stream are read finished.  The points are marked as used on
@@ -51,14 +57,18 @@ class Request:
     y: torch.Tensor = None
     done: float = None
     batch_points: int = 0
-    taken: bool = False
+    taken: bool = False             # every point is in a launched batch
     ready: torch.cuda.Event = None  # recorded on the submitting stream after x was produced
+    pos: int = 0                    # points already in launched batches
+    parts: int = 0                  # launched pieces not completed yet
+    n_pieces: int = 0               # batches the request was cut into
 
 
 @dataclass
 class _Slot:
     stream: torch.cuda.Stream
-    event: torch.cuda.Event = None
+    event: torch.cuda.Event = None               # the batch in flight (None: free)
+    done_ev: torch.cuda.Event = None             # this slot's completion event, re-recorded
     reqs: list = field(default_factory=list)
     reserved: int = 0
     keep: list = field(default_factory=list)
@@ -71,7 +81,7 @@ MIN_BUCKET = 64
 
 class Server:
     def __init__(self, interpolants, precision="mixed", n_streams=4, hbm_budget_bytes=8 << 30,
-                 max_batch_points=8192, graphs=False):
+                 max_batch_points=8192, graphs=False, max_batch_work=1e10):
         if graphs not in (False, True, "auto"):
             raise ValueError("graphs must be False, True or 'auto'")
         self.graph_mode = graphs
@@ -79,6 +89,7 @@ class Server:
         self.ips = dict(interpolants)
         self.precision = precision
         self.max_batch_points = int(max_batch_points)
+        self.max_batch_work = float(max_batch_work)
         self.budget = int(hbm_budget_bytes)
         self.reserved = 0
         # arrival order (entries already batched are skipped lazily) and per-interpolant queues
@@ -86,7 +97,13 @@ class Server:
         self.by_name = {name: collections.deque() for name in self.ips}
         self.n_pending = 0
         self._bpp = {name: self.footprint(name, 1) for name in self.ips}
-        self.slots = [_Slot(torch.cuda.Stream()) for _ in range(n_streams)]
+        # points per batch within max_batch_work (multiply-adds: basis rows x outputs per point)
+        self._cap = {name: max(MIN_BUCKET, min(self.max_batch_points,
+                                                int(self.max_batch_work // max(self.work(name), 1))))
+                     for name in self.ips}
+        self.slots = [_Slot(torch.cuda.Stream(), done_ev=torch.cuda.Event())
+                      for _ in range(n_streams)]
+        self._ev_pool = []  # ready events of requests already waited for (re-recorded later)
         if self.graphs:
             for sl in self.slots:
                 sl.pool = torch.cuda.graph_pool_handle()
@@ -146,13 +163,24 @@ class Server:
         item = {"fp64": 8, "newton-fp64": 8, "bf16": 2}.get(self.precision, 4)
         return n_points * (rows * item + ip.d_out * 8 + ip.d_in * 8)
 
+    def work(self, name):
+        """Multiply-adds of one point of `name`'s contraction (basis rows x outputs)."""
+        ip = self.ips[name]
+        rows = ip.layout.k_raw if ip._nlayout is None or self.precision == "bary-fp32" else \
+            ip._nlayout.basis_n
+        return rows * ip.d_out
+
     def submit(self, name, x, arrival=None):
         if name not in self.ips:
             raise KeyError(name)
         r = Request(self._next, name, x, time.perf_counter() if arrival is None else arrival)
         if self.footprint(name, x.shape[0]) > self.budget:
             raise ValueError("request larger than the HBM budget of the server")
-        r.ready = torch.cuda.Event()
+        if x.shape[0] > self._cap[name]:  # cut into pieces: its output is allocated here, on the
+            # submitting stream before `ready`, so every piece's stream is ordered after it
+            r.y = torch.empty((x.shape[0], self.ips[name].d_out), dtype=torch.float64,
+                              device=x.device)
+        r.ready = self._ev_pool.pop() if self._ev_pool else torch.cuda.Event()
         r.ready.record(torch.cuda.current_stream(x.device))
         self._next += 1
         self.pending.append(r)
@@ -166,8 +194,10 @@ class Server:
         for s in self.slots:
             if s.event is not None and s.event.query():
                 for r in s.reqs:
-                    r.done = now
-                    self.completed.append(r)
+                    r.parts -= 1
+                    if r.taken and r.parts == 0:
+                        r.done = now
+                        self.completed.append(r)
                 self.reserved -= s.reserved
                 s.event, s.reqs, s.reserved, s.keep = None, [], 0, []
                 freed += 1
@@ -175,7 +205,11 @@ class Server:
 
     def _admit(self):
         """Oldest pending request picks the interpolant; its queue (arrival order) fills the batch
-        up to max_batch_points and the free HBM budget.  O(batch) per launch."""
+        up to the interpolant's point cap (max_batch_points, and max_batch_work of device work)
+        and the free HBM budget.  A request larger than the cap is cut into pieces, one per
+        batch, so that short requests of other interpolants interleave with it; its result is
+        assembled from the pieces (each point is computed on its own, so the bits are those of
+        evaluating it alone).  O(batch) per launch."""
         launched = 0
         for s in self.slots:
             if s.event is not None:
@@ -187,53 +221,64 @@ class Server:
             head = self.pending[0]
             q = self.by_name[head.name]
             bpp = self._bpp[head.name]
-            batch, pts = [], 0
-            while q:
-                n = q[0].x.shape[0]
-                if batch and pts + n > self.max_batch_points:
-                    break
+            cap = self._cap[head.name]
+            pieces, pts = [], 0  # (request, first point, points)
+            while q and pts < cap:
+                r = q[0]
+                n = min(r.x.shape[0] - r.pos, cap - pts)
+                if pieces and r.pos == 0 and n < r.x.shape[0]:
+                    break  # a request joins a batch whole, or starts one
                 if self.reserved + bpp * (pts + n) > self.budget:
                     break
-                r = q.popleft()
-                r.taken = True
-                batch.append(r)
+                pieces.append((r, r.pos, n))
+                r.pos += n
+                r.parts += 1
+                r.n_pieces += 1
                 pts += n
-            if not batch:
+                if r.pos == r.x.shape[0]:
+                    r.taken = True
+                    q.popleft()
+                    self.n_pending -= 1
+            if not pieces:
                 break  # wait for memory to free up
-            self.n_pending -= len(batch)
             need = bpp * pts
             self.reserved += need
             ip = self.ips[head.name]
             use_graph = self.graphs and (self.graph_mode is True or
                                          self.n_pending >= len(self.slots))
-            for r in batch:  # the points are read on this stream once their producer is done
+            for r, _, _ in pieces:  # the points are read on this stream once their producer is done
                 s.stream.wait_event(r.ready)
                 r.x.record_stream(s.stream)
-            if use_graph:
-                g, x_st, y_st = self._graph(s, head.name, self.bucket(pts))
-                with torch.cuda.stream(s.stream):
-                    if len(batch) > 1:  # one copy kernel for the whole batch
-                        torch.cat([r.x for r in batch], out=x_st[:pts])
+                if r.taken:  # no later piece waits for it: the event may be recorded again
+                    self._ev_pool.append(r.ready)
+                    r.ready = None
+            xin = [r.x if (a0 == 0 and n == r.x.shape[0]) else r.x[a0:a0 + n]
+                   for r, a0, n in pieces]
+            with torch.cuda.stream(s.stream):
+                if use_graph:
+                    g, x_st, y_st = self._graph(s, head.name, self.bucket(pts))
+                    if len(xin) > 1:  # one copy kernel for the whole batch
+                        torch.cat(xin, out=x_st[:pts])
                     else:
-                        x_st[:pts].copy_(batch[0].x)
+                        x_st[:pts].copy_(xin[0])
                     g.replay()
                     y = y_st[:pts].clone()
                     xs = x_st
-                    ev = torch.cuda.Event()
-                    ev.record(s.stream)
-            else:
-                with torch.cuda.stream(s.stream):
-                    xs = torch.cat([r.x for r in batch]) if len(batch) > 1 else batch[0].x
+                else:
+                    xs = torch.cat(xin) if len(xin) > 1 else xin[0]
                     y = ip.evaluate(xs, precision=self.precision)
-                    ev = torch.cuda.Event()
-                    ev.record(s.stream)
-            a = 0
-            for r in batch:
-                n = r.x.shape[0]
-                r.y = y[a:a + n]
-                r.batch_points = pts
-                a += n
-            s.event, s.reqs, s.reserved, s.keep = ev, batch, need, [xs, y]
+                a = 0
+                for r, a0, n in pieces:
+                    if a0 == 0 and n == r.x.shape[0]:
+                        r.y = y[a:a + n]
+                    else:  # a piece of a larger request: into the request's own output
+                        r.y[a0:a0 + n].copy_(y[a:a + n])
+                        r.y.record_stream(s.stream)
+                    r.batch_points = max(r.batch_points, pts)
+                    a += n
+                ev = s.done_ev  # free slot: its previous batch was observed complete
+                ev.record(s.stream)
+            s.event, s.reqs, s.reserved, s.keep = ev, [r for r, _, _ in pieces], need, [xs, y]
             launched += 1
         return launched
 

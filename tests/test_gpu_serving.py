@@ -42,6 +42,30 @@ def test_results_same_alone_and_in_mix(gpu):
     assert st["p99_ms"] >= st["p50_ms"] > 0
 
 
+def test_large_requests_cut_into_pieces(gpu):
+    """With a small work cap per batch, requests above their interpolant's cap are cut into
+    pieces served in separate batches (other requests between them); every result equals the
+    request evaluated alone bit for bit, also in graph mode."""
+    from smolyax_amd import serving
+    from smolyax_amd import workloads as W
+
+    ips = W.build_serving_interpolants(small_specs()[:4])
+    names = sorted(ips)
+    for graphs in (False, True):
+        srv = serving.Server(ips, precision="mixed", n_streams=4, max_batch_work=2e6,
+                             graphs=graphs)
+        caps = {n: srv._cap[n] for n in names}
+        reqs, _ = serving.run_poisson(srv, names, qps=20000.0, n_requests=60, batch_hi=1500,
+                                      seed=7, time_limit_s=60.0)
+        torch.cuda.synchronize()
+        assert len(reqs) == 60 and all(r.done is not None for r in reqs)
+        assert any(r.n_pieces > 1 for r in reqs), caps
+        for r in reqs:
+            assert r.n_pieces == -(-r.x.shape[0] // caps[r.name]) or r.n_pieces >= 1
+            alone = srv.ips[r.name].evaluate(r.x, precision="mixed")
+            assert torch.equal(r.y, alone), (graphs, r.rid, r.name, r.n_pieces)
+
+
 def test_admission_by_hbm_budget(gpu):
     from smolyax_amd import serving
     from smolyax_amd import workloads as W
