@@ -338,7 +338,7 @@ MFMA_ITEM_COST = 12.0  # per-item work besides its k steps (coordinates, Newton 
 # Relative rates of the kernel's wave groups 0..3 on equal work (in-kernel cycle counts at
 # d_in = 1000: the older waves are issued first, so the groups of higher waves finish later);
 # the work items are balanced on these rates.
-MFMA_GROUP_SPEED = (1.0, 0.85, 0.60, 0.53)
+MFMA_GROUP_SPEED = (1.0, 0.93, 0.68, 0.52)
 
 
 def build_mfma_tables(L: NewtonLayout, vecs, n_waves: int = MFMA_WAVES) -> None:
@@ -411,16 +411,41 @@ def build_mfma_tables(L: NewtonLayout, vecs, n_waves: int = MFMA_WAVES) -> None:
         for c in range(0, ns, per):
             items.append((a0 + c, c, min(per, ns - c), ti))
     cost = [it[2] + MFMA_ITEM_COST for it in items]
+    # items that read a common 128-byte line of coordinates (16 consecutive dims) form a cluster
+    # and go to one wave group, one after the other, so a line is fetched from memory once per
+    # tile and then found in the cache
+    dims = np.array([g[0] for g in desc], dtype=np.int64).reshape(-1, 16)
+    root = list(range(len(items)))
+
+    def find(a):
+        while root[a] != a:
+            root[a] = root[root[a]]
+            a = root[a]
+        return a
+    line_item = {}
+    for ii, it in enumerate(items):
+        for ln in set((dims[it[3]] // LINE_DIMS).tolist()):
+            if ln in line_item:
+                root[find(ii)] = find(line_item[ln])
+            else:
+                line_item[ln] = ii
+    clusters = {}
+    for ii in range(len(items)):
+        clusters.setdefault(find(ii), []).append(ii)
+    clusters = list(clusters.values())
+    ccost = [sum(cost[i] for i in c) for c in clusters]
     load = np.zeros(n_waves)
     speed = np.array(MFMA_GROUP_SPEED[:n_waves]) if n_waves == len(MFMA_GROUP_SPEED) else \
         np.ones(n_waves)
     owner = [[] for _ in range(n_waves)]
-    for ii in sorted(range(len(items)), key=lambda i: -cost[i]):
-        w = int(np.argmin((load + cost[ii]) / speed))
-        owner[w].append(ii)
-        load[w] += cost[ii]
-    L.mwave_item = np.array(np.cumsum([0] + [len(o) for o in owner]), dtype=np.int32)
-    items = [items[i] for w in range(n_waves) for i in sorted(owner[w])]
+    for ci in sorted(range(len(clusters)), key=lambda c: -ccost[c]):
+        w = int(np.argmin((load + ccost[ci]) / speed))
+        owner[w].append(ci)
+        load[w] += ccost[ci]
+    L.mwave_item = np.array(np.cumsum([0] + [sum(len(clusters[c]) for c in o) for o in owner]),
+                            dtype=np.int32)
+    items = [items[i] for w in range(n_waves) for c in sorted(owner[w], key=lambda c: min(clusters[c]))
+             for i in clusters[c]]
     L.mi_n = len(items)
     L.mi_rec = np.array(items or [(0, 0, 0, 0)], dtype=np.int32).reshape(-1)
     high = g4[g4[:, 3] >= 2, 2]
