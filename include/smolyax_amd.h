@@ -394,6 +394,31 @@ int sbi_contract_bf16_k8(const double* A64, int64_t lda64, const double* B64, in
                          int64_t ldb16, int64_t K16, const double* offset, double* Y, int64_t ldy,
                          int64_t P, int64_t n, void* stream);
 
+/* ----------------------------------------------------------------------------------------------
+ * Set-up on the device (all pointers device memory, row-major, stream = hipStream_t).
+ * ------------------------------------------------------------------------------------------- */
+
+/* interpolation.py:199-230 (set_f fills each group's F from the f values at the grid nodes):
+ * F[r, :] = scale[r] * V[node[r], :] for r < n_rows, with V[n_nodes, d_out] the f values at the
+ * distinct grid nodes, node[r] the grid node of F's row r and scale[r] the Smolyak coefficient of
+ * its block (this layout folds zeta into F); rows whose node lies outside [0, n_nodes) are zero
+ * (padding).  One product per element, so F equals the host packing's V[node] * scale bit for bit. */
+int sbi_pack_rows_f64(const double* V, int64_t n_nodes, int64_t d_out, const int32_t* node,
+                      const double* scale, int64_t n_rows, double* F, void* stream);
+
+/* No counterpart in the reference (its evaluation is barycentric; the Newton form is this
+ * library's own): grid values -> Newton surpluses in place, s[n_rows, ld] with d_out columns.
+ * Steps run in order; step k updates the targets tgt[step_off[k] .. step_off[k+1]) and target i
+ * takes the sources src[src_off[i] .. src_off[i+1]) in order:
+ *   s[tgt_i] = (...((s[tgt_i] - coef_q0 * s[src_q0]) - coef_q1 * s[src_q1]) ...),
+ * each product and difference rounded on its own (the host restatement's order and rounding, so
+ * the same bits).  A step's targets must not be sources of the same step.  Rows outside
+ * [0, n_rows) are skipped. */
+int sbi_hierarchize_f64(double* s, int64_t n_rows, int64_t d_out, int64_t ld,
+                        const int32_t* step_off, int32_t n_steps, const int32_t* tgt,
+                        const int32_t* src_off, const int32_t* src, const double* coef,
+                        void* stream);
+
 /* Descriptor accessors of the plan objects (shared by the library's translation units). */
 int sbi_bary_plan_desc(const sbi_plan* plan, const sbi_bary_desc** out);
 int sbi_newton_plan_desc(const sbi_newton_plan* plan, const sbi_newton_desc** out);

@@ -119,6 +119,9 @@ SIGNATURES = {
                                    c_vp, c_vp, c_i64, c_i64, c_i64, c_vp]),
     "sbi_contract": (c_i32, [c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64,
                              c_i32, c_vp]),
+    "sbi_pack_rows_f64": (c_i32, [c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp]),
+    "sbi_hierarchize_f64": (c_i32, [c_vp, c_i64, c_i64, c_i64, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp,
+                                    c_vp]),
     "sbi_bary_plan_desc": (c_i32, [c_vp, c_vp]),
     "sbi_newton_plan_desc": (c_i32, [c_vp, c_vp]),
 }

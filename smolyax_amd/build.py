@@ -16,7 +16,10 @@ OUT = os.path.join(HERE, "libsmolyax_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["bary.hip", "newton.hip", "basis.hip", "gemm.hip"]
+HIP_SOURCES = ["bary.hip", "newton.hip", "basis.hip", "gemm.hip", "pack.hip"]
+# per-file flags: the set-up kernels must round every product and difference on its own (their
+# results equal the host packing bit for bit)
+FILE_FLAGS = {"pack.hip": ["-ffp-contract=off"]}
 HOST_SOURCES = ["indices.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function"]
@@ -45,7 +48,7 @@ def build(verbose=True):
         if not os.path.exists(path):
             continue
         obj = os.path.join(objdir, src + ".o")
-        _run([HIPCC, "-c", *COMMON, *HIP_FLAGS, path, "-o", obj])
+        _run([HIPCC, "-c", *COMMON, *HIP_FLAGS, *FILE_FLAGS.get(src, []), path, "-o", obj])
         objs.append(obj)
     tmp = OUT + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", *objs, "-o", tmp])

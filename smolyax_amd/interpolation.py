@@ -118,39 +118,34 @@ class SmolyakBarycentricInterpolator:
             f_evals = {}
         blocks = packing.nonzero_blocks(self._k, self._t)
         L = packing.build_layout(blocks, self._node_gen, self._d_in, self._d_out)
-        F, offset, n_new = packing.assemble_values(L, blocks, self._node_gen, f, f_evals, vectorized)
-        self._n_f_evals_new += n_new
+        vt = packing.gather_values(L, blocks, self._node_gen, f, f_evals, vectorized)
+        self._n_f_evals_new += vt.n_new
         L.slot_q = packing.quadrature_table(L, self._node_gen)
         self._layout = L
         self._prog = None
-        self._F_host = F
-        self._offset_host = offset
+        self._values = vt          # host value table, packed into F on the device by _upload
         self._nlayout = None
         self._nform = None
         self._newton_off = None  # why the Newton form is unavailable (None: available)
         self._newton_fp64_ok = self._is_nested
         self._proxy_err = None
-        if self._is_nested:
-            try:
-                nf = newton.NewtonForm(self._k, self._t, self._node_gen, self._d_out)
-                vals = np.array([np.broadcast_to(np.asarray(f_evals[nu], dtype=np.float64)
-                                                 .reshape(-1), (self._d_out,)) for nu in nf.nus])
-                nf.set_values(vals)
-                nl = newton.build_newton_layout(nf)
-                self._nform, self._nlayout = nf, nl
-            except newton.CapacityError as e:
-                self._newton_off = str(e)
         self._upload()
-        if not self._is_nested:  # Newton form on a nested proxy grid (needs the FP64 plan)
-            try:
+        self._values = None
+        try:
+            if self._is_nested:
+                nf = newton.NewtonForm(self._k, self._t, self._node_gen, self._d_out)
+                self._set_surpluses(nf, packing.stack_values([f_evals[nu] for nu in nf.nus],
+                                                             self._d_out))
+                self._nform = nf
+            else:  # Newton form on a nested proxy grid (needs the FP64 plan)
                 self._nform = self._newton_proxy()
-                self._nlayout = newton.build_newton_layout(self._nform)
-                self._upload_newton()
-            except newton.CapacityError as e:
-                self._nform = self._nlayout = None
-                self._newton_off = str(e)
-            if self._nplan is not None:
-                self._check_proxy()
+            self._nlayout = newton.build_newton_layout(self._nform)
+            self._upload_newton()
+        except newton.CapacityError as e:
+            self._nform = self._nlayout = None
+            self._newton_off = str(e)
+        if not self._is_nested and self._nplan is not None:
+            self._check_proxy()
         self._calibrate_mixed()
         if self._newton_off and self._precision in ("newton-fp64", "bf16"):
             raise ValueError(f"precision {self._precision!r}: {self._newton_off}")
@@ -189,8 +184,27 @@ class SmolyakBarycentricInterpolator:
         keep = self._n_inputs
         Y = self.evaluate(torch.as_tensor(X, device=self._device), precision="fp64")
         self._n_inputs = keep
-        nf.set_values(Y.cpu().numpy())
+        self._set_surpluses(nf, Y)
         return nf
+
+    def _set_surpluses(self, nf, values):
+        """nf.s from the grid values at nf.nus (rows; host array or device tensor), hierarchised
+        on the device by sbi_hierarchize_f64 (the host program newton.hierarchy_steps)."""
+        dev, i32 = self._device, torch.int32
+        S = torch.as_tensor(values, dtype=torch.float64, device=dev).reshape(len(nf.nus), -1)
+        S = S.clone().contiguous()
+        if S.shape[1] != self._d_out:
+            S = S.expand(len(nf.nus), self._d_out).contiguous()
+        steps = newton.hierarchy_steps(nf.nus, nf.node_gen)
+        a = [_dev_tensor(steps.step_off, i32, dev), _dev_tensor(steps.tgt if len(steps.tgt) else [0], i32, dev),
+             _dev_tensor(steps.src_off, i32, dev), _dev_tensor(steps.src if len(steps.src) else [0], i32, dev),
+             _dev_tensor(steps.coef if len(steps.coef) else [0.0], torch.float64, dev)]
+        st = torch.cuda.current_stream(dev)
+        N.check(N.fn["sbi_hierarchize_f64"](S.data_ptr(), S.shape[0], self._d_out, self._d_out,
+                                            a[0].data_ptr(), steps.n_steps, a[1].data_ptr(),
+                                            a[2].data_ptr(), a[3].data_ptr(), a[4].data_ptr(),
+                                            ctypes.c_void_p(st.cuda_stream)), "hierarchize")
+        nf.s = S.cpu().numpy()
 
     def _calibrate_mixed(self):
         """The mixed GEMM path (d_out > 1) contracts the Newton basis rows with the largest terms
@@ -287,8 +301,16 @@ class SmolyakBarycentricInterpolator:
         t["fac_slot"] = _dev_tensor(L.fac_slot, i32, dev)
         t["blk_last_slot"] = _dev_tensor(L.blk_last_slot, i32, dev)
         t["blk_foff"] = _dev_tensor(L.blk_foff, i64, dev)
-        t["F"] = _dev_tensor(self._F_host if self._F_host.size else np.zeros((1, self._d_out)), f64, dev)
-        t["offset"] = _dev_tensor(self._offset_host, f64, dev)
+        vt = self._values
+        st = torch.cuda.current_stream(dev)
+        V = _dev_tensor(vt.V if vt.V.size else np.zeros((1, self._d_out)), f64, dev)
+        rn = _dev_tensor(vt.row_node if vt.row_node.size else [-1], i32, dev)
+        rs = _dev_tensor(vt.row_scale if vt.row_scale.size else [0.0], f64, dev)
+        t["F"] = torch.empty((max(L.k_raw, 1), self._d_out), dtype=f64, device=dev)
+        N.check(N.fn["sbi_pack_rows_f64"](V.data_ptr(), len(vt.V), self._d_out, rn.data_ptr(),
+                                          rs.data_ptr(), max(L.k_raw, 1), t["F"].data_ptr(),
+                                          ctypes.c_void_p(st.cuda_stream)), "pack_rows")
+        t["offset"] = _dev_tensor(vt.offset, f64, dev)
         t["blk_desc"] = _dev_tensor(L.blk_desc, i32, dev)
         t["seg_off"] = _dev_tensor(L.seg_off, i32, dev)
         t["seg_slot"] = _dev_tensor(L.seg_slot, i32, dev)
@@ -305,7 +327,13 @@ class SmolyakBarycentricInterpolator:
             t["pseg"] = _dev_tensor(prog.seg, i32, dev)
             t["pbasis"] = _dev_tensor(prog.basis, f64, dev)
             t["prow"] = _dev_tensor(prog.rows, i32, dev)
-            t["pF"] = _dev_tensor(packing.program_values(prog, self._F_host), f64, dev)
+            pn, ps = packing.program_rows(prog, vt.row_node, vt.row_scale)
+            pn, ps = _dev_tensor(pn, i32, dev), _dev_tensor(ps, f64, dev)
+            t["pF"] = torch.empty(len(prog.f_index), dtype=f64, device=dev)
+            N.check(N.fn["sbi_pack_rows_f64"](V.data_ptr(), len(vt.V), 1, pn.data_ptr(),
+                                              ps.data_ptr(), len(prog.f_index), t["pF"].data_ptr(),
+                                              ctypes.c_void_p(st.cuda_stream)), "pack_rows")
+        st.synchronize()  # F is read on other streams (serving) once set_f returns
         self._dev = t
         d = N.BaryDesc()
         d.d_in, d.d_out = self._d_in, self._d_out

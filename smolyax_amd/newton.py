@@ -39,46 +39,76 @@ def newton_basis(x, xi, c, k):
     return out
 
 
-def hierarchize(nus, values, node_gen):
-    """Surpluses for the multi-indices `nus` (sparse tuples, downward closed) from grid values.
+class HierSteps:
+    """Hierarchisation as a program: steps in order; step k updates the rows
+    tgt[step_off[k]:step_off[k+1]], target i from the rows src[src_off[i]:src_off[i+1]] with
+    coefficients coef[...] in that order: s[tgt] = (...((s[tgt] - c0 s[src0]) - c1 s[src1]) ...).
+    The layout of sbi_hierarchize_f64's arguments."""
 
-    values: (|Lambda|, d_out) f at the grid nodes, row order of `nus`.  Works in place on a copy.
-    """
+    def __init__(self, step_off, tgt, src_off, src, coef):
+        self.step_off, self.tgt, self.src_off, self.src, self.coef = step_off, tgt, src_off, src, coef
+
+    @property
+    def n_steps(self):
+        return len(self.step_off) - 1
+
+
+def hierarchy_steps(nus, node_gen):
+    """1D hierarchisation along every dimension (ascending), level by level (ascending), of the
+    multi-indices `nus` (sparse tuples, downward closed, rows in this order): the target nu with
+    nu_j = lv takes the rows of nu with nu_j replaced by l = 0 .. lv - 1 (already final along j),
+    with coefficients h_l(xi_lv) of dim j's Newton basis."""
     index = {nu: i for i, nu in enumerate(nus)}
-    s = np.array(values, dtype=np.float64, copy=True)
-    dims = sorted({j for nu in nus for j, _ in nu})
     maxdeg = {}
     for nu in nus:
         for j, v in nu:
             maxdeg[j] = max(maxdeg.get(j, 0), v)
-    for j in dims:
-        g = node_gen[j]
-        xi = np.asarray(g(maxdeg[j]), dtype=np.float64)
+    by_dim = {}
+    for nu in nus:
+        for pos, (j, v) in enumerate(nu):
+            by_dim.setdefault(j, {}).setdefault(v, []).append((nu, pos))
+    step_off, tgt, src_off, src, coef = [0], [], [0], [], []
+    for j in sorted(by_dim):
+        xi = np.asarray(node_gen[j](maxdeg[j]), dtype=np.float64)
         c = newton_basis_consts(xi)
-        # H[i, l] = h_l(xi_i) for l < i
-        H = np.zeros((len(xi), len(xi)))
+        H = np.zeros((len(xi), len(xi)))  # H[i, l] = h_l(xi_i) for l < i
         for i in range(len(xi)):
             for l in range(i):
                 H[i, l] = newton_basis(xi[i], xi, c, l)
-        # process nu with nu_j = 1, 2, ... in increasing order (lower levels already final)
-        by_level = {}
-        for nu in nus:
-            d = dict(nu)
-            lv = d.get(j, 0)
-            if lv > 0:
-                by_level.setdefault(lv, []).append(nu)
-        for lv in sorted(by_level):
-            for nu in by_level[lv]:
-                d = dict(nu)
-                tgt = index[nu]
+        for lv in sorted(by_dim[j]):
+            for nu, pos in by_dim[j][lv]:
+                tgt.append(index[nu])
+                head, tail = nu[:pos], nu[pos + 1:]
                 for l in range(lv):
-                    if l == 0:
-                        d.pop(j, None)
-                    else:
-                        d[j] = l
-                    key = tuple(sorted(d.items()))
-                    s[tgt] -= H[lv, l] * s[index[key]]
+                    key = head + tail if l == 0 else head + ((j, l),) + tail
+                    src.append(index[key])
+                    coef.append(H[lv, l])
+                src_off.append(len(src))
+            step_off.append(len(tgt))
+    i32 = np.int32
+    return HierSteps(np.array(step_off, dtype=i32), np.array(tgt, dtype=i32),
+                     np.array(src_off, dtype=i32), np.array(src, dtype=i32),
+                     np.array(coef, dtype=np.float64))
+
+
+def apply_steps(steps, values):
+    """Host form of sbi_hierarchize_f64: the program applied to a copy of values (rows x d_out)."""
+    s = np.array(values, dtype=np.float64, copy=True)
+    for k in range(steps.n_steps):
+        for i in range(steps.step_off[k], steps.step_off[k + 1]):
+            t = steps.tgt[i]
+            v = s[t].copy()
+            for q in range(steps.src_off[i], steps.src_off[i + 1]):
+                v = v - steps.coef[q] * s[steps.src[q]]
+            s[t] = v
     return s
+
+
+def hierarchize(nus, values, node_gen):
+    """Surpluses for the multi-indices `nus` (sparse tuples, downward closed) from grid values.
+
+    values: (|Lambda|, d_out) f at the grid nodes, row order of `nus`."""
+    return apply_steps(hierarchy_steps(nus, node_gen), values)
 
 
 class CapacityError(ValueError):

@@ -322,13 +322,44 @@ def entry_keys(fd, fg):
         yield mu, key
 
 
-def assemble_values(L: Layout, blocks: Blocks, node_gen, f, f_evals, vectorized=False):
-    """Evaluate f at the grid nodes (reusing f_evals) and fill the zeta-scaled device F.
+@dataclass
+class ValueTable:
+    """f at the distinct grid nodes and the map from F's rows to them:
+    F[r] = row_scale[r] * V[row_node[r]] (the block's zeta folded in)."""
+    V: np.ndarray            # (n_nodes, d_out) f values, one row per distinct grid node
+    row_node: np.ndarray     # (k_raw,) int32 row of V for each row of F
+    row_scale: np.ndarray    # (k_raw,) zeta of the row's block
+    offset: np.ndarray       # (d_out,) zeta_0 * f(root)
+    n_new: int               # calls of f (new nodes)
+
+
+def stack_values(vals, d_out):
+    """Rows of f values (scalars or arrays of d_out, or of 1 to be broadcast) as (n, d_out)."""
+    n = len(vals)
+    if n == 0:
+        return np.zeros((0, d_out))
+    try:
+        A = np.asarray(vals, dtype=np.float64).reshape(n, -1)
+    except ValueError:  # ragged: mixed scalars and arrays
+        A = np.stack([np.asarray(v, dtype=np.float64).reshape(-1) for v in vals]) \
+            if all(np.size(v) == d_out for v in vals) else None
+        if A is None:
+            A = np.empty((n, d_out))
+            for i, v in enumerate(vals):
+                A[i] = np.broadcast_to(np.asarray(v, dtype=np.float64).reshape(-1), (d_out,))
+    if A.shape[1] != d_out:
+        A = np.ascontiguousarray(np.broadcast_to(A, (n, d_out)))
+    return A
+
+
+def gather_values(L: Layout, blocks: Blocks, node_gen, f, f_evals, vectorized=False) -> ValueTable:
+    """Evaluate f at the grid nodes (reusing f_evals) and map F's rows to the distinct nodes.
 
     f_evals follows the reference format (interpolation.py:141-228): for nested nodes a flat dict
     keyed by sparse node tuples ((dim, mu), ...) (mu > 0, dims ascending); for non-nested nodes a
-    dict keyed by the block's sparse multi-index, each holding such a node dict.
-    Returns (F[k_raw, d_out], offset[d_out], n_new).
+    dict keyed by the block's sparse multi-index, each holding such a node dict.  For nested
+    nodes V has one row per distinct grid node (blocks share them); for non-nested ones one row
+    per row of F.
     """
     d_in, d_out = L.d_in, L.d_out
     zero = np.array([g(0)[0] for g in node_gen], dtype=np.float64)
@@ -338,38 +369,46 @@ def assemble_values(L: Layout, blocks: Blocks, node_gen, f, f_evals, vectorized=
     def get(store, key, coords):
         nonlocal n_new
         if key in store:
-            return store[key]
+            return
         x = zero.copy()
         for dm, val in coords:
             x[dm] = val
         if vectorized:
             pending.append((store, key, x))
             store[key] = None
-            n_new += 1
-            return None
-        v = f(x)
-        store[key] = v
+        else:
+            store[key] = f(x)
         n_new += 1
-        return v
 
     # offset term (nu = 0), evaluated only when its coefficient is non-zero (interpolation.py:151)
     root_store = f_evals if L.nested else f_evals.setdefault((), {})
     if L.zeta0 != 0:
         get(root_store, (), [])
 
-    # grid node values per slot
     node_vals = [L.slot_nodes[s, : L.slot_deg[s] + 1] for s in range(len(L.slot_key))]
     slot_of = {k: s for s, k in enumerate(L.slot_key)}
-    entries = []  # (store, key) per F row in device order
+    row_node = np.empty(L.k_raw, dtype=np.int32)
+    uid = {}           # nested: node key -> row of V
+    src = []           # (store, key) per row of V
+    r = 0
     for pos, b in enumerate(L.order):
         fd, fg = L.blk_dims[pos], L.blk_degs[pos]
         nu = tuple((int(a), int(g)) for a, g in zip(fd, fg))
         store = f_evals if L.nested else f_evals.setdefault(nu, {})
         sl = [slot_of[(int(a), int(g))] for a, g in zip(fd, fg)]
         for mu, key in entry_keys(fd, fg):
-            coords = [(int(fd[i]), node_vals[sl[i]][mu[i]]) for i in range(len(fd))]
-            get(store, key, coords)
-            entries.append((store, key))
+            if key not in store:
+                get(store, key, [(int(fd[i]), node_vals[sl[i]][mu[i]]) for i in range(len(fd))])
+            if L.nested:
+                u = uid.get(key)
+                if u is None:
+                    u = uid[key] = len(src)
+                    src.append((store, key))
+            else:
+                u = len(src)
+                src.append((store, key))
+            row_node[r] = u
+            r += 1
 
     if vectorized and pending:
         X = np.stack([p[2] for p in pending])
@@ -377,17 +416,29 @@ def assemble_values(L: Layout, blocks: Blocks, node_gen, f, f_evals, vectorized=
         for (store, key, _), yv in zip(pending, Y):
             store[key] = yv
 
-    F = np.empty((L.k_raw, d_out), dtype=np.float64)
-    zrow = np.repeat(L.blk_zeta, np.diff(L.blk_foff)).astype(np.float64)
-    for r, (store, key) in enumerate(entries):
-        F[r] = np.broadcast_to(np.asarray(store[key], dtype=np.float64).reshape(-1), (d_out,))
-    F *= zrow[:, None]
+    V = stack_values([st[key] for st, key in src], d_out)
+    row_scale = np.repeat(L.blk_zeta, np.diff(L.blk_foff)).astype(np.float64)
     if L.zeta0 != 0:
         v = np.broadcast_to(np.asarray(root_store[()], dtype=np.float64).reshape(-1), (d_out,))
         offset = float(L.zeta0) * v
     else:
         offset = np.zeros(d_out)
-    return F, np.ascontiguousarray(offset, dtype=np.float64), n_new
+    return ValueTable(V=V, row_node=row_node, row_scale=row_scale,
+                      offset=np.ascontiguousarray(offset, dtype=np.float64), n_new=n_new)
+
+
+def pack_rows(V, row_node, row_scale):
+    """Host form of the device packing (sbi_pack_rows_f64): F[r] = row_scale[r] * V[row_node[r]],
+    zero where row_node is -1."""
+    Vz = np.concatenate([np.asarray(V, dtype=np.float64), np.zeros((1, V.shape[1]))])
+    return Vz[row_node] * np.asarray(row_scale, dtype=np.float64)[:, None]
+
+
+def assemble_values(L: Layout, blocks: Blocks, node_gen, f, f_evals, vectorized=False):
+    """The zeta-scaled F on the host (the device builds the same from gather_values' table with
+    sbi_pack_rows_f64).  Returns (F[k_raw, d_out], offset[d_out], n_new)."""
+    vt = gather_values(L, blocks, node_gen, f, f_evals, vectorized)
+    return pack_rows(vt.V, vt.row_node, vt.row_scale), vt.offset, vt.n_new
 
 
 @dataclass
@@ -494,17 +545,19 @@ def build_eval_program(L: Layout):
             return PROG_CYC_GROUP1
         return sum(cyc for cyc, _ in grp)
 
+    cntl, gel, runl = cnt.tolist(), gen_e.tolist(), run_of.tolist()
+
     def split(bound):
         """Greedy cut of the block sequence into waves of cost <= bound (segments in groups of
-        PROG_GROUP from each wave's first, as the kernel walks them); the cut list or None."""
+        PROG_GROUP from each wave's first, as the kernel walks them); the cut list or None.
+        split.last is the cost of the last wave."""
         cuts = [0]
         done, grp = 0.0, []                # cost of the wave's complete groups; the open group
         seg_c, seg_ge, run = [0] * nt, 0.0, -1
         for pos in range(nb):
-            new_seg = run_of[pos] != run
-            c = [int(v) for v in cnt[pos]] if new_seg else [seg_c[i] + int(cnt[pos, i])
-                                                             for i in range(nt)]
-            ge = (0.0 if new_seg else seg_ge) + gen_e[pos]
+            new_seg = runl[pos] != run
+            c = list(cntl[pos]) if new_seg else [a + b for a, b in zip(seg_c, cntl[pos])]
+            ge = (0.0 if new_seg else seg_ge) + gel[pos]
             info = (seg_cost(c, units[pos], ge), dl_of[pos] == 2 and c == [1, 0, 0] and ge == 0)
             if new_seg:
                 d2, g2 = (done + group_cost(grp), [info]) if len(grp) == PROG_GROUP else \
@@ -515,17 +568,24 @@ def build_eval_program(L: Layout):
                 cuts.append(pos)
                 if len(cuts) > N_WAVES:
                     return None
-                c, ge = [int(v) for v in cnt[pos]], gen_e[pos]
+                c, ge = list(cntl[pos]), gel[pos]
                 info = (seg_cost(c, units[pos], ge), dl_of[pos] == 2 and c == [1, 0, 0] and ge == 0)
                 d2, g2 = 0.0, [info]
             done, grp = d2, g2
-            seg_c, seg_ge, run = c, ge, run_of[pos]
+            seg_c, seg_ge, run = c, ge, runl[pos]
+        split.last = done + group_cost(grp)
         return cuts + [nb] * (N_WAVES + 1 - len(cuts))
 
-    lo, hi = 0.0, 1.0
+    # min-max bound by bisection, bracketed around an even share of the one-wave cost and
+    # narrowed to 2e-4 of it (a fraction of one record chunk's cost per wave)
+    split(float("inf"))
+    share = max(split.last / N_WAVES, 1.0)
+    lo, hi = 0.9 * share, 1.1 * share
     while split(hi) is None:
-        hi *= 2.0
-    for _ in range(30):
+        hi *= 1.25
+    while lo > 1e-9 and split(lo) is not None:
+        lo *= 0.5
+    while hi - lo > 2e-4 * hi:
         mid = 0.5 * (lo + hi)
         if split(mid) is None:
             lo = mid
@@ -612,3 +672,13 @@ def program_values(prog: EvalProgram, F: np.ndarray) -> np.ndarray:
     """The program's F stream: the device F (d_out = 1) gathered in record order, zeros at -1."""
     Ff = np.append(np.asarray(F, dtype=np.float64).reshape(-1), 0.0)
     return Ff[prog.f_index]
+
+
+def program_rows(prog: EvalProgram, row_node, row_scale):
+    """Row map of the program's F stream for sbi_pack_rows_f64: the F row map gathered in record
+    order, node -1 (a zero) at padding."""
+    fi = prog.f_index
+    pad = fi < 0
+    node = np.where(pad, -1, np.asarray(row_node)[np.where(pad, 0, fi)]).astype(np.int32)
+    scale = np.where(pad, 0.0, np.asarray(row_scale, dtype=np.float64)[np.where(pad, 0, fi)])
+    return node, scale
