@@ -154,3 +154,61 @@ def test_matrix_core_layout_matches_oracle(d, m):
     # the depth levels of the cost-ordered records are those of level_off
     depth = [len(v) for v in sorted({nu[:-1] for nu in nf.nus if len(nu) > 1}, key=len)]
     assert list(np.diff(L.level_off)) == [depth.count(dd) for dd in sorted(set(depth))]
+
+
+@pytest.mark.parametrize("d,m", [(1, 9), (4, 120), (10, 1000), (60, 3000)])
+def test_hierarchy_steps_program(d, m):
+    """The step program that sbi_hierarchize_f64 runs: every multi-index is a target once per
+    active dim, in ascending dim then level order; a target's sources are the multi-indices with
+    its level along the step's dim lowered to 0 .. lv - 1 (in that order); no step reads a row it
+    writes (the kernel's precondition); applied to the grid values it gives surpluses whose
+    Newton form interpolates them (checked at every grid node)."""
+    from smolyax_amd import indices as SI
+    from smolyax_amd import newton as NW
+    from smolyax_amd import nodes as SN
+
+    k = np.log((2 + np.arange(d)) / np.log(2))
+    t = OI.find_approximate_threshold(k, m, True)
+    nus = SI.indexset(k, t)
+    gen = SN.Leja(dim=d)
+    st = NW.hierarchy_steps(nus, gen)
+    assert st.step_off[0] == 0 and st.step_off[-1] == len(st.tgt) and len(st.src_off) == len(st.tgt) + 1
+    assert len(st.tgt) == sum(len(nu) for nu in nus)
+    assert st.src_off[-1] == len(st.src) == len(st.coef) == sum(v for nu in nus for _, v in nu)
+    prev = (-1, 0)
+    for s in range(st.n_steps):
+        tg = st.tgt[st.step_off[s]:st.step_off[s + 1]]
+        assert len(tg) > 0 and len(set(tg.tolist())) == len(tg)
+        srcs = set(st.src[st.src_off[st.step_off[s]]:st.src_off[st.step_off[s + 1]]].tolist())
+        assert not (srcs & set(tg.tolist()))
+        # one (dim, level) per step, in ascending order: the dim that the first source drops
+        i0 = st.step_off[s]
+        nu0, low0 = dict(nus[st.tgt[i0]]), dict(nus[st.src[st.src_off[i0]]])
+        dropped = set(nu0) - set(low0)
+        assert len(dropped) == 1
+        jd = dropped.pop()
+        key = (jd, nu0[jd])
+        assert key > prev
+        prev = key
+        for i in range(st.step_off[s], st.step_off[s + 1]):
+            nu = dict(nus[st.tgt[i]])
+            lv = nu[key[0]]
+            assert lv == key[1]
+            q0, q1 = st.src_off[i], st.src_off[i + 1]
+            assert q1 - q0 == lv
+            for l, q in enumerate(range(q0, q1)):
+                low = dict(nu)
+                if l == 0:
+                    low.pop(key[0])
+                else:
+                    low[key[0]] = l
+                assert nus[st.src[q]] == tuple(sorted(low.items()))
+    # interpolation at the grid nodes
+    rng = np.random.default_rng(d)
+    vals = rng.standard_normal((len(nus), 2))
+    s = NW.apply_steps(st, vals)
+    nf = NW.NewtonForm(k, t, gen, 2)
+    nf.s = s
+    zero = np.array([g(0)[0] for g in gen])
+    X = np.stack([nf.grid_point(nu, zero) for nu in nus])
+    assert np.max(np.abs(nf.evaluate(X) - vals)) < 1e-9 * max(1.0, np.max(np.abs(vals)))
