@@ -273,6 +273,15 @@ typedef struct sbi_newton_desc {
                                  4 s ..), step count, tile} */
   const int32_t* mwave_item;  /* [5] work-item ranges of the 4 wave groups of a point tile (a
                                  group of 4 waves takes 16 of the 64 points each) */
+  /* mixed GEMM path (sbi_newton_eval_mixed_gemm; optional: gemm_kq = gemm_head = 0 disables it).
+   * The basis rows in GEMM order: rows 0 .. gemm_head - 1 (the heaviest) in FP64, the rest FP32. */
+  int32_t gemm_head;          /* FP64 head rows */
+  int32_t gemm_kq;            /* FP32 rows basis_n - gemm_head rounded up to a multiple of 16 */
+  const int32_t* head_fac;    /* [16 * gemm_head] factors of the head rows (sbi_newton_rows_f64) */
+  const double* gemm_s64;     /* [gemm_head, ld_s64] FP64 surpluses of the head rows */
+  int64_t ld_s64;
+  const float* gemm_sq;       /* FP32 surpluses of the other rows, k-interleaved as in
+                                 sbi_contract_split_k4 ([gemm_kq / 4, d_out, 4], zero rows pad) */
 } sbi_newton_desc;
 
 typedef struct sbi_newton_plan sbi_newton_plan;
@@ -290,6 +299,17 @@ int sbi_newton_eval_mixed(const sbi_newton_plan* plan, const double* x, int64_t 
  * sums on the FP32 matrix cores), 1 the LDS-staged kernel (node tables fit in LDS), 0 the
  * streaming kernel that reads them from global memory. */
 int sbi_newton_mixed_staged(const sbi_newton_plan* plan);
+/* interpolation.py:264-304 __call__ for any d_out on the mixed path, in one call: the FP64 head
+ * rows of the basis (sbi_newton_rows_f64), the other rows in FP32 k-interleaved
+ * (sbi_newton_basis_rows dtype 3), then sbi_contract_split_k4_ws against the plan's gemm_s64 /
+ * gemm_sq: y[P, ldy] (FP64), the same bits as those three calls.  workspace: device memory of
+ * sbi_newton_eval_mixed_gemm_workspace(plan, P) bytes, 256-byte aligned, not shared with a
+ * concurrent call (basis matrices and, for small batches, the split contraction's partial tiles).
+ * P <= 65535 * 64 per call. */
+int64_t sbi_newton_eval_mixed_gemm_workspace(const sbi_newton_plan* plan, int64_t P);
+int sbi_newton_eval_mixed_gemm(const sbi_newton_plan* plan, const double* x, int64_t P,
+                               int64_t ldx, double* y, int64_t ldy, void* workspace,
+                               int64_t ws_bytes, void* stream);
 /* Same in FP64 throughout. */
 int sbi_newton_eval_f64(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
                         double* y, void* stream);
@@ -381,6 +401,20 @@ int sbi_contract_split_k4(const double* A64, int64_t lda64, const double* B64, i
                           int64_t K64, const float* A32, int64_t lda32, const float* B32,
                           int64_t ldb32, int64_t K32, const double* offset, double* Y,
                           int64_t ldy, int64_t P, int64_t n, void* stream);
+
+/* sbi_contract_split_k4 for small batches (the same contraction of interpolation.py:264-304, the
+ * same bits): when its P x n tiles are too few to fill the GPU, the k range is split over extra
+ * workgroups that write each 256-k partial tile (and the FP64 head) to `workspace`, and a second
+ * kernel adds them in sbi_contract_split_k4's order.  sbi_contract_split_k4_workspace gives the
+ * bytes it needs for a shape (0: no split, the workspace is not used; -1: invalid); with less
+ * (or NULL) the call runs sbi_contract_split_k4.  workspace: device memory, 256-byte aligned,
+ * not shared with a concurrent call. */
+int64_t sbi_contract_split_k4_workspace(int64_t P, int64_t n, int64_t K64, int64_t K32);
+int sbi_contract_split_k4_ws(const double* A64, int64_t lda64, const double* B64, int64_t ldb64,
+                             int64_t K64, const float* A32, int64_t lda32, const float* B32,
+                             int64_t ldb32, int64_t K32, const double* offset, double* Y,
+                             int64_t ldy, int64_t P, int64_t n, void* workspace, int64_t ws_bytes,
+                             void* stream);
 
 /* BF16 form of sbi_contract_split_k4 (gfx950 BF16 matrix cores): the K16 remainder rows of A and
  * B in BF16, k-interleaved by 8 (element (k, p) of A16 at A16[((k / 8) * lda16 + p) * 8 + k % 8],

@@ -66,6 +66,8 @@ class NewtonDesc(ctypes.Structure):
         ("rw_n", c_i32), ("_pad6", c_i32), ("rint_rec", c_vp), ("rint_h", c_vp),
         ("mt_n", c_i32), ("mt_steps", c_i32), ("mi_n", c_i32), ("n_xi_dims", c_i32),
         ("mt_a", c_vp), ("mt_grp", c_vp), ("mi_rec", c_vp), ("mwave_item", c_vp),
+        ("gemm_head", c_i32), ("gemm_kq", c_i32), ("head_fac", c_vp), ("gemm_s64", c_vp),
+        ("ld_s64", c_i64), ("gemm_sq", c_vp),
     ]
 
 
@@ -113,6 +115,13 @@ SIGNATURES = {
     "sbi_newton_rows_f64": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i32, c_vp, c_i64, c_vp]),
     "sbi_contract_split_k4": (c_i32, [c_vp, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64,
                                       c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp]),
+    "sbi_contract_split_k4_workspace": (c_i64, [c_i64, c_i64, c_i64, c_i64]),
+    "sbi_newton_eval_mixed_gemm_workspace": (c_i64, [c_vp, c_i64]),
+    "sbi_newton_eval_mixed_gemm": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64,
+                                           c_vp]),
+    "sbi_contract_split_k4_ws": (c_i32, [c_vp, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64,
+                                         c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64,
+                                         c_vp]),
     "sbi_contract_bf16_k8": (c_i32, [c_vp, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64,
                                      c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp]),
     "sbi_contract_split": (c_i32, [c_vp, c_i64, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64,

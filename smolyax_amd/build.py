@@ -61,12 +61,13 @@ def build(verbose=True):
 # code of the .hip sources is built as usual and never runs in the check.
 SANITIZE = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined"]
 CHECK_SOURCES = [("indices.cpp", True), ("host_check.cpp", True), ("bary.hip", False),
-                 ("newton.hip", False)]
+                 ("newton.hip", False), ("basis.hip", False), ("gemm.hip", False)]
 
 
 def build_host_check(verbose=True):
-    """Build smolyax_amd/_build/asan/host_check (indices.cpp, the plan entry points of bary.hip
-    and newton.hip, and the driver csrc/host_check.cpp) with ASan + UBSan on the host side."""
+    """Build smolyax_amd/_build/asan/host_check (indices.cpp, the plan entry points of bary.hip,
+    newton.hip, basis.hip and gemm.hip, and the driver csrc/host_check.cpp) with ASan + UBSan on
+    the host side."""
     objdir = os.path.join(HERE, "_build", "asan")
     os.makedirs(objdir, exist_ok=True)
     base = ["-O1", "-g", "-fno-omit-frame-pointer", "-std=c++17", "-Wall", "-Wno-unused-function"]

@@ -292,19 +292,24 @@ __global__ __launch_bounds__(kTile* kHWaves) void newton_basis_direct_kernel(
 
 // Rows of H^T computed directly in FP64 from the coordinates (no factor table): each row is a
 // product of at most 8 factors h_k(x_j), given as (dim, deg) pairs (dim < 0: none).  Used for the
-// few heaviest rows, which the mixed contraction keeps in FP64.
+// few heaviest rows, which the mixed contraction keeps in FP64.  A workgroup covers 64 points
+// (one per lane) and kRowsPerBlock rows, one wave per fourth row, so that small batches still
+// spread over many workgroups; each element is computed on its own (same bits for any P).
+constexpr int kRowsPerBlock = 16;
 __global__ __launch_bounds__(256) void newton_rows_f64_kernel(sbi_newton_desc d,
                                                              const double* __restrict__ x,
                                                              int64_t P, int64_t ldx,
                                                              const int32_t* __restrict__ fac,
                                                              int n_rows, double* __restrict__ HT,
                                                              int64_t ldh) {
-  const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t p = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+  const int wave = threadIdx.x >> 6;
   if (p >= P) return;
   const double* __restrict__ xp = x + p * ldx;
   cptr<int32_t> f = C(fac);
   cptr<double> dim_xi = C(d.dim_xi), dim_c = C(d.dim_c);
-  for (int r = 0; r < n_rows; ++r) {
+  const int r1 = min(n_rows, (int)(blockIdx.y + 1) * kRowsPerBlock);
+  for (int r = (int)blockIdx.y * kRowsPerBlock + wave; r < r1; r += 4) {
     double v = 1.0;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -807,15 +812,73 @@ extern "C" int sbi_newton_rows_f64(const sbi_newton_plan* plan, const double* x,
                                    int64_t ldh, void* stream) {
   const sbi_newton_desc* dp = nullptr;
   if (sbi_newton_plan_desc(plan, &dp) != SBI_OK || !x || (n_rows > 0 && (!fac || !HT)) ||
-      P < 0 || ldh < P || n_rows < 0 || ldx < dp->d_in) {
+      P < 0 || ldh < P || n_rows < 0 || n_rows > 65535 * kRowsPerBlock || ldx < dp->d_in) {
     sbi::set_error("sbi_newton_rows_f64: invalid argument");
     return SBI_ERR_INVALID;
   }
   if (P == 0 || n_rows == 0) return SBI_OK;
-  hipLaunchKernelGGL(newton_rows_f64_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0,
-                     (hipStream_t)stream, *dp, x, P, ldx, fac, n_rows, HT, ldh);
+  if ((P + 63) / 64 > 0x7fffffff) {
+    sbi::set_error("sbi_newton_rows_f64: too many points for one call");
+    return SBI_ERR_INVALID;
+  }
+  const dim3 grid((unsigned)((P + 63) / 64), (unsigned)((n_rows + kRowsPerBlock - 1) / kRowsPerBlock));
+  hipLaunchKernelGGL(newton_rows_f64_kernel, grid, dim3(256), 0, (hipStream_t)stream, *dp, x, P,
+                     ldx, fac, n_rows, HT, ldh);
   SBI_HIP_CHECK(hipGetLastError());
   return SBI_OK;
+}
+
+static int64_t align256(int64_t v) { return (v + 255) / 256 * 256; }
+
+extern "C" int64_t sbi_newton_eval_mixed_gemm_workspace(const sbi_newton_plan* plan, int64_t P) {
+  const sbi_newton_desc* dp = nullptr;
+  if (sbi_newton_plan_desc(plan, &dp) != SBI_OK || P < 0 || dp->gemm_head + dp->gemm_kq <= 0) {
+    sbi::set_error("sbi_newton_eval_mixed_gemm_workspace: invalid argument (or the plan has no "
+                   "mixed GEMM fields)");
+    return -1;
+  }
+  const int64_t sk = sbi_contract_split_k4_workspace(P, dp->d_out, dp->gemm_head, dp->gemm_kq);
+  if (sk < 0) return -1;
+  return align256((int64_t)dp->gemm_kq * P * 4) +
+         align256((int64_t)(dp->gemm_head > 0 ? dp->gemm_head : 1) * P * 8) + sk;
+}
+
+extern "C" int sbi_newton_eval_mixed_gemm(const sbi_newton_plan* plan, const double* x, int64_t P,
+                                          int64_t ldx, double* y, int64_t ldy, void* workspace,
+                                          int64_t ws_bytes, void* stream) {
+  const sbi_newton_desc* dp = nullptr;
+  if (sbi_newton_plan_desc(plan, &dp) != SBI_OK || !x || !y || P < 0 || ldx < dp->d_in ||
+      ldy < dp->d_out || dp->gemm_head + dp->gemm_kq <= 0 || !workspace ||
+      ((uintptr_t)workspace & 255) || P > (int64_t)65535 * 64) {
+    sbi::set_error("sbi_newton_eval_mixed_gemm: invalid argument (or the plan has no mixed GEMM "
+                   "fields, or the workspace is not 256-byte aligned)");
+    return SBI_ERR_INVALID;
+  }
+  if (P == 0) return SBI_OK;
+  const sbi_newton_desc& d = *dp;
+  const int64_t ht = align256((int64_t)d.gemm_kq * P * 4);
+  const int64_t h64 = align256((int64_t)(d.gemm_head > 0 ? d.gemm_head : 1) * P * 8);
+  if (ws_bytes < ht + h64) {
+    sbi::set_error("sbi_newton_eval_mixed_gemm: workspace smaller than "
+                   "sbi_newton_eval_mixed_gemm_workspace");
+    return SBI_ERR_INVALID;
+  }
+  char* w = (char*)workspace;
+  float* HT = (float*)w;
+  double* H64 = (double*)(w + ht);
+  int rc = SBI_OK;
+  if (d.basis_n > d.gemm_head) {
+    rc = sbi_newton_basis_rows(plan, x, P, ldx, d.gemm_head, d.basis_n, HT, P, 3, stream);
+    if (rc != SBI_OK) return rc;
+  }
+  if (d.gemm_head > 0) {
+    rc = sbi_newton_rows_f64(plan, x, P, ldx, d.head_fac, d.gemm_head, H64, P, stream);
+    if (rc != SBI_OK) return rc;
+  }
+  return sbi_contract_split_k4_ws(H64, P, d.gemm_s64, d.ld_s64, d.gemm_head,
+                                  d.gemm_kq ? HT : nullptr, P, d.gemm_kq ? d.gemm_sq : nullptr,
+                                  d.d_out, d.gemm_kq, nullptr, y, ldy, P, d.d_out,
+                                  w + ht + h64, ws_bytes - ht - h64, stream);
 }
 
 static bool jac_global_table(const sbi_newton_desc& d);

@@ -260,6 +260,284 @@ __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// Split-k form of contract_kernel for small batches: the kFlushK-chunks of k are spread over
+// blockIdx.z (chunks [z * cps, (z + 1) * cps) per slice) and slice z = nsplit computes the FP64
+// head; every chunk's partial tile (the MFMA accumulators of its kFlushK terms, exactly what
+// contract_kernel adds into its FP64 sums at that flush) is written to `part`, the head to
+// `head`, and contract_reduce_kernel adds them in contract_kernel's order: the same bits.
+// ------------------------------------------------------------------------------------------
+template <typename T, int FN, bool kK4>
+__global__ __launch_bounds__(64 * kWaves) void contract_split_kernel(
+    const T* __restrict__ A, int64_t lda, const T* __restrict__ B, int64_t ldb, int64_t P,
+    int64_t n, int64_t K, const double* __restrict__ A64, int64_t lda64,
+    const double* __restrict__ B64, int64_t ldb64, int64_t K64, int64_t cps, int nsplit,
+    T* __restrict__ part, double* __restrict__ head) {
+  using AccT = typename Acc<T>::type;  // partial tiles are stored as T (the accumulator's type)
+  constexpr bool kF32 = sizeof(T) == 4;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int li = lane & 15, g = lane >> 4;
+  const int64_t m_base = (int64_t)blockIdx.x * kBlockM + wm * 32;
+  const int64_t n_base = (int64_t)blockIdx.y * (32 * FN) + wn * (16 * FN);
+  const int64_t Pn = P * n;
+  int64_t pcol[2], jcol[FN];
+#pragma unroll
+  for (int fm = 0; fm < 2; ++fm) pcol[fm] = min(m_base + fm * 16 + li, P - 1);
+#pragma unroll
+  for (int fn = 0; fn < FN; ++fn) jcol[fn] = min(n_base + fn * 16 + li, n - 1);
+  // element (fm, fn, r) of this lane: row and column of Y (FP32 layout for FP32 operands)
+  auto out_at = [&](int fm, int fn, int r, int64_t& row, int64_t& col) {
+    row = m_base + fm * 16 + (kF32 ? 4 * g + r : g + 4 * r);
+    col = n_base + fn * 16 + li;
+  };
+
+  if ((int)blockIdx.z == nsplit) {  // the FP64 head (FP32 operands only)
+    if constexpr (kF32) {
+      int64_t ph[2];
+#pragma unroll
+      for (int fm = 0; fm < 2; ++fm)
+        ph[fm] = min(m_base + fm * 16 + 4 * (li & 3) + (li >> 2), P - 1);
+      v4d h[2][FN];
+#pragma unroll
+      for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn) h[fm][fn] = v4d{0, 0, 0, 0};
+      for (int64_t k0 = 0; k0 < K64; k0 += 4) {
+        const int64_t k = k0 + g;
+        const bool in = k < K64;
+        const int64_t kc = in ? k : K64 - 1;
+        double ha[2], hb[FN];
+#pragma unroll
+        for (int fm = 0; fm < 2; ++fm) {
+          const double v = A64[kc * lda64 + ph[fm]];
+          ha[fm] = in ? v : 0.0;
+        }
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn) {
+          const double v = B64[kc * ldb64 + jcol[fn]];
+          hb[fn] = in ? v : 0.0;
+        }
+#pragma unroll
+        for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+          for (int fn = 0; fn < FN; ++fn) h[fm][fn] = mfma(ha[fm], hb[fn], h[fm][fn]);
+      }
+#pragma unroll
+      for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            int64_t row, col;
+            out_at(fm, fn, r, row, col);
+            if (row < P && col < n) head[row * n + col] = h[fm][fn][r];
+          }
+    }
+    return;
+  }
+
+  const int64_t nch = (K + kFlushK - 1) / kFlushK;
+  const int64_t c0 = (int64_t)blockIdx.z * cps, c1 = min(nch, c0 + cps);
+  if (c0 >= c1) return;
+  const int64_t K16 = K & ~(int64_t)(kChunk - 1);
+  AccT acc[2][FN];
+#pragma unroll
+  for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+    for (int fn = 0; fn < FN; ++fn) acc[fm][fn] = AccT{0, 0, 0, 0};
+  int64_t cur = c0;  // chunk whose partial tile is accumulating
+  auto emit = [&]() {
+#pragma unroll
+    for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+      for (int fn = 0; fn < FN; ++fn) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int64_t row, col;
+          out_at(fm, fn, r, row, col);
+          if (row < P && col < n) part[cur * Pn + row * n + col] = acc[fm][fn][r];
+        }
+        acc[fm][fn] = AccT{0, 0, 0, 0};
+      }
+    ++cur;
+  };
+
+  T a[2][4], b[FN][4];
+  typedef T v4t __attribute__((ext_vector_type(4)));
+  auto load = [&](int64_t k0) {
+    if constexpr (kK4) {
+      const int64_t kq = (k0 >> 2) + g;
+#pragma unroll
+      for (int fm = 0; fm < 2; ++fm) {
+        const v4t v = *reinterpret_cast<const v4t*>(A + (kq * lda + pcol[fm]) * 4);
+        a[fm][0] = v.x;
+        a[fm][1] = v.y;
+        a[fm][2] = v.z;
+        a[fm][3] = v.w;
+      }
+#pragma unroll
+      for (int fn = 0; fn < FN; ++fn) {
+        const v4t v = *reinterpret_cast<const v4t*>(B + (kq * ldb + jcol[fn]) * 4);
+        b[fn][0] = v.x;
+        b[fn][1] = v.y;
+        b[fn][2] = v.z;
+        b[fn][3] = v.w;
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int64_t k = k0 + 4 * g + s;
+        const T* __restrict__ ar = A + k * lda;
+        const T* __restrict__ br = B + k * ldb;
+#pragma unroll
+        for (int fm = 0; fm < 2; ++fm) a[fm][s] = ar[pcol[fm]];
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn) b[fn][s] = br[jcol[fn]];
+      }
+    }
+  };
+
+  const int64_t kb = c0 * kFlushK, ke = min(c1 * kFlushK, K16);
+  if (kb < ke) {
+    load(kb);
+    for (int64_t k0 = kb; k0 < ke; k0 += kChunk) {
+      T ca[2][4], cb[FN][4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+#pragma unroll
+        for (int fm = 0; fm < 2; ++fm) ca[fm][s] = a[fm][s];
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn) cb[fn][s] = b[fn][s];
+      }
+      if (k0 + kChunk < ke) load(k0 + kChunk);
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+          for (int fn = 0; fn < FN; ++fn)
+            acc[fm][fn] = mfma(ca[fm][s], cb[fn][s], acc[fm][fn]);
+      if (((k0 + kChunk) % kFlushK) == 0) emit();
+    }
+  }
+  if (!kK4 && c1 == nch && K16 < K) {  // last partial chunk of 16: rows k >= K contribute zero
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int64_t k = K16 + 4 * g + s;
+      const bool in = k < K;
+      const int64_t kc = in ? k : K - 1;
+#pragma unroll
+      for (int fm = 0; fm < 2; ++fm) {
+        const T v = A[kc * lda + pcol[fm]];
+        a[fm][s] = in ? v : T(0);
+      }
+#pragma unroll
+      for (int fn = 0; fn < FN; ++fn) {
+        const T v = B[kc * ldb + jcol[fn]];
+        b[fn][s] = in ? v : T(0);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn) acc[fm][fn] = mfma(a[fm][s], b[fn][s], acc[fm][fn]);
+  }
+  if (cur < c1) emit();  // the slice's last chunk when K ends inside it
+}
+
+// Y[p, j] = offset[j] + (((head + part_0) + part_1) + ...) (+ 0.0 when K is a multiple of
+// kFlushK: contract_kernel's final flush then adds an empty partial tile), head = 0 without one.
+template <typename E>
+__global__ __launch_bounds__(256) void contract_reduce_kernel(const E* __restrict__ part,
+                                                              int64_t nch,
+                                                              const double* __restrict__ head,
+                                                              const double* __restrict__ offset,
+                                                              double* __restrict__ Y, int64_t ldy,
+                                                              int64_t P, int64_t n, int64_t K) {
+  const int64_t Pn = P * n;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < Pn;
+       i += (int64_t)gridDim.x * 256) {
+    const int64_t row = i / n, col = i - row * n;
+    double v = head ? head[i] : 0.0;
+    for (int64_t c = 0; c < nch; ++c) v += (double)part[c * Pn + i];
+    if (K % kFlushK == 0) v += 0.0;
+    const double off = offset ? offset[col] : 0.0;
+    Y[row * ldy + col] = off + v;
+  }
+}
+
+// Split plan for a contraction of P points x n outputs: the slices of k (0: no split) and chunks
+// per slice.  A launch of fewer than kSplitMinGroups workgroups is split so that about
+// kSplitTarget workgroups run (small batches: the k loop of one workgroup is the latency).
+constexpr int64_t kSplitMinGroups = 256;
+constexpr int64_t kSplitTarget = 512;
+constexpr int64_t kSplitMaxBytes = int64_t(256) << 20;  // workspace of one split call
+struct SplitPlan {
+  int64_t nsplit = 0, cps = 0, nch = 0, part_bytes = 0, head_bytes = 0;
+};
+template <typename T>
+SplitPlan split_plan(int64_t P, int64_t n, int64_t K, int64_t K64) {
+  SplitPlan sp;
+  const int64_t gm = (P + kBlockM - 1) / kBlockM;
+  const int64_t tn = n <= 32 ? 32 : n <= 64 ? 64 : 128;
+  const int64_t groups = gm * ((n + tn - 1) / tn);
+  sp.nch = (K + kFlushK - 1) / kFlushK;
+  if (P <= 0 || n <= 0 || groups >= kSplitMinGroups || (sp.nch < 2 && K64 == 0)) return sp;
+  int64_t want = (kSplitTarget + groups - 1) / groups;
+  if (want > sp.nch) want = sp.nch;
+  if (want < 1) want = 1;
+  sp.cps = sp.nch > 0 ? (sp.nch + want - 1) / want : 0;
+  sp.nsplit = sp.cps > 0 ? (sp.nch + sp.cps - 1) / sp.cps : 0;
+  if (sp.nsplit + (K64 > 0 ? 1 : 0) < 2) {  // one slice: nothing to gain
+    sp = SplitPlan{};
+    return sp;
+  }
+  sp.part_bytes = ((sp.nch * P * n * (int64_t)sizeof(T)) + 255) /
+                  256 * 256;
+  sp.head_bytes = K64 > 0 ? P * n * 8 : 0;
+  if (sp.part_bytes + sp.head_bytes > kSplitMaxBytes || sp.nsplit >= 65535) sp = SplitPlan{};
+  return sp;
+}
+
+template <typename T, bool kK4>
+int contract_split_launch(const SplitPlan& sp, const T* A, int64_t lda, const T* B, int64_t ldb,
+                          const double* offset, double* Y, int64_t ldy, int64_t P, int64_t n,
+                          int64_t K, const double* A64, int64_t lda64, const double* B64,
+                          int64_t ldb64, int64_t K64, void* ws, hipStream_t st) {
+  using E = T;
+  E* part = (E*)ws;
+  double* head = K64 > 0 ? (double*)((char*)ws + sp.part_bytes) : nullptr;
+  const dim3 block(64 * kWaves);
+  const unsigned gm = (unsigned)((P + kBlockM - 1) / kBlockM);
+  const unsigned gz = (unsigned)(sp.nsplit + (K64 > 0 ? 1 : 0));
+  const int ns = (int)sp.nsplit;
+  if (n <= 32) {
+    hipLaunchKernelGGL((contract_split_kernel<T, 1, kK4>), dim3(gm, (unsigned)((n + 31) / 32), gz),
+                       block, 0, st, A, lda, B, ldb, P, n, K, A64, lda64, B64, ldb64, K64, sp.cps,
+                       ns, part, head);
+  } else if (n <= 64) {
+    hipLaunchKernelGGL((contract_split_kernel<T, 2, kK4>), dim3(gm, (unsigned)((n + 63) / 64), gz),
+                       block, 0, st, A, lda, B, ldb, P, n, K, A64, lda64, B64, ldb64, K64, sp.cps,
+                       ns, part, head);
+  } else {
+    hipLaunchKernelGGL((contract_split_kernel<T, 4, kK4>),
+                       dim3(gm, (unsigned)((n + 127) / 128), gz), block, 0, st, A, lda, B, ldb, P,
+                       n, K, A64, lda64, B64, ldb64, K64, sp.cps, ns, part, head);
+  }
+  SBI_HIP_CHECK(hipGetLastError());
+  const int64_t Pn = P * n;
+  int64_t rg = (Pn + 255) / 256;
+  if (rg > 4096) rg = 4096;
+  hipLaunchKernelGGL((contract_reduce_kernel<E>), dim3((unsigned)rg), dim3(256), 0, st, part,
+                     sp.nch, head, offset, Y, ldy, P, n, K);
+  SBI_HIP_CHECK(hipGetLastError());
+  return SBI_OK;
+}
+
 template <typename T, bool kK4 = false>
 int contract_launch(const T* A, int64_t lda, const T* B, int64_t ldb, const double* offset,
                     double* Y, int64_t ldy, int64_t P, int64_t n, int64_t K, const double* A64,
@@ -627,6 +905,42 @@ extern "C" int sbi_contract_split_k4(const double* A64, int64_t lda64, const dou
   if (P == 0 || n == 0) return SBI_OK;
   return contract_launch<float, true>(A32, lda32, B32, ldb32, offset, Y, ldy, P, n, K32, A64,
                                       lda64, B64, ldb64, K64, (hipStream_t)stream);
+}
+
+extern "C" int64_t sbi_contract_split_k4_workspace(int64_t P, int64_t n, int64_t K64,
+                                                   int64_t K32) {
+  if (P < 0 || n < 0 || K64 < 0 || K32 < 0 || K32 % 16 != 0) {
+    sbi::set_error("sbi_contract_split_k4_workspace: invalid argument");
+    return -1;
+  }
+  const SplitPlan sp = split_plan<float>(P, n, K32, K64);
+  return sp.nsplit ? sp.part_bytes + sp.head_bytes : 0;
+}
+
+extern "C" int sbi_contract_split_k4_ws(const double* A64, int64_t lda64, const double* B64,
+                                        int64_t ldb64, int64_t K64, const float* A32,
+                                        int64_t lda32, const float* B32, int64_t ldb32,
+                                        int64_t K32, const double* offset, double* Y, int64_t ldy,
+                                        int64_t P, int64_t n, void* workspace, int64_t ws_bytes,
+                                        void* stream) {
+  if (P < 0 || n < 0 || K64 < 0 || K32 < 0 || K32 % 16 != 0 || ws_bytes < 0 ||
+      (P > 0 && n > 0 && (!Y || ldy < n)) ||
+      (P > 0 && n > 0 && K64 > 0 && (!A64 || !B64 || lda64 < P || ldb64 < n)) ||
+      (P > 0 && n > 0 && K32 > 0 &&
+       (!A32 || !B32 || lda32 < P || ldb32 < n || ((uintptr_t)A32 & 15) || ((uintptr_t)B32 & 15))) ||
+      ((uintptr_t)workspace & 255) || P > (int64_t)65535 * kBlockM || n > (int64_t)65535 * 32) {
+    sbi::set_error("sbi_contract_split_k4_ws: invalid argument (K32 must be a multiple of 16, the "
+                   "k-interleaved operands 16-byte aligned, the workspace 256-byte aligned)");
+    return SBI_ERR_INVALID;
+  }
+  if (P == 0 || n == 0) return SBI_OK;
+  const SplitPlan sp = split_plan<float>(P, n, K32, K64);
+  if (sp.nsplit == 0 || !workspace || ws_bytes < sp.part_bytes + sp.head_bytes)
+    return contract_launch<float, true>(A32, lda32, B32, ldb32, offset, Y, ldy, P, n, K32, A64,
+                                        lda64, B64, ldb64, K64, (hipStream_t)stream);
+  return contract_split_launch<float, true>(sp, A32, lda32, B32, ldb32, offset, Y, ldy, P, n,
+                                            K32, A64, lda64, B64, ldb64, K64, workspace,
+                                            (hipStream_t)stream);
 }
 
 extern "C" int sbi_contract_split(const double* A64, int64_t lda64, const double* B64,

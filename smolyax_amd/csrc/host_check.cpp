@@ -137,7 +137,21 @@ void plan_paths() {
   sbi_newton_plan* np = nullptr;
   expect(sbi_newton_plan_create(&n, &np) == SBI_OK && np, "newton plan");
   expect(sbi_newton_mixed_staged(np) == 0, "newton plan without staged tables streams");
+  expect(sbi_newton_eval_mixed_gemm_workspace(np, 64) == -1, "no GEMM fields: no workspace size");
   expect(sbi_newton_plan_destroy(np) == SBI_OK, "newton plan destroy");
+  n.gemm_kq = 16;  // basis_n = 0: no FP32 rows to pad
+  expect(sbi_newton_plan_create(&n, &np) == SBI_ERR_INVALID, "newton plan with stray GEMM rows");
+  n.basis_n = 20;
+  n.gemm_head = 4;
+  n.head_fac = (const int32_t*)p;
+  n.gemm_s64 = (const double*)p;
+  n.ld_s64 = 1;
+  n.gemm_sq = (const float*)p;
+  expect(sbi_newton_plan_create(&n, &np) == SBI_OK && np, "newton plan with GEMM fields");
+  expect(sbi_newton_eval_mixed_gemm_workspace(np, 64) >= 16 * 64 * 4 + 4 * 64 * 8,
+         "GEMM workspace holds both basis parts");
+  expect(sbi_newton_plan_destroy(np) == SBI_OK, "newton plan with GEMM fields destroy");
+  n.gemm_kq = n.gemm_head = 0;
   n.n_split = 4;
   expect(sbi_newton_plan_create(&n, &np) == SBI_ERR_INVALID, "newton plan with 4 waves");
 }

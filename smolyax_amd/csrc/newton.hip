@@ -708,6 +708,17 @@ extern "C" int sbi_newton_plan_create(const sbi_newton_desc* desc, sbi_newton_pl
     sbi::set_error("sbi_newton_plan_create: n_split must be 16");
     return SBI_ERR_INVALID;
   }
+  if (d.gemm_head < 0 || d.gemm_kq < 0 || d.gemm_kq % 16 != 0 ||
+      (d.gemm_kq > 0 && !d.gemm_sq) ||
+      (d.gemm_head > 0 && (!d.head_fac || !d.gemm_s64 || d.ld_s64 < d.d_out)) ||
+      (d.gemm_head + d.gemm_kq > 0 && (d.gemm_head > d.basis_n ||
+                                        d.gemm_kq < d.basis_n - d.gemm_head ||
+                                        d.gemm_kq >= d.basis_n - d.gemm_head + 16))) {
+    sbi::set_error("sbi_newton_plan_create: invalid mixed GEMM fields (gemm_kq must be basis_n - "
+                   "gemm_head rounded up to a multiple of 16, with gemm_sq; gemm_head rows need "
+                   "head_fac and gemm_s64)");
+    return SBI_ERR_INVALID;
+  }
   const size_t red = (size_t)kWaves * kTile * sizeof(double);
   const size_t t32 = (size_t)(d.n_internal + 1) * kTile * sizeof(float);
   const size_t t64 = (size_t)(d.n_internal + 1) * kTile * sizeof(double);

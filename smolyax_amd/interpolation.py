@@ -419,6 +419,11 @@ class SmolyakBarycentricInterpolator:
         self._S = {"f64": _dev_tensor(self._nform.s, f64, dev)}
         self._Sg = {"f64": _dev_tensor(self._nform.s[NL.basis_order], f64, dev)}
         self._head_fac = _dev_tensor(NL.head_fac, i32, dev)
+        # the mixed GEMM path's operands, for sbi_newton_eval_mixed_gemm
+        nh = NL.n_head
+        t["head_fac"] = self._head_fac
+        t["gemm_s64"] = self._Sg["f64"][:nh]
+        t["gemm_sq"] = self._surpluses_k4(nh)
         self._ndev = t
         d = N.NewtonDesc()
         d.d_in, d.d_out = self._d_in, self._d_out
@@ -428,6 +433,8 @@ class SmolyakBarycentricInterpolator:
         d.basis_n, d.n_tab_dims, d.n_tab_rows = NL.basis_n, NL.n_tab_dims, NL.n_tab_rows
         d.n_vjp_ent = NL.n_vjp_ent
         d.n_jac_units, d.n_jac_partials, d.n_jac_red = NL.n_jac_units, NL.n_jac_partials, NL.n_jac_red
+        d.gemm_head, d.gemm_kq = nh, t["gemm_sq"].shape[0] * 4
+        d.ld_s64 = max(t["gemm_s64"].stride(0), self._d_out)
         if "mt_a" in t:
             d.rw_n = NL.rw_n
             d.mt_n, d.mt_steps, d.mi_n = NL.mt_n, NL.mt_steps, NL.mi_n
@@ -644,8 +651,9 @@ class SmolyakBarycentricInterpolator:
     def _eval_newton_gemm(self, x, precision):
         """y = H S: Newton basis matrix (HIP kernel), then the contraction with the surpluses on
         the matrix cores: sbi_contract for FP64; for "mixed" (FP32) and "bf16" the heaviest rows
-        in FP64 (sbi_newton_rows_f64, FP64 MFMA) and the rest in FP32 (sbi_contract_split_k4) or
-        BF16 (sbi_contract_bf16_k8), k-interleaved operands, FP64 accumulation."""
+        in FP64 (sbi_newton_rows_f64, FP64 MFMA) and the rest in FP32 or BF16, k-interleaved
+        operands, FP64 accumulation ("mixed": all three steps in one call,
+        sbi_newton_eval_mixed_gemm, which splits k over more workgroups for small batches)."""
         P = x.shape[0]
         Nn = self._nlayout.basis_n
         kind = {"newton-fp64": "f64", "mixed": "f32", "bf16": "bf16"}[precision]
@@ -663,23 +671,32 @@ class SmolyakBarycentricInterpolator:
                                                       st), "basis")
                 self._contract(HT, S, None, y[a:b])
             return y
-        # FP64 head rows, FP32 / BF16 remainder in the k-interleaved layout (16-byte operand loads)
+        if kind == "f32":  # one call per chunk: head rows, FP32 rows, contraction
+            pc = self._point_chunk(self._ndesc.gemm_kq + 2 * self._ndesc.gemm_head, 4, P)
+            for a in range(0, P, pc):
+                b = min(P, a + pc)
+                xc, yc = x[a:b], y[a:b]
+                ws = N.fn["sbi_newton_eval_mixed_gemm_workspace"](self._nplan, b - a)
+                W = torch.empty(max(ws, 1), dtype=torch.uint8, device=self._device)
+                N.check(N.fn["sbi_newton_eval_mixed_gemm"](self._nplan, xc.data_ptr(), b - a,
+                                                           xc.stride(0), yc.data_ptr(),
+                                                           yc.stride(0), W.data_ptr(), ws, st),
+                        "mixed")
+            return y
+        # FP64 head rows, BF16 remainder in the k-interleaved layout (16-byte operand loads)
         nh = self._nlayout.n_head
         S64 = self._surpluses("f64", gemm=True)[:nh]
-        if kind == "f32":
-            SQ, q, dt, code, fn = self._surpluses_k4(nh), 4, torch.float32, 3, "sbi_contract_split_k4"
-        else:
-            SQ, q, dt, code, fn = self._surpluses_k8(nh), 8, torch.bfloat16, 4, "sbi_contract_bf16_k8"
-        KQ = SQ.shape[0] * q
-        pc = self._point_chunk(KQ + 2 * nh, torch.finfo(dt).bits // 8, P)
+        SQ = self._surpluses_k8(nh)
+        KQ = SQ.shape[0] * 8
+        pc = self._point_chunk(KQ + 2 * nh, 2, P)
         for a in range(0, P, pc):
             b = min(P, a + pc)
             xc = x[a:b]
-            HT = torch.empty((KQ // q, b - a, q), dtype=dt, device=self._device)
+            HT = torch.empty((KQ // 8, b - a, 8), dtype=torch.bfloat16, device=self._device)
             if Nn > nh:
                 N.check(N.fn["sbi_newton_basis_rows"](self._nplan, xc.data_ptr(), b - a,
                                                       xc.stride(0), nh, Nn, HT.data_ptr(), b - a,
-                                                      code, st), "basis")
+                                                      4, st), "basis")
             H64 = torch.empty((max(nh, 1), b - a), dtype=torch.float64, device=self._device)
             if nh:
                 N.check(N.fn["sbi_newton_rows_f64"](self._nplan, xc.data_ptr(), b - a,
@@ -687,7 +704,7 @@ class SmolyakBarycentricInterpolator:
                                                     nh, H64.data_ptr(), b - a,
                                                     st), "rows")
             yc = y[a:b]
-            N.check(N.fn[fn](
+            N.check(N.fn["sbi_contract_bf16_k8"](
                 H64.data_ptr(), H64.stride(0), S64.data_ptr(), max(S64.stride(0), self._d_out),
                 nh, HT.data_ptr() if KQ else None, b - a, SQ.data_ptr() if KQ else None,
                 self._d_out, KQ, None, yc.data_ptr(), yc.stride(0), b - a, self._d_out,

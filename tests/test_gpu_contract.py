@@ -170,3 +170,41 @@ def test_contract_split_k4_equals_plain_layout(gpu, P, n, K64, K32):
         Y1.data_ptr(), n, P, n, st), "contract_k4")
     torch.cuda.synchronize()
     assert torch.equal(Y0, Y1)
+
+
+@pytest.mark.parametrize("P,n,K64,K32,split", [(64, 761, 256, 3792, True), (256, 43, 0, 1264, True),
+                                               (200, 1, 64, 512, True), (1000, 1024, 256, 2048, True),
+                                               (33, 100, 7, 16, True), (5000, 1024, 64, 1024, False),
+                                               (64, 64, 0, 256, False), (7, 5, 300, 0, False)])
+def test_contract_split_k4_ws_same_bits(gpu, P, n, K64, K32, split):
+    """Small batches split k over extra workgroups (256-k partial tiles and the FP64 head in the
+    workspace, added in the unsplit order by a second kernel): the same bits as
+    sbi_contract_split_k4, for ragged P and n, K a multiple of 256 or not, with and without a
+    head; shapes that fill the GPU (or have one chunk) need no workspace and run unsplit."""
+    print(f"split_k4_ws P={P} n={n} K64={K64} K32={K32}", flush=True)
+    g = torch.Generator(device="cuda").manual_seed(P + 3 * n + K32)
+    A64 = torch.randn((max(K64, 1), P), dtype=torch.float64, device="cuda", generator=g)[:K64].contiguous()
+    B64 = torch.randn((max(K64, 1), n), dtype=torch.float64, device="cuda", generator=g)[:K64].contiguous()
+    A32 = torch.randn((max(K32, 16), P), dtype=torch.float32, device="cuda", generator=g)[:K32]
+    B32 = torch.randn((max(K32, 16), n), dtype=torch.float32, device="cuda", generator=g)[:K32]
+    off = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    A4 = _k4(A32.contiguous(), K32) if K32 else A32
+    B4 = _k4(B32.contiguous(), K32) if K32 else B32
+    args = (A64.data_ptr() if K64 else None, P, B64.data_ptr() if K64 else None, n, K64,
+            A4.data_ptr() if K32 else None, P, B4.data_ptr() if K32 else None, n, K32, off.data_ptr())
+    st = torch.cuda.current_stream().cuda_stream
+    Y0 = torch.full((P, n), float("nan"), dtype=torch.float64, device="cuda")
+    N.check(N.fn["sbi_contract_split_k4"](*args, Y0.data_ptr(), n, P, n, st), "k4")
+    ws = N.fn["sbi_contract_split_k4_workspace"](P, n, K64, K32)
+    assert (ws > 0) == split, ws
+    W = torch.empty(max(ws, 1), dtype=torch.uint8, device="cuda")
+    Y1 = torch.full((P, n), float("nan"), dtype=torch.float64, device="cuda")
+    N.check(N.fn["sbi_contract_split_k4_ws"](*args, Y1.data_ptr(), n, P, n, W.data_ptr(), ws, st),
+            "k4_ws")
+    # a workspace too small runs the unsplit kernel
+    Y2 = torch.full((P, n), float("nan"), dtype=torch.float64, device="cuda")
+    N.check(N.fn["sbi_contract_split_k4_ws"](*args, Y2.data_ptr(), n, P, n, W.data_ptr(),
+                                             max(ws - 256, 0), st), "k4_ws_small")
+    torch.cuda.synchronize()
+    assert torch.equal(Y0, Y1)
+    assert torch.equal(Y0, Y2)

@@ -219,3 +219,38 @@ def test_vjp_global_factor_table(gpu):
     g_ref = np.einsum("pm,pmj->pj", v, exact_grad(ref, x))
     assert rel(ip.vjp(x, v, "fp64").cpu().numpy(), g_ref) <= GRAD_TOL
     assert rel(ip.vjp(x, v, "mixed").cpu().numpy(), g_ref) <= MIXED_TOL
+
+
+@pytest.mark.parametrize("d,d_out,m,P", [(8, 37, 400, 1), (8, 37, 400, 100), (6, 300, 900, 700),
+                                         (12, 5, 1500, 5000), (30, 64, 2000, 256)])
+def test_mixed_gemm_one_call_equals_steps(gpu, d, d_out, m, P):
+    """sbi_newton_eval_mixed_gemm (head rows, FP32 rows and the contraction in one call, k split
+    over more workgroups for small batches) gives the bits of its three steps called one by one
+    with the unsplit contraction (sbi_newton_rows_f64, sbi_newton_basis_rows dtype 3,
+    sbi_contract_split_k4)."""
+    import ctypes
+
+    from smolyax_amd import _native as N
+
+    print(f"mixed one call d={d} d_out={d_out} m={m} P={P}", flush=True)
+    _, ip, _ = make(d, d_out, m, seed=d + P)
+    x = torch.as_tensor(np.random.default_rng(P).uniform(-1, 1, (P, d)), device=gpu)
+    y1 = ip.evaluate(x, "mixed")
+    nd = ip._ndesc
+    nh, KQ, Nn = nd.gemm_head, nd.gemm_kq, nd.basis_n
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    HT = torch.empty((max(KQ // 4, 1), P, 4), dtype=torch.float32, device=gpu)
+    H64 = torch.empty((max(nh, 1), P), dtype=torch.float64, device=gpu)
+    if Nn > nh:
+        N.check(N.fn["sbi_newton_basis_rows"](ip._nplan, x.data_ptr(), P, x.stride(0), nh, Nn,
+                                              HT.data_ptr(), P, 3, st), "basis")
+    if nh:
+        N.check(N.fn["sbi_newton_rows_f64"](ip._nplan, x.data_ptr(), P, x.stride(0),
+                                            nd.head_fac, nh, H64.data_ptr(), P, st), "rows")
+    y0 = torch.full((P, d_out), float("nan"), dtype=torch.float64, device=gpu)
+    N.check(N.fn["sbi_contract_split_k4"](H64.data_ptr(), P, nd.gemm_s64, nd.ld_s64, nh,
+                                          HT.data_ptr() if KQ else None, P,
+                                          nd.gemm_sq if KQ else None, d_out, KQ, None,
+                                          y0.data_ptr(), d_out, P, d_out, st), "contract")
+    torch.cuda.synchronize()
+    assert torch.equal(y0, y1)
