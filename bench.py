@@ -361,10 +361,12 @@ SERVE_QPS = 10000.0  # fixed Poisson arrival rate of the headline latency figure
 def bench_serving(torch, W, n_requests=600, qps=SERVE_QPS, seed=0):
     """Serving workload: 32 resident Gauss-Hermite interpolants, Poisson arrivals, FP32.
 
-    For each launch mode (direct, HIP graphs, adaptive): one untimed warm-up pass, the capacity
-    (all requests submitted at once), then p50/p99 under Poisson arrivals at `qps`.  The adaptive
-    mode also reports p50/p99 at 25 %, 50 % and 70 % of its capacity (latency against load), each
-    after one untimed pass at that load (the first pass allocates buffers for its batch sizes)."""
+    For each scheduler (Python: direct launches, HIP graphs, adaptive; the C++ scheduler of
+    sbi_server_*): one untimed warm-up pass, the capacity (all requests submitted at once), then
+    p50/p99 under Poisson arrivals at `qps`.  The adaptive and the C++ scheduler also report
+    p50/p99 at 25 %, 50 % and 70 % of their capacity (latency against load), each after one
+    untimed pass at that load (the first pass allocates buffers for its batch sizes).  The
+    headline figures are the C++ scheduler's."""
     from smolyax_amd import serving
 
     ts = time.perf_counter()
@@ -385,8 +387,11 @@ def bench_serving(torch, W, n_requests=600, qps=SERVE_QPS, seed=0):
         return srv.stats(reqs)
 
     def run(graphs, curve=False):
-        srv = serving.Server(ips, precision="mixed", n_streams=4, graphs=graphs)
-        if graphs is not False:
+        if graphs == "native":
+            srv = serving.NativeServer(ips, n_streams=4)
+        else:
+            srv = serving.Server(ips, precision="mixed", n_streams=4, graphs=graphs)
+        if graphs not in (False, "native"):
             t0 = time.perf_counter()
             srv.prewarm()
             _log(f"serve: {sum(len(sl.graphs) for sl in srv.slots)} graphs captured in "
@@ -417,12 +422,14 @@ def bench_serving(torch, W, n_requests=600, qps=SERVE_QPS, seed=0):
     e = run(False)
     g = run(True)
     a = run("auto", curve=True)
-    res = {"qps": qps, "p50_ms": a["p50_ms"], "p99_ms": a["p99_ms"], "mean_ms": a["mean_ms"],
-           "point_evals_per_s": a["point_evals_per_s"], "requests": a["requests"],
-           "capacity_qps": a["capacity_qps"],
-           "capacity_point_evals_per_s": a["capacity_point_evals_per_s"],
-           "load_curve": a["load_curve"],
-           "launches": "HIP graphs under a backlog, direct launches otherwise",
+    n = run("native", curve=True)
+    res = {"qps": qps, "p50_ms": n["p50_ms"], "p99_ms": n["p99_ms"], "mean_ms": n["mean_ms"],
+           "point_evals_per_s": n["point_evals_per_s"], "requests": n["requests"],
+           "capacity_qps": n["capacity_qps"],
+           "capacity_point_evals_per_s": n["capacity_point_evals_per_s"],
+           "load_curve": n["load_curve"],
+           "scheduler": "C++ (sbi_server_*), direct launches",
+           "python_auto": {k: a[k] for k in ("p50_ms", "p99_ms", "capacity_qps", "load_curve")},
            "eager": {k: e[k] for k in ("p50_ms", "p99_ms", "capacity_qps")},
            "graphs": {k: g[k] for k in ("p50_ms", "p99_ms", "capacity_qps")},
            "precision": "mixed (FP32)", "interpolants": len(names), "setup_s": setup_s,

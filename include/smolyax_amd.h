@@ -6,7 +6,8 @@
  * it jit/vmaps (interpolation.py:241-248).  Every entry point below names the reference function
  * whose work it replaces.  Plain pointers and sizes only; device pointers are borrowed (the caller,
  * e.g. torch, owns device memory); no entry point allocates device memory or synchronises the
- * stream; return value 0 = success, < 0 = error (message via sbi_last_error(), thread-local).
+ * stream, except the serving scheduler (sbi_server_*), which owns its streams and staging buffers;
+ * return value 0 = success, < 0 = error (message via sbi_last_error(), thread-local).
  */
 #ifndef SMOLYAX_AMD_H
 #define SMOLYAX_AMD_H
@@ -452,6 +453,38 @@ int sbi_hierarchize_f64(double* s, int64_t n_rows, int64_t d_out, int64_t ld,
                         const int32_t* step_off, int32_t n_steps, const int32_t* tgt,
                         const int32_t* src_off, const int32_t* src, const double* coef,
                         void* stream);
+
+/* ----------------------------------------------------------------------------------------------
+ * Serving: a continuous-batching scheduler over many resident interpolants (mixed path of
+ * __call__, interpolation.py:264-304, for many interpolants and requests at once; no counterpart
+ * in the reference, which evaluates one interpolant per call).  Unlike the plans, the server
+ * owns device resources: n_streams HIP streams (default priority), one completion event per
+ * stream, pooled events for requests, and per-stream staging buffers sized at sbi_server_add
+ * (cap_points x (d_in + d_out) doubles plus the mixed GEMM workspace, the largest over the
+ * interpolants).  One host thread drives a server; after an error it must be destroyed.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct sbi_server sbi_server;
+int sbi_server_create(int32_t n_streams, sbi_server** out);
+/* Register an interpolant (its Newton plan must outlive the server): d_out = 1 runs the
+ * streaming mixed kernels (sbi_newton_eval_mixed; sbi_newton_eval_mixed_gemm if those return
+ * SBI_ERR_CAPACITY), other d_out sbi_newton_eval_mixed_gemm; a batch holds at most cap_points
+ * points.  Only before the first submit.  *id numbers the interpolants from 0. */
+int sbi_server_add(sbi_server* server, const sbi_newton_plan* plan, int64_t cap_points,
+                   int32_t* id);
+/* Queue a request: P points x[P, ldx] (device) of interpolant `id`, results into y[P, ldy]
+ * (device, FP64).  An event is recorded on `stream` (the stream that produced x; NULL = default
+ * stream) and every batch that reads x waits for it.  x and y must stay allocated until the
+ * request is reported done.  *rid numbers the requests from 0 in submission order. */
+int sbi_server_submit(sbi_server* server, int32_t id, const double* x, int64_t P, int64_t ldx,
+                      double* y, int64_t ldy, void* stream, int64_t* rid);
+/* One scheduler iteration: observe finished batches (a request is done when every batch holding
+ * a piece of it has finished: its rid goes to done[], at most max_done per call, *n_done of
+ * them), then launch batches on the free streams.  *busy = 1 while requests are queued or
+ * batches are in flight. */
+int sbi_server_step(sbi_server* server, int64_t* done, int32_t max_done, int32_t* n_done,
+                    int32_t* busy);
+/* Wait for the batches in flight, release the server's streams, events and buffers. */
+int sbi_server_destroy(sbi_server* server);
 
 /* Descriptor accessors of the plan objects (shared by the library's translation units). */
 int sbi_bary_plan_desc(const sbi_plan* plan, const sbi_bary_desc** out);

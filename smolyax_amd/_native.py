@@ -131,6 +131,11 @@ SIGNATURES = {
     "sbi_pack_rows_f64": (c_i32, [c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp]),
     "sbi_hierarchize_f64": (c_i32, [c_vp, c_i64, c_i64, c_i64, c_vp, c_i32, c_vp, c_vp, c_vp, c_vp,
                                     c_vp]),
+    "sbi_server_create": (c_i32, [c_i32, c_vp]),
+    "sbi_server_add": (c_i32, [c_vp, c_vp, c_i64, c_vp]),
+    "sbi_server_submit": (c_i32, [c_vp, c_i32, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp, c_vp]),
+    "sbi_server_step": (c_i32, [c_vp, c_vp, c_i32, c_vp, c_vp]),
+    "sbi_server_destroy": (c_i32, [c_vp]),
     "sbi_bary_plan_desc": (c_i32, [c_vp, c_vp]),
     "sbi_newton_plan_desc": (c_i32, [c_vp, c_vp]),
 }

@@ -16,7 +16,7 @@ OUT = os.path.join(HERE, "libsmolyax_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["bary.hip", "newton.hip", "basis.hip", "gemm.hip", "pack.hip"]
+HIP_SOURCES = ["bary.hip", "newton.hip", "basis.hip", "gemm.hip", "pack.hip", "server.hip"]
 # per-file flags: the set-up kernels must round every product and difference on its own (their
 # results equal the host packing bit for bit)
 FILE_FLAGS = {"pack.hip": ["-ffp-contract=off"]}

@@ -41,6 +41,7 @@ Latency = completion (event observed) - arrival; throughput = point evaluations 
 """
 
 import collections
+import ctypes
 import time
 from dataclasses import dataclass, field
 
@@ -303,6 +304,103 @@ class Server:
                 "point_evals_per_s": pts / span, "span_s": span}
 
 
+class NativeServer:
+    """The same scheduling as Server in the library's C++ scheduler (sbi_server_*): arrival
+    order, per-interpolant batches of whole requests up to the interpolant's point cap (at most
+    max_batch_points and max_batch_work multiply-adds, at most 32 requests), requests above the
+    cap cut into pieces, up to `n_streams` batches in flight on HIP streams of their own.  Python
+    only allocates each request's output and reads completions, so the host work per batch is a
+    few HIP calls instead of the Python scheduler's tensor operations.  Mixed path (FP32
+    products, FP64 accumulation) of every interpolant, which must have a Newton form.
+
+    Buffers: a request keeps its points and its output alive until it is reported done; the
+    output is allocated on the caller's current stream and written on a server stream, and is
+    final once `r.done` is set (the host has seen the batch finish)."""
+
+    def __init__(self, interpolants, n_streams=4, max_batch_points=8192, max_batch_work=1e10):
+        from . import _native as N
+
+        self._N = N
+        self.ips = dict(interpolants)
+        self.max_batch_points = int(max_batch_points)
+        self.max_batch_work = float(max_batch_work)
+        srv = ctypes.c_void_p()
+        N.check(N.fn["sbi_server_create"](int(n_streams), ctypes.byref(srv)), "server_create")
+        self._srv = srv
+        self._id = {}
+        self._cap = {}
+        for name, ip in self.ips.items():
+            if ip._nplan is None:
+                raise ValueError(f"{name}: the native server needs a Newton form ({ip._newton_off})")
+            nl = ip._nlayout
+            work = max(nl.basis_n * ip.d_out, 1)
+            cap = max(MIN_BUCKET, min(self.max_batch_points, int(self.max_batch_work // work)))
+            self._cap[name] = cap
+            i = ctypes.c_int32()
+            N.check(N.fn["sbi_server_add"](srv, ip._nplan, cap, ctypes.byref(i)), "server_add")
+            self._id[name] = i.value
+        self._live = {}   # rid -> Request
+        self._done_buf = (ctypes.c_int64 * 256)()
+        self._n_done, self._busy = ctypes.c_int32(), ctypes.c_int32()
+        self._rid = ctypes.c_int64()
+        self.completed = []
+        self.n_pending = 0
+
+    def submit(self, name, x, arrival=None):
+        ip = self.ips[name]
+        if x.dim() != 2 or x.shape[1] != ip.d_in or x.dtype != torch.float64 or not x.is_cuda:
+            raise ValueError(f"{name}: points must be a (P, {ip.d_in}) float64 device tensor")
+        if x.shape[0] == 0:
+            raise ValueError("empty request")
+        if x.stride(1) != 1:
+            x = x.contiguous()
+        r = Request(-1, name, x, time.perf_counter() if arrival is None else arrival)
+        r.y = torch.empty((x.shape[0], ip.d_out), dtype=torch.float64, device=x.device)
+        st = torch.cuda.current_stream(x.device).cuda_stream
+        self._N.check(self._N.fn["sbi_server_submit"](
+            self._srv, self._id[name], x.data_ptr(), x.shape[0], max(x.stride(0), ip.d_in),
+            r.y.data_ptr(), ip.d_out, st, ctypes.byref(self._rid)), "server_submit")
+        r.rid = self._rid.value
+        self._live[r.rid] = r
+        self.n_pending += 1
+        return r
+
+    def step(self):
+        """Observe finished batches and launch new ones.  Returns True while busy."""
+        N = self._N
+        N.check(N.fn["sbi_server_step"](self._srv, self._done_buf, len(self._done_buf),
+                                        ctypes.byref(self._n_done), ctypes.byref(self._busy)),
+                "server_step")
+        k = self._n_done.value
+        if k:
+            now = time.perf_counter()
+            for j in range(k):
+                r = self._live.pop(self._done_buf[j])
+                r.done = now
+                r.taken = True
+                self.completed.append(r)
+            self.n_pending -= k
+        return bool(self._busy.value)
+
+    def drain(self):
+        while self.step():
+            time.sleep(20e-6)
+
+    def close(self):
+        if self._srv is not None:
+            self._N.check(self._N.fn["sbi_server_destroy"](self._srv), "server_destroy")
+            self._srv = None
+            self._live.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    stats = Server.stats
+
+
 def run_poisson(server, names, qps, n_requests, batch_lo=1, batch_hi=4096, seed=0, inputs=None,
                 time_limit_s=None, progress=None):
     """Open-loop Poisson arrivals at `qps`; each request picks an interpolant uniformly and a
@@ -330,7 +428,8 @@ def run_poisson(server, names, qps, n_requests, batch_lo=1, batch_hi=4096, seed=
             raise RuntimeError(
                 f"serving run exceeded {time_limit_s} s: {i} of {n_requests} submitted, "
                 f"{len(server.completed)} completed, {server.n_pending} pending, "
-                f"{sum(s.event is not None for s in server.slots)} batches in flight")
+                f"{sum(s.event is not None for s in getattr(server, 'slots', []))} batches of the "
+                f"Python scheduler in flight")
         if progress is not None and now - last_report > 10.0:
             progress(f"serve: {i} submitted, {len(server.completed)} completed")
             last_report = now

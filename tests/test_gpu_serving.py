@@ -66,6 +66,48 @@ def test_large_requests_cut_into_pieces(gpu):
             assert torch.equal(r.y, alone), (graphs, r.rid, r.name, r.n_pieces)
 
 
+def test_native_server_same_as_alone(gpu):
+    """The C++ scheduler (sbi_server_*): every request of a Poisson run, including requests cut
+    into pieces by a small work cap and batches of several requests (gathered and scattered by
+    the copy kernel), equals the request evaluated alone bit for bit, and is within 1e-4 of the
+    FP64 path; points produced on another stream are read after their producer."""
+    from smolyax_amd import serving
+    from smolyax_amd import workloads as W
+
+    ips = W.build_serving_interpolants(small_specs())
+    names = sorted(ips)
+    for work in (1e10, 2e6):
+        print(f"native server, max_batch_work={work:.0e}", flush=True)
+        srv = serving.NativeServer(ips, n_streams=4, max_batch_points=2048, max_batch_work=work)
+        reqs, _ = serving.run_poisson(srv, names, qps=20000.0, n_requests=150, batch_hi=1500,
+                                      seed=5, time_limit_s=60.0)
+        assert len(reqs) == 150 and all(r.done is not None for r in reqs)
+        if work < 1e9:
+            assert any(r.x.shape[0] > srv._cap[r.name] for r in reqs), "no request was cut"
+        torch.cuda.synchronize()
+        for r in reqs:
+            alone = ips[r.name].evaluate(r.x, precision="mixed")
+            assert torch.equal(r.y, alone), (r.rid, r.name, r.x.shape[0])
+            ref64 = ips[r.name].evaluate(r.x, precision="fp64")
+            scale = float(ref64.abs().max()) + 1e-30
+            assert float((r.y - ref64).abs().max()) / scale < 1e-4
+        srv.close()
+    # points written by a kernel still pending on another stream
+    srv = serving.NativeServer(ips, n_streams=2)
+    side = torch.cuda.Stream()
+    name = names[0]
+    d_in = ips[name].d_in
+    with torch.cuda.stream(side):
+        base = torch.randn((3000, d_in), dtype=torch.float64, device="cuda")
+        torch.cuda._sleep(2_000_000)
+        x = base * 0.5
+        r = srv.submit(name, x)
+    srv.drain()
+    torch.cuda.synchronize()
+    assert torch.equal(r.y, ips[name].evaluate(x, precision="mixed"))
+    srv.close()
+
+
 def test_admission_by_hbm_budget(gpu):
     from smolyax_amd import serving
     from smolyax_amd import workloads as W
