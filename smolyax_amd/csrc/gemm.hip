@@ -260,6 +260,16 @@ __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
     }
 }
 
+// Outputs per workgroup tile (32 * FN): 128 for wide outputs, 64 when 128-wide tiles would leave
+// fewer than two workgroups per CU (8192 points x 256 outputs: 0.337 ms for the mixed step
+// against 0.371 ms); a tile's results do not depend on its width (same bits).
+inline int64_t tile_n(int64_t P, int64_t n) {
+  if (n <= 32) return 32;
+  if (n <= 64) return 64;
+  const int64_t gm = (P + kBlockM - 1) / kBlockM;
+  return gm * ((n + 127) / 128) < 512 ? 64 : 128;
+}
+
 // ------------------------------------------------------------------------------------------
 // Split-k form of contract_kernel for small batches: the kFlushK-chunks of k are spread over
 // blockIdx.z (chunks [z * cps, (z + 1) * cps) per slice) and slice z = nsplit computes the FP64
@@ -483,7 +493,7 @@ template <typename T>
 SplitPlan split_plan(int64_t P, int64_t n, int64_t K, int64_t K64) {
   SplitPlan sp;
   const int64_t gm = (P + kBlockM - 1) / kBlockM;
-  const int64_t tn = n <= 32 ? 32 : n <= 64 ? 64 : 128;
+  const int64_t tn = tile_n(P, n);
   const int64_t groups = gm * ((n + tn - 1) / tn);
   sp.nch = (K + kFlushK - 1) / kFlushK;
   if (P <= 0 || n <= 0 || groups >= kSplitMinGroups || (sp.nch < 2 && K64 == 0)) return sp;
@@ -515,11 +525,12 @@ int contract_split_launch(const SplitPlan& sp, const T* A, int64_t lda, const T*
   const unsigned gm = (unsigned)((P + kBlockM - 1) / kBlockM);
   const unsigned gz = (unsigned)(sp.nsplit + (K64 > 0 ? 1 : 0));
   const int ns = (int)sp.nsplit;
-  if (n <= 32) {
+  const int64_t tn = tile_n(P, n);
+  if (tn == 32) {
     hipLaunchKernelGGL((contract_split_kernel<T, 1, kK4>), dim3(gm, (unsigned)((n + 31) / 32), gz),
                        block, 0, st, A, lda, B, ldb, P, n, K, A64, lda64, B64, ldb64, K64, sp.cps,
                        ns, part, head);
-  } else if (n <= 64) {
+  } else if (tn == 64) {
     hipLaunchKernelGGL((contract_split_kernel<T, 2, kK4>), dim3(gm, (unsigned)((n + 63) / 64), gz),
                        block, 0, st, A, lda, B, ldb, P, n, K, A64, lda64, B64, ldb64, K64, sp.cps,
                        ns, part, head);
@@ -544,10 +555,11 @@ int contract_launch(const T* A, int64_t lda, const T* B, int64_t ldb, const doub
                     int64_t lda64, const double* B64, int64_t ldb64, int64_t K64, hipStream_t st) {
   const dim3 block(64 * kWaves);
   const unsigned gm = (unsigned)((P + kBlockM - 1) / kBlockM);
-  if (n <= 32) {
+  const int64_t tn = tile_n(P, n);
+  if (tn == 32) {
     hipLaunchKernelGGL((contract_kernel<T, 1, kK4>), dim3(gm, (unsigned)((n + 31) / 32)), block, 0, st,
                        A, lda, B, ldb, offset, Y, ldy, P, n, K, A64, lda64, B64, ldb64, K64);
-  } else if (n <= 64) {
+  } else if (tn == 64) {
     hipLaunchKernelGGL((contract_kernel<T, 2, kK4>), dim3(gm, (unsigned)((n + 63) / 64)), block, 0, st,
                        A, lda, B, ldb, offset, Y, ldy, P, n, K, A64, lda64, B64, ldb64, K64);
   } else {
@@ -565,16 +577,15 @@ int contract_launch(const T* A, int64_t lda, const T* B, int64_t ldb, const doub
 // layout (FP64 MFMA, as in contract_kernel).  BF16 products are exact in FP32; the FP32 partial
 // sums are added into FP64 registers every kBfFlush k.
 //
-// Work split: a workgroup of 4 waves owns a 128-point x 128-output tile; each wave a 64 x 64
-// sub-tile of 4 x 4 MFMA 16x16 fragments.  Per 32-k step a lane loads one 16-byte vector (its
+// Work split: a workgroup of 4 waves (2 x 2) owns a tile of 32 * kBfWM points x 32 * kBfWN
+// outputs; each wave a sub-tile of kBfWM x kBfWN MFMA 16x16 fragments (4 x 2 for large grids,
+// 2 x 2 otherwise; 4 x 4 held one wave per SIMD and measured 15-40 % slower: config 3 shape
+// 1.16 ms -> 1.00 ms, config 4 shape 113 us -> 76 us).  Per 32-k step a lane loads one 16-byte vector (its
 // eight k) per fragment row and column, the next step's vectors in flight during the current one.
 // Lane group g = lane / 16 supplies k = 8g .. 8g + 7 of the step to both operands.  Every
 // workgroup is independent and each Y row depends only on its own column of A (fixed order).
 // ------------------------------------------------------------------------------------------
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-constexpr int kBfWM = 4, kBfWN = 4;           // fragments per wave (points, outputs)
-constexpr int kBfTileM = 32 * kBfWM;          // points per workgroup (2 waves along M)
-constexpr int kBfTileN = 32 * kBfWN;          // outputs per workgroup (2 waves along N)
 constexpr int kBfFlush = 1024;                // k per FP32 partial sum
 
 __device__ __forceinline__ v8bf as_v8bf(const uint4& u) {
@@ -583,6 +594,8 @@ __device__ __forceinline__ v8bf as_v8bf(const uint4& u) {
   return r;
 }
 
+// kBfWM x kBfWN fragments per wave (points, outputs); 2 x 2 waves per workgroup
+template <int kBfWM, int kBfWN>
 __global__ __launch_bounds__(256) void contract_bf16_kernel(
     const uint4* __restrict__ A, int64_t lda, const uint4* __restrict__ B, int64_t ldb,
     int64_t K, const double* __restrict__ A64, int64_t lda64, const double* __restrict__ B64,
@@ -592,8 +605,8 @@ __global__ __launch_bounds__(256) void contract_bf16_kernel(
   const int wave = threadIdx.x >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   const int li = lane & 15, g = lane >> 4;
-  const int64_t m_base = (int64_t)blockIdx.x * kBfTileM + wm * (16 * kBfWM);
-  const int64_t n_base = (int64_t)blockIdx.y * kBfTileN + wn * (16 * kBfWN);
+  const int64_t m_base = (int64_t)blockIdx.x * (32 * kBfWM) + wm * (16 * kBfWM);
+  const int64_t n_base = (int64_t)blockIdx.y * (32 * kBfWN) + wn * (16 * kBfWN);
   int64_t pcol[kBfWM], jcol[kBfWN];
 #pragma unroll
   for (int fm = 0; fm < kBfWM; ++fm) pcol[fm] = min(m_base + fm * 16 + li, P - 1);
@@ -970,16 +983,27 @@ extern "C" int sbi_contract_bf16_k8(const double* A64, int64_t lda64, const doub
       (P > 0 && n > 0 && K64 > 0 && (!A64 || !B64 || lda64 < P || ldb64 < n)) ||
       (P > 0 && n > 0 && K16 > 0 &&
        (!A16 || !B16 || lda16 < P || ldb16 < n || ((uintptr_t)A16 & 15) || ((uintptr_t)B16 & 15))) ||
-      (P + kBfTileM - 1) / kBfTileM > 0x7fffffff || (n + kBfTileN - 1) / kBfTileN > 65535) {
+      (P + 63) / 64 > 0x7fffffff || (n + 63) / 64 > 65535) {
     sbi::set_error("sbi_contract_bf16_k8: invalid argument (K16 must be a multiple of 32, the "
                    "k-interleaved operands 16-byte aligned)");
     return SBI_ERR_INVALID;
   }
   if (P == 0 || n == 0) return SBI_OK;
-  const dim3 grid((unsigned)((P + kBfTileM - 1) / kBfTileM), (unsigned)((n + kBfTileN - 1) / kBfTileN));
-  hipLaunchKernelGGL(contract_bf16_kernel, grid, dim3(256), 0, (hipStream_t)stream,
-                     (const uint4*)A16, lda16, (const uint4*)B16, ldb16, K16, A64, lda64, B64,
-                     ldb64, K64, offset, Y, ldy, P, n);
+  // per-wave tiles: 64 x 32 (workgroup 128 x 64) while that gives at least 1024 workgroups, else
+  // 32 x 32 (64 x 64): smaller tiles hold fewer registers (2 and 3 waves per SIMD instead of one
+  // at 64 x 64) and spread small grids further; every element's sum is the same (same bits)
+  const int64_t g42 = ((P + 127) / 128) * ((n + 63) / 64);
+  if (g42 >= 1024) {
+    const dim3 grid((unsigned)((P + 127) / 128), (unsigned)((n + 63) / 64));
+    hipLaunchKernelGGL((contract_bf16_kernel<4, 2>), grid, dim3(256), 0, (hipStream_t)stream,
+                       (const uint4*)A16, lda16, (const uint4*)B16, ldb16, K16, A64, lda64, B64,
+                       ldb64, K64, offset, Y, ldy, P, n);
+  } else {
+    const dim3 grid((unsigned)((P + 63) / 64), (unsigned)((n + 63) / 64));
+    hipLaunchKernelGGL((contract_bf16_kernel<2, 2>), grid, dim3(256), 0, (hipStream_t)stream,
+                       (const uint4*)A16, lda16, (const uint4*)B16, ldb16, K16, A64, lda64, B64,
+                       ldb64, K64, offset, Y, ldy, P, n);
+  }
   SBI_HIP_CHECK(hipGetLastError());
   return SBI_OK;
 }

@@ -173,7 +173,7 @@ def test_contract_split_k4_equals_plain_layout(gpu, P, n, K64, K32):
 
 
 @pytest.mark.parametrize("P,n,K64,K32,split", [(64, 761, 256, 3792, True), (256, 43, 0, 1264, True),
-                                               (200, 1, 64, 512, True), (1000, 1024, 256, 2048, True),
+                                               (200, 1, 64, 512, True), (500, 1024, 256, 2048, True),
                                                (33, 100, 7, 16, True), (5000, 1024, 64, 1024, False),
                                                (64, 64, 0, 256, False), (7, 5, 300, 0, False)])
 def test_contract_split_k4_ws_same_bits(gpu, P, n, K64, K32, split):
