@@ -207,10 +207,11 @@ def test_serving_workload_at_size(gpu):
     """The serving workload's extremes at their full size: d_in 500 and d_in 8 at 2e4 nodes with
     d_out 1024, d_in 500 at 1e2 nodes with d_out 1, and the largest interpolant of the 32-spec set
     (by nodes x d_out), beside two small ones.  Served results (FP32 operands, FP64 accumulation)
-    equal each request evaluated alone bit for bit; on a sample of 8 points per interpolant the
-    FP64 path is held to the exact interpolant of the FP64 data (the oracle in extended precision)
-    within 4x the FP64 oracle's own distance from it, and the served values to 1e-6 of max |y|
-    where the Newton form serves 'mixed' (1e-4 on the FP32 barycentric form)."""
+    equal each request evaluated alone bit for bit (Python and C++ schedulers); on a sample of 8
+    points per interpolant the FP64 path is held to the exact interpolant of the FP64 data (the
+    oracle in extended precision) within 4x the FP64 oracle's own distance from it, and the served
+    values to 1e-6 of max |y| where the Newton form serves 'mixed' (1e-4 on the FP32 barycentric
+    form)."""
     from oracle import nodes as ON
     from oracle import smolyak as OS
     from smolyax_amd import serving
@@ -237,6 +238,16 @@ def test_serving_workload_at_size(gpu):
     for r in reqs:
         alone = srv.ips[r.name].evaluate(r.x, precision="mixed")
         assert torch.equal(r.y, alone), (r.rid, r.name, r.batch_points)
+    # the C++ scheduler on the same workload (its interpolants all have a Newton form here)
+    if all(ip._nplan is not None for ip in ips.values()):
+        nsrv = serving.NativeServer(ips, n_streams=4, max_batch_points=4096)
+        nreqs, _ = serving.run_poisson(nsrv, names, qps=5000.0, n_requests=90, batch_hi=2048,
+                                       seed=6, time_limit_s=120.0)
+        torch.cuda.synchronize()
+        assert len(nreqs) == 90 and all(r.done is not None for r in nreqs)
+        for r in nreqs:
+            assert torch.equal(r.y, ips[r.name].evaluate(r.x, precision="mixed")), (r.rid, r.name)
+        nsrv.close()
     by_name = {}
     for r in reqs:
         by_name.setdefault(r.name, r)
