@@ -461,7 +461,9 @@ int sbi_hierarchize_f64(double* s, int64_t n_rows, int64_t d_out, int64_t ld,
  * owns device resources: n_streams HIP streams (default priority), one completion event per
  * stream, pooled events for requests, and per-stream staging buffers sized at sbi_server_add
  * (cap_points x (d_in + d_out) doubles plus the mixed GEMM workspace, the largest over the
- * interpolants).  One host thread drives a server; after an error it must be destroyed.
+ * interpolants).  One host thread drives a server, on the device that was current at
+ * sbi_server_create (plans and buffers of that device); after a failed sbi_server_add, _submit
+ * or _step every later call but sbi_server_destroy is refused.
  * ------------------------------------------------------------------------------------------- */
 typedef struct sbi_server sbi_server;
 int sbi_server_create(int32_t n_streams, sbi_server** out);
@@ -478,8 +480,8 @@ int sbi_server_add(sbi_server* server, const sbi_newton_plan* plan, int64_t cap_
 int sbi_server_submit(sbi_server* server, int32_t id, const double* x, int64_t P, int64_t ldx,
                       double* y, int64_t ldy, void* stream, int64_t* rid);
 /* One scheduler iteration: observe finished batches (a request is done when every batch holding
- * a piece of it has finished: its rid goes to done[], at most max_done per call, *n_done of
- * them), then launch batches on the free streams.  *busy = 1 while requests are queued or
+ * a piece of it has finished: its rid goes to done[], at most max_done (>= 32) per call, *n_done
+ * of them), then launch batches on the free streams.  *busy = 1 while requests are queued or
  * batches are in flight. */
 int sbi_server_step(sbi_server* server, int64_t* done, int32_t max_done, int32_t* n_done,
                     int32_t* busy);

@@ -102,6 +102,7 @@ struct sbi_server {
   int64_t next = 0;
   int64_t n_pending = 0;
   int64_t xs_bytes = 0, ys_bytes = 0, ws_bytes = 0;  // per-slot buffer parts
+  bool broken = false;  // a call failed: every later call but destroy is refused
 };
 
 namespace {
@@ -252,6 +253,7 @@ extern "C" int sbi_server_add(sbi_server* s, const sbi_newton_plan* plan, int64_
   if (!s || !id || sbi_newton_plan_desc(plan, &d) != SBI_OK || cap_points < 1 ||
       cap_points > (int64_t)65535 * 64)
     return set_invalid("sbi_server_add: invalid argument (1 <= cap_points <= 65535 * 64)");
+  if (s->broken) return set_invalid("sbi_server_add: the server failed earlier; destroy it");
   if (s->n_pending > 0 || !s->reqs.empty())
     return set_invalid("sbi_server_add: interpolants are added before requests are submitted");
   Interp ip{plan, d->d_in, d->d_out, cap_points, d->d_out == 1, d->gemm_head + d->gemm_kq > 0};
@@ -275,6 +277,7 @@ extern "C" int sbi_server_add(sbi_server* s, const sbi_newton_plan* plan, int64_
         sl.buf = nullptr;
         free_slot_buffers(s);
         s->xs_bytes = s->ys_bytes = s->ws_bytes = 0;
+        s->broken = true;
         sbi::set_error(std::string("sbi_server_add: staging buffers: ") + hipGetErrorString(e));
         return SBI_ERR_HIP;
       }
@@ -294,6 +297,7 @@ extern "C" int sbi_server_submit(sbi_server* s, int32_t id, const double* x, int
                                  int64_t ldx, double* y, int64_t ldy, void* stream, int64_t* rid) {
   if (!s || !rid || id < 0 || id >= (int32_t)s->ips.size() || !x || !y || P < 1)
     return set_invalid("sbi_server_submit: invalid argument");
+  if (s->broken) return set_invalid("sbi_server_submit: the server failed earlier; destroy it");
   const Interp& ip = s->ips[id];
   if (ldx < ip.d_in || ldy < ip.d_out)
     return set_invalid("sbi_server_submit: row strides below d_in / d_out");
@@ -310,7 +314,12 @@ extern "C" int sbi_server_submit(sbi_server* s, int32_t id, const double* x, int
   } else {
     SBI_HIP_CHECK(hipEventCreateWithFlags(&r.ready, hipEventDisableTiming));
   }
-  SBI_HIP_CHECK(hipEventRecord(r.ready, (hipStream_t)stream));
+  const hipError_t er = hipEventRecord(r.ready, (hipStream_t)stream);
+  if (er != hipSuccess) {
+    s->ev_pool.push_back(r.ready);
+    sbi::set_error(std::string("sbi_server_submit: ") + hipGetErrorString(er));
+    return SBI_ERR_HIP;
+  }
   const int64_t k = s->next++;
   try {
     s->reqs.emplace(k, r);
@@ -330,14 +339,18 @@ static int server_step(sbi_server* s, int64_t* done, int32_t max_done, int32_t* 
 
 extern "C" int sbi_server_step(sbi_server* s, int64_t* done, int32_t max_done, int32_t* n_done,
                                int32_t* busy) {
-  if (!s || !n_done || !busy || max_done < 0 || (max_done > 0 && !done))
-    return set_invalid("sbi_server_step: invalid argument");
+  if (!s || !n_done || !busy || max_done < kMaxBatchReqs || !done)
+    return set_invalid("sbi_server_step: invalid argument (done[] holds at least 32 entries)");
+  if (s->broken) return set_invalid("sbi_server_step: the server failed earlier; destroy it");
+  int rc;
   try {
-    return server_step(s, done, max_done, n_done, busy);
+    rc = server_step(s, done, max_done, n_done, busy);
   } catch (const std::exception& e) {
     sbi::set_error(std::string("sbi_server_step: ") + e.what());
-    return SBI_ERR_INVALID;
+    rc = SBI_ERR_INVALID;
   }
+  if (rc != SBI_OK) s->broken = true;
+  return rc;
 }
 
 static int server_step(sbi_server* s, int64_t* done, int32_t max_done, int32_t* n_done,
