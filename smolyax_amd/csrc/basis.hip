@@ -169,8 +169,10 @@ __device__ __forceinline__ void store_group(void* HT, int64_t ldh, int r, int64_
 
 // H^T[nu, p] for a chunk of nodes.  Node descriptor: 8 x u16 table rows (0xffff = none).
 // kQ = 0: plain layout HT[r * ldh + p] of type OUT; kQ = 4 / 8: k-interleaved (see above).
-template <typename T, typename OUT, int kQ = 0>
-__global__ __launch_bounds__(kTile* kHWaves) void newton_basis_kernel(sbi_newton_desc d,
+// W waves per workgroup share the tile's table: 16 for large batches, 4 (with 256-row chunks)
+// when 16-wave workgroups would be too few to fill the GPU; each element is computed the same way.
+template <typename T, typename OUT, int kQ = 0, int W = kHWaves>
+__global__ __launch_bounds__(kTile* W) void newton_basis_kernel(sbi_newton_desc d,
                                                                       const double* __restrict__ x,
                                                                       int64_t P, int64_t ldx,
                                                                       OUT* __restrict__ HT,
@@ -184,7 +186,7 @@ __global__ __launch_bounds__(kTile* kHWaves) void newton_basis_kernel(sbi_newton
   const int64_t p_raw = (int64_t)blockIdx.x * kTile + lane;
   const bool valid = p_raw < P;
   const int64_t p = valid ? p_raw : (P - 1);
-  build_h_table<T>(d, x + p * ldx, tab, nullptr, wave, lane, kHWaves);
+  build_h_table<T>(d, x + p * ldx, tab, nullptr, wave, lane, W);
   __syncthreads();
 
   const int n0 = row0 + blockIdx.y * chunk;
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(kTile* kHWaves) void newton_basis_kernel(sbi_newton
   const int n1 = min(rend, n0 + chunk);
   const uint4* __restrict__ desc = reinterpret_cast<const uint4*>(d.basis_desc);
   // each wave: 64-node groups, descriptors one per lane, broadcast with v_readlane
-  for (int base = n0 + wave * 64; base < n1; base += kHWaves * 64) {
+  for (int base = n0 + wave * 64; base < n1; base += W * 64) {
     const int mine = base + lane;
     const uint4 dv = (mine < min(n1, row1)) ? desc[mine]
                                             : make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu,
@@ -745,6 +747,16 @@ static int basis_launch(const sbi_newton_desc& d, const double* x, int64_t P, in
   if (direct) {
     hipLaunchKernelGGL((newton_basis_direct_kernel<T, OUT, kQ>), grid, dim3(kTile * kHWaves), 0,
                        st, d, x, P, ldx, HT, ldh, chunk, row0, row1);
+    SBI_HIP_CHECK(hipGetLastError());
+    return SBI_OK;
+  }
+  const int64_t tiles = (P + kTile - 1) / kTile;
+  if (tiles * ((rows + chunk - 1) / chunk) < 256) {  // small batch: 4-wave workgroups, 256 rows
+    constexpr int kSmall = 256;
+    const dim3 g2((unsigned)tiles, (unsigned)((rows + kSmall - 1) / kSmall));
+    sbi::allow_lds((const void*)newton_basis_kernel<T, OUT, kQ, 4>);
+    hipLaunchKernelGGL((newton_basis_kernel<T, OUT, kQ, 4>), g2, dim3(kTile * 4), lds, st, d, x,
+                       P, ldx, HT, ldh, kSmall, row0, row1);
     SBI_HIP_CHECK(hipGetLastError());
     return SBI_OK;
   }
