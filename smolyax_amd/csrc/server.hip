@@ -263,9 +263,13 @@ extern "C" int sbi_server_add(sbi_server* s, const sbi_newton_plan* plan, int64_
   // per-slot buffers: staging for batches of several requests, and the GEMM workspace
   const int64_t xs = align256(cap_points * ip.d_in * 8), ys = align256(cap_points * ip.d_out * 8);
   int64_t ws = 0;
-  if (ip.gemm_ok) {
-    ws = sbi_newton_eval_mixed_gemm_workspace(plan, cap_points);
-    if (ws < 0) return SBI_ERR_INVALID;
+  if (ip.gemm_ok) {  // the largest over batch sizes (small batches split k: partial tiles)
+    for (int64_t q = 64;; q += 64) {
+      const int64_t w = sbi_newton_eval_mixed_gemm_workspace(plan, q < cap_points ? q : cap_points);
+      if (w < 0) return SBI_ERR_INVALID;
+      if (w > ws) ws = w;
+      if (q >= cap_points) break;
+    }
   }
   const int64_t nx = xs > s->xs_bytes ? xs : s->xs_bytes, ny = ys > s->ys_bytes ? ys : s->ys_bytes,
                 nw = align256(ws) > s->ws_bytes ? align256(ws) : s->ws_bytes;
