@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--oracle-points", type=int, default=256)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-legs", action="store_true",
-                    help="skip the config 3 and config 4 legs of the default run")
+                    help="skip the c3_sin_d100_o4096 and c4_tanh_d200_o256 legs of the default run")
     ap.add_argument("--serve-requests", type=int, default=600)
     return ap.parse_args()
 
@@ -130,11 +130,11 @@ def contraction_times(ip, x, torch, steps):
 
 
 def bench_gemm_leg(args, torch, W, name, cpu=True, log=None):
-    """A large-d_out workload (config 3 or 4): every path is a basis kernel + one contraction on
+    """A large-d_out workload (c3_sin_d100_o4096 or c4_tanh_d200_o256): every path is a basis kernel + one contraction on
     the matrix cores.  Reports each path's throughput (inputs resident, whole step), the
     contraction kernel's roofline on the matrix cores, accuracy against the FP64 oracle and the
     exact interpolant (the oracle's extended precision) on a sample of points, the CPU baseline
-    on a sample, and for config 4 the adjoint (vjp) in mixed and FP64 with its own accuracy."""
+    on a sample, and for c4_tanh_d200_o256 the adjoint (vjp) in mixed and FP64 with its own accuracy."""
     from oracle import cpu_baseline as CB
     from oracle import nodes as ON
     from oracle import smolyak as OS

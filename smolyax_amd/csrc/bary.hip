@@ -1019,8 +1019,8 @@ struct IntSplit {
 static IntSplit integral_split(const sbi_bary_desc& d) {
   // output tile width: the smallest power of two >= d_out, at most 64 (4 entry lanes), or 256
   // (one entry lane: a workgroup reads 2 KB runs of each F row) from d_out = 1024 on, where F is
-  // large enough to keep ~2048 such workgroups busy (config 3: 75 % against 65 % of HBM; config
-  // 4, d_out 256: 33 against 25 us)
+  // large enough to keep ~2048 such workgroups busy (F of 69 185 x 4 096: 75 % against 65 % of
+  // HBM; F of 27 003 x 256: 33 against 25 us)
   IntSplit sp;
   const int wmax = d.d_out >= 1024 ? 256 : 64;
   int w = 1;

@@ -579,10 +579,12 @@ int contract_launch(const T* A, int64_t lda, const T* B, int64_t ldb, const doub
 //
 // Work split: a workgroup of 4 waves (2 x 2) owns a tile of 32 * kBfWM points x 32 * kBfWN
 // outputs; each wave a sub-tile of kBfWM x kBfWN MFMA 16x16 fragments (4 x 2 for large grids,
-// 2 x 2 otherwise; 4 x 4 held one wave per SIMD and measured 15-40 % slower: config 3 shape
-// 1.16 ms -> 1.00 ms, config 4 shape 113 us -> 76 us).  Per 32-k step a lane loads one 16-byte vector (its
-// eight k) per fragment row and column, the next step's vectors in flight during the current one.
-// Lane group g = lane / 16 supplies k = 8g .. 8g + 7 of the step to both operands.  Every
+// 2 x 2 otherwise; 4 x 4 held one wave per SIMD and measured 15-40 % slower: 4096 points x
+// 4096 outputs, K 10 016: 1.16 ms -> 1.00 ms; 8192 x 256, K 5 024: 113 us -> 76 us).  Per 32-k
+// step a lane loads one 16-byte vector (its eight k) per fragment row and column, the next
+// step's vectors in flight during the current one (two steps ahead measured 3-5 % faster for
+// 2 x 2 and slower for 4 x 2: not kept).  Lane group g = lane / 16 supplies k = 8g .. 8g + 7 of
+// the step to both operands.  Every
 // workgroup is independent and each Y row depends only on its own column of A (fixed order).
 // ------------------------------------------------------------------------------------------
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));

@@ -110,7 +110,7 @@ def test_gradient_against_exact(case):
 def test_integral_against_exact(case):
     ip, fx, name = case["ip"], case["fx"], case["name"]
     Q = ip.integral().cpu().numpy()[fx["cols"]]
-    # relative to the size of the values: the integral of an odd f (config 4) is close to zero
+    # relative to the size of the values: the integral of an odd f (the tanh workload) is close to zero
     scale = float(np.max(np.abs(fx["y_hi"])))
     e = float(np.max(np.abs(Q - fx["q_hi"]))) / scale
     e64 = float(np.max(np.abs(fx["q_f64"] - fx["q_hi"]))) / scale

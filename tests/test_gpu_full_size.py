@@ -38,7 +38,7 @@ from oracle import indices as OI
 from oracle import nodes as ON
 from oracle import smolyak as OS
 
-# workload, points of the gradient slice, monomials of p (config 4: enough that the Newton
+# workload, points of the gradient slice, monomials of p (d_in 200, d_out 256: enough that the Newton
 # surpluses of p spill past the FP64 head rows of the mixed contraction into its FP32 rows)
 FULL = [("c2_exp_d1000", 1024, 48), ("c3_sin_d100_o4096", 64, 48), ("c4_tanh_d200_o256", 512, 400)]
 NEWTON_FP64_TOL = 1e-14
