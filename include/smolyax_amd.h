@@ -480,7 +480,7 @@ int sbi_server_add(sbi_server* server, const sbi_newton_plan* plan, int64_t cap_
 int sbi_server_submit(sbi_server* server, int32_t id, const double* x, int64_t P, int64_t ldx,
                       double* y, int64_t ldy, void* stream, int64_t* rid);
 /* One scheduler iteration: observe finished batches (a request is done when every batch holding
- * a piece of it has finished: its rid goes to done[], at most max_done (>= 32) per call, *n_done
+ * a piece of it has finished: its rid goes to done[], at most max_done (>= 64) per call, *n_done
  * of them), then launch batches on the free streams.  *busy = 1 while requests are queued or
  * batches are in flight. */
 int sbi_server_step(sbi_server* server, int64_t* done, int32_t max_done, int32_t* n_done,

@@ -30,7 +30,7 @@
 
 namespace {
 
-constexpr int kMaxBatchReqs = 32;
+constexpr int kMaxBatchReqs = 64;
 
 struct RowCopy {
   int32_t n;                         // requests in the batch
@@ -344,7 +344,7 @@ static int server_step(sbi_server* s, int64_t* done, int32_t max_done, int32_t* 
 extern "C" int sbi_server_step(sbi_server* s, int64_t* done, int32_t max_done, int32_t* n_done,
                                int32_t* busy) {
   if (!s || !n_done || !busy || max_done < kMaxBatchReqs || !done)
-    return set_invalid("sbi_server_step: invalid argument (done[] holds at least 32 entries)");
+    return set_invalid("sbi_server_step: invalid argument (done[] holds at least 64 entries)");
   if (s->broken) return set_invalid("sbi_server_step: the server failed earlier; destroy it");
   int rc;
   try {

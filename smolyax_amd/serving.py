@@ -307,7 +307,7 @@ class Server:
 class NativeServer:
     """The same scheduling as Server in the library's C++ scheduler (sbi_server_*): arrival
     order, per-interpolant batches of whole requests up to the interpolant's point cap (at most
-    max_batch_points and max_batch_work multiply-adds, at most 32 requests), requests above the
+    max_batch_points and max_batch_work multiply-adds, at most 64 requests), requests above the
     cap cut into pieces, up to `n_streams` batches in flight on HIP streams of their own.  Python
     only allocates each request's output and reads completions, so the host work per batch is a
     few HIP calls instead of the Python scheduler's tensor operations.  Mixed path (FP32
