@@ -92,6 +92,18 @@ def test_native_server_same_as_alone(gpu):
             scale = float(ref64.abs().max()) + 1e-30
             assert float((r.y - ref64).abs().max()) / scale < 1e-4
         srv.close()
+    # row views of wider arrays (row stride above d_in), alone and batched with other requests
+    srv = serving.NativeServer(ips, n_streams=4)
+    views = []
+    for i, name in enumerate(names):
+        wide = torch.randn((300 + 50 * i, ips[name].d_in + 3), dtype=torch.float64, device="cuda")
+        views.append((name, wide[:, 1:1 + ips[name].d_in]))
+    reqs = [srv.submit(name, v) for name, v in views + views]
+    srv.drain()
+    torch.cuda.synchronize()
+    for r in reqs:
+        assert torch.equal(r.y, ips[r.name].evaluate(r.x, precision="mixed")), (r.rid, r.name)
+    srv.close()
     # points written by a kernel still pending on another stream
     srv = serving.NativeServer(ips, n_streams=2)
     side = torch.cuda.Stream()
