@@ -365,8 +365,8 @@ static int server_step(sbi_server* s, int64_t* done, int32_t max_done, int32_t* 
     // a slot's completions are reported together: keep it busy while the buffer is too small
     int32_t fin = 0;
     for (const Piece& pc : sl.pieces) {
-      const Req& r = s->reqs.at(pc.rid);
-      if (r.taken && r.parts == 1) ++fin;
+      const auto it = s->reqs.find(pc.rid);
+      if (it != s->reqs.end() && it->second.taken && it->second.parts == 1) ++fin;
     }
     if (*n_done + fin > max_done) continue;
     const hipError_t e = hipEventQuery(sl.done);
@@ -377,6 +377,7 @@ static int server_step(sbi_server* s, int64_t* done, int32_t max_done, int32_t* 
     }
     for (const Piece& pc : sl.pieces) {
       auto it = s->reqs.find(pc.rid);
+      if (it == s->reqs.end()) continue;  // (every piece's request is live until its last piece)
       Req& r = it->second;
       r.parts -= 1;
       if (r.taken && r.parts == 0) {
