@@ -44,7 +44,7 @@ def parse():
 
 TRAFFIC_FILES = {(k, "c2_exp_d1000"): "profiles/newton_mixed_mfma_c2_d1000_traffic.md"
                  for k in ("newton_mixed_mfma_kernel", "bary_eval_prog_kernel",
-                           "newton_eval_kernel<double>")}
+                           "newton_eval_kernel<double>", "newton_f64_mfma_kernel")}
 
 
 def measured_traffic(kernel, config):
@@ -565,12 +565,14 @@ def main():
     # node and point plus the per-group FP64 work: 2 |Lambda| + 4 groups per point)
     t_n = results["newton-fp64"]["kernel_ms"] * 1e-3
     flop_nt = (2.0 * NL.n_nodes + 4.0 * NL.n_groups) * P
-    roof_newton = {"bound": "valu", "kernel": "newton_eval_kernel<double>",
+    mc64 = int(N.fn["sbi_newton_f64_matrix_cores"](ip._nplan)) == 1
+    kname_n = "newton_f64_mfma_kernel" if mc64 else "newton_eval_kernel<double>"
+    roof_newton = {"bound": "mfma" if mc64 else "valu", "kernel": kname_n,
                    "achieved": flops / t_n / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                    "frac": flops / t_n / 1e12 / PEAK_FP64_TFLOPS,
                    "flop_basis": "2 d_out K_raw per point (the reference's terms)",
                    "own_terms_tflops": flop_nt / t_n / 1e12,
-                   "traffic": measured_traffic("newton_eval_kernel<double>", args.config),
+                   "traffic": measured_traffic(kname_n, args.config),
                    "avg_launch_ms": results["newton-fp64"]["kernel_ms"]}
 
     rows = bench_gradient_integral(ip, x, torch)

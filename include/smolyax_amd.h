@@ -283,6 +283,9 @@ typedef struct sbi_newton_desc {
   int64_t ld_s64;
   const float* gemm_sq;       /* FP32 surpluses of the other rows, k-interleaved as in
                                  sbi_contract_split_k4 ([gemm_kq / 4, d_out, 4], zero rows pad) */
+  /* FP64 matrix-core kernel of sbi_newton_eval_f64 (optional: NULL selects the streaming one):
+   * the A steps of mt_a in FP64 ([64 * mt_steps], read from global memory) */
+  const double* mt_a64;
 } sbi_newton_desc;
 
 typedef struct sbi_newton_plan sbi_newton_plan;
@@ -311,7 +314,11 @@ int64_t sbi_newton_eval_mixed_gemm_workspace(const sbi_newton_plan* plan, int64_
 int sbi_newton_eval_mixed_gemm(const sbi_newton_plan* plan, const double* x, int64_t P,
                                int64_t ldx, double* y, int64_t ldy, void* workspace,
                                int64_t ws_bytes, void* stream);
-/* Same in FP64 throughout. */
+/* Same in FP64 throughout: the matrix-core kernel in FP64 (W and the h table in FP64, the A
+ * steps of mt_a64 from global memory, v_mfma_f64_16x16x4) when the plan has mt_a64 and its FP64
+ * tables fit LDS, else the streaming kernel; sbi_newton_f64_matrix_cores(plan) says which (1 / 0).
+ * The two differ by FP64 rounding only. */
+int sbi_newton_f64_matrix_cores(const sbi_newton_plan* plan);
 int sbi_newton_eval_f64(const sbi_newton_plan* plan, const double* x, int64_t P, int64_t ldx,
                         double* y, void* stream);
 

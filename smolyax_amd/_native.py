@@ -67,7 +67,7 @@ class NewtonDesc(ctypes.Structure):
         ("mt_n", c_i32), ("mt_steps", c_i32), ("mi_n", c_i32), ("n_xi_dims", c_i32),
         ("mt_a", c_vp), ("mt_grp", c_vp), ("mi_rec", c_vp), ("mwave_item", c_vp),
         ("gemm_head", c_i32), ("gemm_kq", c_i32), ("head_fac", c_vp), ("gemm_s64", c_vp),
-        ("ld_s64", c_i64), ("gemm_sq", c_vp),
+        ("ld_s64", c_i64), ("gemm_sq", c_vp), ("mt_a64", c_vp),
     ]
 
 
@@ -110,6 +110,7 @@ SIGNATURES = {
     "sbi_bary_basis_f64": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp]),
     "sbi_bary_basis_f32": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i64, c_vp]),
     "sbi_newton_mixed_staged": (c_i32, [c_vp]),
+    "sbi_newton_f64_matrix_cores": (c_i32, [c_vp]),
     "sbi_newton_basis_rows": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_i32, c_i32, c_vp, c_i64, c_i32,
                                       c_vp]),
     "sbi_newton_rows_f64": (c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_i32, c_vp, c_i64, c_vp]),

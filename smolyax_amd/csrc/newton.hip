@@ -263,6 +263,7 @@ __global__ __launch_bounds__(kTile* kWaves) void newton_eval_kernel(sbi_newton_d
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef double v2d __attribute__((ext_vector_type(2)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
 typedef unsigned int v2u __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) float lds_f32;
 typedef __attribute__((address_space(3))) double lds_f64;
@@ -685,11 +686,233 @@ __global__ __launch_bounds__(kTile* kMWaves) void newton_mixed_mfma_kernel(
   }
 }
 
+// ------------------------------------------------------------------------------------------
+// FP64 form of the matrix-core kernel (sbi_newton_eval_f64 when the plan has mt_a64 and the FP64
+// W table fits LDS): the same work items and tables, W and the h table in FP64, the A steps
+// (c_k * surplus in FP64) read from global memory per item (they do not fit in LDS beside the
+// FP64 W), v_mfma_f64_16x16x4, whose result holds point 16 s + l / 16 + 4 r of the subset in
+// register r (the coordinates and the reduction use that order).
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(kTile* kMWaves) void newton_f64_mfma_kernel(
+    sbi_newton_desc d, const double* __restrict__ x, int64_t P, int64_t ldx, double* __restrict__ y) {
+  extern __shared__ double lds_raw[];
+  lds_f64* W = (lds_f64*)lds_raw;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t tile0 = (int64_t)blockIdx.x * kTile;
+  const int64_t p_raw = tile0 + lane;
+  const bool valid = p_raw < P;
+  const int64_t p = valid ? p_raw : (P - 1);
+  const double* __restrict__ xp = x + p * ldx;
+  const int rw_pad = (d.rw_n + 3) & ~3;
+  lds_f64* R = W + rw_pad * kTile;  // region R (16-byte aligned: rw_pad * 512 bytes)
+
+  cptr<int32_t> hs_dim = C(d.hs_dim), hs_kmax = C(d.hs_kmax), hs_row = C(d.hs_row),
+                rint_rec = C(d.rint_rec), rint_h = C(d.rint_h), level_off = C(d.level_off),
+                wave_item = C(d.mwave_item), mi_rec = C(d.mi_rec);
+  cptr<double> dim_xi = C(d.dim_xi), dim_c = C(d.dim_c);
+
+  // Work items (step 3): wave w = 4 g + s takes the items of wave group g for the 16 points
+  // 16 s .. 16 s + 15 of the tile.  Lane l owns group l % 16 of an item's tile and the points
+  // 16 s + 4 (l / 16) + r (r < 4).  Coordinates are read four items ahead of their use; the first
+  // four items' are requested here, before the tables, from the descriptors in global memory.
+  const int li = lane & 15, qd = lane >> 4;
+  const int sub = wave & 3, wgrp = wave >> 2;
+  const double* xr[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) xr[r] = x + min(tile0 + 16 * sub + qd + 4 * r, P - 1) * ldx;
+  const int i_begin = wave_item[wgrp], i_end = wave_item[wgrp + 1];
+  const v4i* __restrict__ gdesc = (const v4i*)d.mt_grp;
+  struct Pre {
+    int j, k;
+    double a, xv[4];
+  };
+  auto fetch = [&](const v4i gdsc) {
+    Pre q;
+    q.j = gdsc.x;
+    q.k = gdsc.y;
+    q.a = __hiloint2double(gdsc.w, gdsc.z);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) q.xv[r] = xr[r][q.j];
+    return q;
+  };
+  // (past the group's items: item 0's descriptor, loaded but never used; mi_n >= 1)
+  auto first = [&](int i) { return gdesc[mi_rec[4 * (i < i_end ? i : 0) + 3] * 16 + li]; };
+  // The internal-node records of this wave (record q = wave + 16 u in lane u) and the level
+  // offsets (lane l), by vector loads issued before anything else: the products of 1b then need
+  // no scalar round trip per level (records beyond 64 per wave take the scalar path).
+  const int nint = d.n_internal, nlev = d.n_levels;
+  const bool rec_v_ok = nint <= kTile * kMWaves && nlev < kTile;
+  const int qv = min(wave + kMWaves * lane, max(nint - 1, 0));
+  const int rec_v = d.rint_rec[qv], hrow_v = d.rint_h[qv];
+  const int lo_v = d.level_off[min(lane, nlev)];
+  // The four descriptors first, then the coordinates of the first kHsBatch table dims of this
+  // wave (1a), then the four items' coordinates: vector loads complete in order, so each wait
+  // below covers exactly the loads it needs (every such load is unconditional, indices clamped).
+  const v4i f1 = first(i_begin), f2 = first(i_begin + 1), f3 = first(i_begin + 2),
+            f4 = first(i_begin + 3);
+  const int nhs = d.n_hs_dims;
+  int hj[kHsBatch];
+  double hx[kHsBatch];
+#pragma unroll
+  for (int u = 0; u < kHsBatch; ++u) hj[u] = hs_dim[min(wave + u * kMWaves, max(nhs - 1, 0))];
+#pragma unroll
+  for (int u = 0; u < kHsBatch; ++u) hx[u] = xp[hj[u]];
+
+  // 1a. h values of the internal nodes' factors (lane = point), FP64
+  auto h_rows = [&](int i, int j, double xj) {
+    const int km = hs_kmax[i], row = hs_row[i];
+    cptr<double> xi = dim_xi + (int64_t)j * d.kmax1;
+    cptr<double> cj = dim_c + (int64_t)j * d.kmax1;
+    double pr = 1.0;
+    for (int k = 1; k <= km; ++k) {
+      pr *= (xj - xi[k - 1]);
+      R[(row + k - 1) * kTile + lane] = pr * cj[k];
+    }
+  };
+#pragma unroll
+  for (int u = 0; u < kHsBatch; ++u)
+    if (wave + u * kMWaves < nhs) h_rows(wave + u * kMWaves, hj[u], hx[u]);
+  for (int i = wave + kHsBatch * kMWaves; i < nhs; i += kMWaves) h_rows(i, hs_dim[i], xp[hs_dim[i]]);
+  for (int q = wave; q < rw_pad; q += kMWaves)
+    if (q == 0 || q >= d.rw_n) W[widx(q, lane)] = (q == 0) ? 1.0 : 0.0;
+  __syncthreads();
+  // 1b. products of the internal nodes, depth by depth: W[row] = W[parent] * h; a wave's records
+  // of a level are read kLvBatch at a time (one scalar round trip per batch)
+  for (int lv = 0; rec_v_ok && lv < nlev; ++lv) {
+    const int a = __builtin_amdgcn_readlane(lo_v, lv), b = __builtin_amdgcn_readlane(lo_v, lv + 1);
+    // this wave's records q in [a, b), q = wave (mod 16): lanes u0 .. u1 - 1
+    const int u0 = a <= wave ? 0 : (a - wave + kMWaves - 1) / kMWaves;
+    const int u1 = b <= wave ? 0 : (b - wave + kMWaves - 1) / kMWaves;
+    for (int u = u0; u < u1; ++u) {
+      const uint32_t rec = (uint32_t)__builtin_amdgcn_readlane(rec_v, u);
+      const int hr = __builtin_amdgcn_readlane(hrow_v, u);
+      W[widx(rec & 0xffff, lane)] = W[widx(rec >> 16, lane)] * R[hr * kTile + lane];
+    }
+    __syncthreads();
+  }
+  for (int lv = 0; !rec_v_ok && lv < nlev; ++lv) {
+    const int a = level_off[lv], b = level_off[lv + 1];
+    for (int q0 = a + wave; q0 < b; q0 += kLvBatch * kMWaves) {
+      uint32_t rec[kLvBatch];
+      int hr[kLvBatch];
+#pragma unroll
+      for (int u = 0; u < kLvBatch; ++u) {
+        const int q = min(q0 + u * kMWaves, b - 1);
+        rec[u] = (uint32_t)rint_rec[q];
+        hr[u] = rint_h[q];
+      }
+#pragma unroll
+      for (int u = 0; u < kLvBatch; ++u)
+        if (q0 + u * kMWaves < b)
+          W[widx(rec[u] & 0xffff, lane)] = W[widx(rec[u] >> 16, lane)] * R[hr[u] * kTile + lane];
+    }
+    __syncthreads();
+  }
+
+  // 2. stage the A steps, the group descriptors and the work items (region R)
+  typedef __attribute__((address_space(3))) char lds_char;
+  lds_char* GD = (lds_char*)R;  // (the A steps stay in global memory)
+  lds_char* IT = GD + (size_t)d.mt_n * 256;
+  lds_char* XI = IT + (size_t)d.mi_n * 16;  // dim_xi rows of dims < n_xi_dims
+  auto stage = [&](const void* src, lds_char* dst, int nbytes) {
+    for (int c = wave; c * 1024 < nbytes; c += kMWaves) {
+      const int off = c * 1024 + lane * 16;
+      if (off < nbytes)
+        __builtin_amdgcn_global_load_lds((const char*)src + off, (lds_void*)(dst + c * 1024), 16, 0,
+                                         0);
+    }
+  };
+  stage(d.mt_grp, GD, d.mt_n * 256);
+  stage(d.mi_rec, IT, d.mi_n * 16);
+  stage(d.dim_xi, XI, d.n_xi_dims * d.kmax1 * 8);
+  // the first four items' coordinates (after the tables: their burst of loads would hold up the
+  // tables' scalar loads)
+  Pre n1 = fetch(f1), n2 = fetch(f2), n3 = fetch(f3), n4 = fetch(f4);
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's copies have landed
+  __syncthreads();
+  const double* __restrict__ A64 = d.mt_a64;
+  const __attribute__((address_space(3))) v4i* GI = (const __attribute__((address_space(3))) v4i*)GD;
+  const __attribute__((address_space(3))) v4i* II = (const __attribute__((address_space(3))) v4i*)IT;
+
+  // 3. work items.  v_mfma_f64_16x16x4 with A = W (lane l: point 16 s + l % 16 of subset s,
+  // row l / 16 of the step) and B = the tile's surpluses (lane l: row l / 16, group l % 16): the
+  // result T[point, group] sits in lane l, register r at point 16 s + l / 16 + 4 r, group l % 16.
+  // So each lane owns one group of the tile (its dim j) and 16 points: its coordinates are x_j of
+  // 16 rows, and the 16 lanes of a lane row read 16 dims of one row (for the consecutive dims of
+  // the prefix tiles, one 128-byte line).
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  // one work item; its coordinates are in `slot`, which is refilled with those of item it + 4
+  // once they are used.  The loop below is unrolled over the four slots so that each keeps its
+  // registers: a refill lands where it is read, never copied (a copy would wait for its loads).
+  auto process = [&](int it, Pre& slot) {
+    const v4i ir = II[it];
+    const int a0 = __builtin_amdgcn_readfirstlane(ir.x), w0 = __builtin_amdgcn_readfirstlane(ir.y);
+    const int ns = __builtin_amdgcn_readfirstlane(ir.z);
+    const v4i gnext = GI[__builtin_amdgcn_readfirstlane(II[it + 4 < i_end ? it + 4 : it].w) * 16 + li];
+    const Pre& cur = slot;
+    v4d D = v4d{0.0, 0.0, 0.0, 0.0};
+    const int wb = sub * 64 + li * 4 + qd;
+    // A steps from global memory, four at a time (clamped to the item's last step, unconditional
+    // loads), then four FP64 MFMAs
+    const double* __restrict__ Ai = A64 + (int64_t)a0 * kTile + lane;
+    for (int st0 = 0; st0 < ns; st0 += 4) {
+      double bv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) bv[u] = Ai[(int64_t)min(st0 + u, ns - 1) * kTile];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (st0 + u < ns)
+          D = __builtin_amdgcn_mfma_f64_16x16x4f64(W[(w0 + st0 + u) * kTile * 4 + wb], bv[u], D,
+                                                   0, 0, 0);
+    }
+    // node values xi_1 .. of the group's dim from the on-chip copy (a global load here would
+    // wait for every coordinate load in flight)
+    const lds_f64* xi = (const lds_f64*)XI + cur.j * d.kmax1;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double pr = cur.xv[r] - cur.a;
+      for (int k = 1; k < cur.k; ++k) pr *= (cur.xv[r] - xi[k]);
+      acc[r] = fma(pr, D[r], acc[r]);
+    }
+    slot = fetch(gnext);  // unconditional (past the end: this item's dims again, never used), so
+                          // that no branch hides the loads from the wait-count bookkeeping
+  };
+  for (int it = i_begin; it < i_end; it += 4) {
+    process(it, n1);
+    if (it + 1 >= i_end) break;
+    process(it + 1, n2);
+    if (it + 2 >= i_end) break;
+    process(it + 2, n3);
+    if (it + 3 >= i_end) break;
+    process(it + 3, n4);
+  }
+
+  // 4. deterministic reduction in FP64: each point's sums over the 16 lanes of its lane row (one
+  // group each) by a fixed butterfly, then over the wave groups in order
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int m = 8; m > 0; m >>= 1) acc[r] += __shfl_xor(acc[r], m, 16);
+  __syncthreads();
+  lds_f64* Rd = (lds_f64*)lds_raw;
+  if (li == 0) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Rd[wgrp * kTile + 16 * sub + qd + 4 * r] = acc[r];
+  }
+  __syncthreads();
+  if (wave == 0 && valid) {
+    double sacc = d.s_root[0];
+    for (int w = 0; w < kMGroups; ++w) sacc += Rd[w * kTile + lane];
+    y[p] = sacc;
+  }
+}
+
 }  // namespace
 
 struct sbi_newton_plan {
   sbi_newton_desc d;
-  size_t lds_f32, lds_f64, lds_staged, lds_mfma;
+  size_t lds_f32, lds_f64, lds_staged, lds_mfma, lds_mfma64;
 };
 
 extern "C" int sbi_newton_plan_create(const sbi_newton_desc* desc, sbi_newton_plan** out) {
@@ -752,6 +975,13 @@ extern "C" int sbi_newton_plan_create(const sbi_newton_desc* desc, sbi_newton_pl
                    d.n_xi_dims >= 0 && d.n_xi_dims <= d.d_in)
                       ? tot
                       : (size_t)-1;
+    // its FP64 form: W and the h table in FP64, the A steps in global memory
+    const size_t w64 = 2 * w, hs64 = 2 * hs;
+    const size_t st64 = (size_t)d.mt_n * 256 + (size_t)d.mi_n * 16 +
+                        (((size_t)d.n_xi_dims * d.kmax1 * 8 + 15) & ~(size_t)15);
+    size_t tot64 = w64 + (hs64 > st64 ? hs64 : st64);
+    if (tot64 < red) tot64 = red;
+    p->lds_mfma64 = (p->lds_mfma != (size_t)-1 && d.mt_a64) ? tot64 : (size_t)-1;
   }
   // a node table beyond LDS leaves the plan to the basis, Jacobian and adjoint entry points: the
   // streaming kernels then return SBI_ERR_CAPACITY (the caller takes the GEMM form)
@@ -766,6 +996,14 @@ extern "C" int sbi_newton_mixed_staged(const sbi_newton_plan* plan) {
   }
   if (plan->lds_mfma <= (size_t)kMaxLds) return 2;
   return plan->lds_staged <= (size_t)kMaxLds ? 1 : 0;
+}
+
+extern "C" int sbi_newton_f64_matrix_cores(const sbi_newton_plan* plan) {
+  if (!plan) {
+    sbi::set_error("sbi_newton_f64_matrix_cores: null plan");
+    return SBI_ERR_INVALID;
+  }
+  return plan->lds_mfma64 <= (size_t)kMaxLds ? 1 : 0;
 }
 
 extern "C" int sbi_newton_plan_desc(const sbi_newton_plan* plan, const sbi_newton_desc** out) {
@@ -842,5 +1080,15 @@ extern "C" int sbi_newton_eval_mixed(const sbi_newton_plan* plan, const double* 
 
 extern "C" int sbi_newton_eval_f64(const sbi_newton_plan* plan, const double* x, int64_t P,
                                    int64_t ldx, double* y, void* stream) {
+  if (plan && x && y && P >= 0 && ldx >= plan->d.d_in && plan->d.d_out == 1 &&
+      plan->lds_mfma64 <= (size_t)kMaxLds) {
+    if (P == 0) return SBI_OK;
+    sbi::allow_lds((const void*)newton_f64_mfma_kernel);
+    hipLaunchKernelGGL(newton_f64_mfma_kernel, dim3((unsigned)((P + kTile - 1) / kTile)),
+                       dim3(kTile * kMWaves), plan->lds_mfma64, (hipStream_t)stream, plan->d, x,
+                       P, ldx, y);
+    SBI_HIP_CHECK(hipGetLastError());
+    return SBI_OK;
+  }
   return newton_launch<double>(plan, x, P, ldx, y, stream);
 }

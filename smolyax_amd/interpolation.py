@@ -411,6 +411,7 @@ class SmolyakBarycentricInterpolator:
             t["rint_rec"] = _dev_tensor(NL.rint_rec, i32, dev)
             t["rint_h"] = _dev_tensor(NL.rint_h, i32, dev)
             t["mt_a"] = _dev_tensor(NL.mt_a, f32, dev)
+            t["mt_a64"] = _dev_tensor(NL.mt_a64, f64, dev)
             t["mt_grp"] = torch.as_tensor(NL.mt_grp.view(np.uint8).copy()).to(dev)
             t["mi_rec"] = _dev_tensor(NL.mi_rec, i32, dev)
             t["mwave_item"] = _dev_tensor(NL.mwave_item, i32, dev)

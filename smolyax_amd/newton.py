@@ -414,7 +414,7 @@ def build_mfma_tables(L: NewtonLayout, vecs, n_waves: int = MFMA_WAVES) -> None:
     s0 = 0
     for t in tiles:
         ns = int(-(-max(lens[g] for g in t) // 4))
-        blk = np.zeros((ns, 4, 16), dtype=np.float32)  # [step, row in step, group]
+        blk = np.zeros((ns, 4, 16))  # [step, row in step, group]
         for i, g in enumerate(t):
             v = np.zeros(4 * ns)
             v[: len(vecs[g])] = vecs[g]
@@ -430,7 +430,8 @@ def build_mfma_tables(L: NewtonLayout, vecs, n_waves: int = MFMA_WAVES) -> None:
             else:
                 desc.append((0, 1, 0.0))
     L.mt_n = len(tiles)
-    L.mt_a = np.concatenate(A).reshape(-1).astype(np.float32) if A else np.zeros(64, np.float32)
+    L.mt_a64 = np.concatenate(A).reshape(-1) if A else np.zeros(64)
+    L.mt_a = L.mt_a64.astype(np.float32)
     L.mt_steps = s0
     descdt = np.dtype([("j", "<i4"), ("k", "<i4"), ("a", "<f8")])
     L.mt_grp = np.array(desc or [(0, 1, 0.0)] * 16, dtype=descdt)

@@ -102,3 +102,48 @@ def test_mixed_strided_points_same_bits(gpu, d, extra):
     assert torch.equal(y_view, y_cont)
     y64 = ip.evaluate(xv.contiguous(), "newton-fp64")
     assert float((y_view - y64).abs().max() / y64.abs().max()) < 1e-6
+
+
+@pytest.mark.parametrize("d,m", [(4, 60), (10, 1000), (40, 2000), (200, 5000), (1000, 10000)])
+def test_newton_fp64_matrix_cores_match_streaming(gpu, d, m):
+    """sbi_newton_eval_f64 on the FP64 matrix-core kernel (W in FP64, A steps from global memory,
+    v_mfma_f64_16x16x4) against the streaming FP64 kernel of the same plan (mt_a64 = NULL): the
+    same Newton form in a different summation order, within 1e-14 of max |y|; a ragged last tile
+    and a strided point array included."""
+    import ctypes
+
+    from smolyax_amd import _native as N
+    from smolyax_amd import nodes as SN
+    from smolyax_amd.interpolation import SmolyakBarycentricInterpolator
+
+    print(f"newton fp64 matrix cores d={d} m={m}", flush=True)
+    rng = np.random.default_rng(d + 1)
+    k = np.log((2 + np.arange(d)) / np.log(2))
+    t = OI.find_approximate_threshold(k, m, True)
+    a = rng.standard_normal(d) / (np.arange(d) + 1.0) ** 2
+    f = lambda x: np.exp(-(np.asarray(x) @ a))
+    ip = SmolyakBarycentricInterpolator(d_out=1, node_gen=SN.Leja(dim=d), k=k, t=t, f=f)
+    assert N.fn["sbi_newton_f64_matrix_cores"](ip._nplan) == 1
+    desc = N.NewtonDesc()
+    ctypes.memmove(ctypes.byref(desc), ctypes.byref(ip._ndesc), ctypes.sizeof(desc))
+    desc.mt_a64 = None
+    plan = ctypes.c_void_p()
+    N.check(N.fn["sbi_newton_plan_create"](ctypes.byref(desc), ctypes.byref(plan)), "plan")
+    try:
+        assert N.fn["sbi_newton_f64_matrix_cores"](plan) == 0
+        P = 1000
+        wide = torch.as_tensor(rng.uniform(-1, 1, (P, d + 3)), device=gpu)
+        for xx in (wide[:, :d].contiguous(), wide[:, 1:1 + d]):
+            st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+            y_mc = torch.empty((P, 1), dtype=torch.float64, device=gpu)
+            y_st = torch.empty((P, 1), dtype=torch.float64, device=gpu)
+            N.check(N.fn["sbi_newton_eval_f64"](ip._nplan, xx.data_ptr(), P, xx.stride(0),
+                                                y_mc.data_ptr(), st), "mc")
+            N.check(N.fn["sbi_newton_eval_f64"](plan, xx.data_ptr(), P, xx.stride(0),
+                                                y_st.data_ptr(), st), "stream")
+            torch.cuda.synchronize()
+            e = float((y_mc - y_st).abs().max() / y_st.abs().max())
+            print(f"  matrix cores vs streaming: {e:.1e}", flush=True)
+            assert e <= 1e-14
+    finally:
+        N.fn["sbi_newton_plan_destroy"](plan)
