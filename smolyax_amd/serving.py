@@ -1,5 +1,10 @@
 """Continuous-batching evaluation server for many resident interpolants on one GPU.
 
+Two schedulers with the same admission rule: `Server` (below, in Python: HIP graphs under a
+backlog, an HBM budget, every precision) and `NativeServer` (the library's C++ scheduler,
+sbi_server_*, mixed path: Python only submits requests and reads completions, so the host work
+per batch is a few HIP calls; the lower latency under load).
+
 Requests (interpolant name, points already on the device) arrive at any time.  A scheduler loop
 on one host thread keeps up to `n_streams` batches in flight, one per HIP stream, so their work
 overlaps on the device.  Whenever a stream's batch completes (polled through HIP events), new
