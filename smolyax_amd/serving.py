@@ -47,6 +47,7 @@ Latency = completion (event observed) - arrival; throughput = point evaluations 
 
 import collections
 import ctypes
+import gc
 import time
 from dataclasses import dataclass, field
 
@@ -322,7 +323,7 @@ class NativeServer:
     output is allocated on the caller's current stream and written on a server stream, and is
     final once `r.done` is set (the host has seen the batch finish)."""
 
-    def __init__(self, interpolants, n_streams=4, max_batch_points=8192, max_batch_work=1e10):
+    def __init__(self, interpolants, n_streams=4, max_batch_points=8192, max_batch_work=5e9):
         from . import _native as N
 
         self._N = N
@@ -411,7 +412,11 @@ def run_poisson(server, names, qps, n_requests, batch_lo=1, batch_hi=4096, seed=
     """Open-loop Poisson arrivals at `qps`; each request picks an interpolant uniformly and a
     batch size log-uniform in [batch_lo, batch_hi].  Inputs are drawn on the device beforehand
     (points ~ N(0, I)) so arrivals only enqueue.  With `time_limit_s`, a run that has not
-    completed every request by then raises RuntimeError naming what is still pending."""
+    completed every request by then raises RuntimeError naming what is still pending.  Python's
+    cyclic garbage collector is paused during the run (objects that exist before it are frozen
+    out of later collections): a full collection over the requests of earlier runs stalled the
+    scheduling loop for tens of milliseconds, as it would any long-running serving process that
+    does not manage it."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / qps, n_requests)
     picks = rng.integers(0, len(names), n_requests)
@@ -423,6 +428,20 @@ def run_poisson(server, names, qps, n_requests, batch_lo=1, batch_hi=4096, seed=
                               device="cuda", generator=gen) for p, b in zip(picks, sizes)]
     torch.cuda.synchronize()
     reqs = []
+    gc_was = gc.isenabled()
+    gc.collect()
+    gc.freeze()
+    gc.disable()
+    try:
+        _poisson_loop(server, names, n_requests, picks, inputs, gaps, reqs, time_limit_s, progress)
+    finally:
+        gc.unfreeze()
+        if gc_was:
+            gc.enable()
+    return reqs, inputs
+
+
+def _poisson_loop(server, names, n_requests, picks, inputs, gaps, reqs, time_limit_s, progress):
     t0 = time.perf_counter()
     arrivals = t0 + np.cumsum(gaps)
     i = 0
@@ -446,4 +465,3 @@ def run_poisson(server, names, qps, n_requests, batch_lo=1, batch_hi=4096, seed=
             break
         if not busy and i < n_requests:
             time.sleep(max(0.0, min(50e-6, arrivals[i] - time.perf_counter())))
-    return reqs, inputs
