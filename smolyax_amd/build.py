@@ -66,7 +66,7 @@ CHECK_SOURCES = [("indices.cpp", True), ("host_check.cpp", True), ("bary.hip", F
 
 def build_host_check(verbose=True):
     """Build smolyax_amd/_build/asan/host_check (indices.cpp, the plan entry points of bary.hip,
-    newton.hip, basis.hip and gemm.hip, and the driver csrc/host_check.cpp) with ASan + UBSan on
+    newton.hip, basis.hip and gemm.hip, and the test program csrc/host_check.cpp) with ASan + UBSan on
     the host side."""
     objdir = os.path.join(HERE, "_build", "asan")
     os.makedirs(objdir, exist_ok=True)

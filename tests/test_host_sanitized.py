@@ -1,7 +1,7 @@
 """Host side of the C ABI under AddressSanitizer and UndefinedBehaviorSanitizer (CPU, no GPU).
 
 smolyax_amd.build.build_host_check() builds csrc/host_check.cpp against indices.cpp and the plan
-entry points of bary.hip / newton.hip with ASan + UBSan on the host code.  The driver runs the
+entry points of bary.hip / newton.hip with ASan + UBSan on the host code.  The test program runs the
 index-set entry points with exactly sized buffers and through their error paths and creates,
 inspects and destroys plans (valid, incomplete and over-capacity descriptors).  Any sanitizer
 report fails the run; its printed index-set results must equal the production library's.
