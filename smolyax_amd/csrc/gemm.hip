@@ -199,26 +199,41 @@ __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
       }
   };
 
+  auto chunk = [&](int64_t k0) {
+    T ca[2][4], cb[FN][4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+#pragma unroll
+      for (int fm = 0; fm < 2; ++fm) ca[fm][s] = a[fm][s];
+#pragma unroll
+      for (int fn = 0; fn < FN; ++fn) cb[fn][s] = b[fn][s];
+    }
+    if (k0 + kChunk < K16) load(k0 + kChunk);  // next chunk in flight during this one
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn) acc[fm][fn] = mfma(ca[fm][s], cb[fn][s], acc[fm][fn]);
+  };
   if (K16 > 0) {
     load(0);
-    for (int64_t k0 = 0; k0 < K16; k0 += kChunk) {
-      T ca[2][4], cb[FN][4];
-#pragma unroll
-      for (int s = 0; s < 4; ++s) {
-#pragma unroll
-        for (int fm = 0; fm < 2; ++fm) ca[fm][s] = a[fm][s];
-#pragma unroll
-        for (int fn = 0; fn < FN; ++fn) cb[fn][s] = b[fn][s];
+    if constexpr (kK4 || !kF32) {
+      // the chunks of one kFlushK run form an inner loop with the flush after it, so the MFMA
+      // accumulators stay in the matrix-core registers across the run (a flush test inside the
+      // chunk loop made the compiler copy them to and from ordinary registers at every chunk:
+      // 4096 x 4096 outputs, K 10 016: FP64 7.42 -> 5.58 ms, FP32 k4 3.11 -> 2.90 ms).  The
+      // plain FP32 layout keeps the flat loop (3.73 ms flat, 4.53 ms nested).  Same sums.
+      for (int64_t kc = 0; kc < K16; kc += kFlushK) {
+        const int64_t kce = min(K16, kc + (int64_t)kFlushK);
+        for (int64_t k0 = kc; k0 < kce; k0 += kChunk) chunk(k0);
+        if (kce % kFlushK == 0) flush();  // a run cut short by K16 is flushed after the tail
       }
-      if (k0 + kChunk < K16) load(k0 + kChunk);  // next chunk in flight during this one
-#pragma unroll
-      for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int fm = 0; fm < 2; ++fm)
-#pragma unroll
-          for (int fn = 0; fn < FN; ++fn)
-            acc[fm][fn] = mfma(ca[fm][s], cb[fn][s], acc[fm][fn]);
-      if (((k0 + kChunk) % kFlushK) == 0) flush();
+    } else {
+      for (int64_t k0 = 0; k0 < K16; k0 += kChunk) {
+        chunk(k0);
+        if (((k0 + kChunk) % kFlushK) == 0) flush();
+      }
     }
   }
   if (!kK4 && K16 < K) {  // last partial chunk: rows k >= K contribute zero (K4: K % 16 == 0)
@@ -412,24 +427,28 @@ __global__ __launch_bounds__(64 * kWaves) void contract_split_kernel(
   const int64_t kb = c0 * kFlushK, ke = min(c1 * kFlushK, K16);
   if (kb < ke) {
     load(kb);
-    for (int64_t k0 = kb; k0 < ke; k0 += kChunk) {
-      T ca[2][4], cb[FN][4];
+    // one inner loop per kFlushK chunk, emitted after it (accumulators stay in place)
+    for (int64_t kc = kb; kc < ke; kc += kFlushK) {
+      const int64_t kce = min(ke, kc + (int64_t)kFlushK);
+      for (int64_t k0 = kc; k0 < kce; k0 += kChunk) {
+        T ca[2][4], cb[FN][4];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
+        for (int s = 0; s < 4; ++s) {
 #pragma unroll
-        for (int fm = 0; fm < 2; ++fm) ca[fm][s] = a[fm][s];
+          for (int fm = 0; fm < 2; ++fm) ca[fm][s] = a[fm][s];
 #pragma unroll
-        for (int fn = 0; fn < FN; ++fn) cb[fn][s] = b[fn][s];
+          for (int fn = 0; fn < FN; ++fn) cb[fn][s] = b[fn][s];
+        }
+        if (k0 + kChunk < ke) load(k0 + kChunk);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+          for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < FN; ++fn)
+              acc[fm][fn] = mfma(ca[fm][s], cb[fn][s], acc[fm][fn]);
       }
-      if (k0 + kChunk < ke) load(k0 + kChunk);
-#pragma unroll
-      for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int fm = 0; fm < 2; ++fm)
-#pragma unroll
-          for (int fn = 0; fn < FN; ++fn)
-            acc[fm][fn] = mfma(ca[fm][s], cb[fn][s], acc[fm][fn]);
-      if (((k0 + kChunk) % kFlushK) == 0) emit();
+      if (kce % kFlushK == 0) emit();
     }
   }
   if (!kK4 && c1 == nch && K16 < K) {  // last partial chunk of 16: rows k >= K contribute zero
@@ -577,24 +596,26 @@ int contract_launch(const T* A, int64_t lda, const T* B, int64_t ldb, const doub
 // layout (FP64 MFMA, as in contract_kernel).  BF16 products are exact in FP32; the FP32 partial
 // sums are added into FP64 registers every kBfFlush k.
 //
-// Work split: a workgroup of 4 waves (2 x 2) owns a tile of 32 * kBfWM points x 32 * kBfWN
-// outputs; each wave a sub-tile of kBfWM x kBfWN MFMA 16x16 fragments (4 x 2 for large grids,
-// 2 x 2 otherwise; 4 x 4 held one wave per SIMD and measured 15-40 % slower: 4096 points x
-// 4096 outputs, K 10 016: 1.16 ms -> 1.00 ms; 8192 x 256, K 5 024: 113 us -> 76 us).  Per 32-k
-// step a lane loads one 16-byte vector (its eight k) per fragment row and column, the next
-// step's vectors in flight during the current one (two steps ahead measured 3-5 % faster for
-// 2 x 2 and slower for 4 x 2: not kept).  Lane group g = lane / 16 supplies k = 8g .. 8g + 7 of
-// the step to both operands.  Every
-// workgroup is independent and each Y row depends only on its own column of A (fixed order).
+// Work split: a workgroup of 4 waves (2 x 2) owns a tile of TM = 32 * kBfWM points x TN =
+// 32 * kBfWN outputs; each wave a sub-tile of kBfWM x kBfWN MFMA 16x16 fragments (4 x 2 for
+// large grids, 2 x 2 otherwise; 4 x 4 held one wave per SIMD).  Per 32-k step the workgroup
+// copies the step's 16-byte A vectors (4 k-groups x TM points) and B vectors (4 x TN outputs)
+// into LDS with direct-to-LDS loads, kBfBuf buffers in rotation, the copies issued two steps
+// ahead and one barrier per step (a buffer is rewritten two steps after it was read, and every
+// wave has passed the barrier of the step in between); each wave then reads its fragments with
+// 16-byte LDS reads.  Lane group g = lane / 16 supplies k = 8g .. 8g + 7 of the step to both
+// operands.  The steps of one kBfFlush chunk form an inner loop with the flush after it, so the
+// FP32 accumulators stay in the matrix-core registers across the chunk (a flush test inside the
+// step loop made the compiler copy them to and from ordinary registers at every step).
+// Measured (same bits as each wave loading its own fragments from global memory): 4096 points x
+// 4096 outputs, K 10 016: 0.42 ms against 0.66-0.71 ms; 8192 x 256, K 5 024: 39 us against 66 us.
+// Every workgroup is independent and each Y row depends only on its own column of A (fixed order).
 // ------------------------------------------------------------------------------------------
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-constexpr int kBfFlush = 1024;                // k per FP32 partial sum
-
-__device__ __forceinline__ v8bf as_v8bf(const uint4& u) {
-  v8bf r;
-  __builtin_memcpy(&r, &u, sizeof(r));
-  return r;
-}
+typedef __attribute__((address_space(3))) v8bf lds_v8bf;
+typedef __attribute__((address_space(3))) void lds_void;
+constexpr int kBfFlush = 1024;  // k per FP32 partial sum
+constexpr int kBfBuf = 4;       // LDS buffers of one 32-k step each
 
 // kBfWM x kBfWN fragments per wave (points, outputs); 2 x 2 waves per workgroup
 template <int kBfWM, int kBfWN>
@@ -603,15 +624,20 @@ __global__ __launch_bounds__(256) void contract_bf16_kernel(
     int64_t K, const double* __restrict__ A64, int64_t lda64, const double* __restrict__ B64,
     int64_t ldb64, int64_t K64, const double* __restrict__ offset, double* __restrict__ Y,
     int64_t ldy, int64_t P, int64_t n) {
+  constexpr int TM = 32 * kBfWM, TN = 32 * kBfWN;
+  constexpr int kAB = 4 * TM * 16, kStep = kAB + 4 * TN * 16;  // bytes per step: A, then B
+  constexpr int nA = kAB / 1024, nI = kStep / 1024, kPer = nI / 4;  // 1 KB copies
+  static_assert(TM % 64 == 0 && TN % 64 == 0 && nI % 4 == 0 && (kPer == 2 || kPer == 3),
+                "tile shape");
+  __shared__ __attribute__((aligned(16))) char smem[kBfBuf * kStep];
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wm = wave & 1, wn = wave >> 1;
   const int li = lane & 15, g = lane >> 4;
-  const int64_t m_base = (int64_t)blockIdx.x * (32 * kBfWM) + wm * (16 * kBfWM);
-  const int64_t n_base = (int64_t)blockIdx.y * (32 * kBfWN) + wn * (16 * kBfWN);
-  int64_t pcol[kBfWM], jcol[kBfWN];
-#pragma unroll
-  for (int fm = 0; fm < kBfWM; ++fm) pcol[fm] = min(m_base + fm * 16 + li, P - 1);
+  const int64_t m0 = (int64_t)blockIdx.x * TM, n0 = (int64_t)blockIdx.y * TN;
+  const int64_t m_base = m0 + wm * (16 * kBfWM);
+  const int64_t n_base = n0 + wn * (16 * kBfWN);
+  int64_t jcol[kBfWN];
 #pragma unroll
   for (int fn = 0; fn < kBfWN; ++fn) jcol[fn] = min(n_base + fn * 16 + li, n - 1);
 
@@ -656,13 +682,6 @@ __global__ __launch_bounds__(256) void contract_bf16_kernel(
   for (int fm = 0; fm < kBfWM; ++fm)
 #pragma unroll
     for (int fn = 0; fn < kBfWN; ++fn) acc[fm][fn] = v4f{0, 0, 0, 0};
-  uint4 a[kBfWM], b[kBfWN];
-  auto load = [&](int64_t kq) {  // kq: index of the lane's 8-k group
-#pragma unroll
-    for (int fm = 0; fm < kBfWM; ++fm) a[fm] = A[kq * lda + pcol[fm]];
-#pragma unroll
-    for (int fn = 0; fn < kBfWN; ++fn) b[fn] = B[kq * ldb + jcol[fn]];
-  };
   auto flush = [&]() {
 #pragma unroll
     for (int fm = 0; fm < kBfWM; ++fm)
@@ -673,24 +692,67 @@ __global__ __launch_bounds__(256) void contract_bf16_kernel(
         acc[fm][fn] = v4f{0, 0, 0, 0};
       }
   };
-  if (K > 0) {
-    load(g);
-    for (int64_t k0 = 0; k0 < K; k0 += 32) {
-      uint4 ca[kBfWM], cb[kBfWN];
+  // copy c of a step (this wave: c = wave + 4 i): c < nA -> A vectors of k-group c / (TM / 64),
+  // points 64 (c % (TM / 64)) .., else B vectors likewise; it lands at LDS byte c * 1024 +
+  // 16 lane, i.e. [k-group][point] then [k-group][output].  Points and outputs past the edge
+  // re-read the last one (their results are not stored).
+  const uint4* src[kPer];
+  int64_t sstep[kPer];
 #pragma unroll
-      for (int fm = 0; fm < kBfWM; ++fm) ca[fm] = a[fm];
-#pragma unroll
-      for (int fn = 0; fn < kBfWN; ++fn) cb[fn] = b[fn];
-      if (k0 + 32 < K) load(((k0 + 32) >> 3) + g);  // next step in flight during this one
-#pragma unroll
-      for (int fm = 0; fm < kBfWM; ++fm)
-#pragma unroll
-        for (int fn = 0; fn < kBfWN; ++fn)
-          acc[fm][fn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_v8bf(ca[fm]), as_v8bf(cb[fn]),
-                                                                acc[fm][fn], 0, 0, 0);
-      if (((k0 + 32) % kBfFlush) == 0) flush();
+  for (int i = 0; i < kPer; ++i) {
+    const int c = wave + 4 * i;
+    if (c < nA) {
+      const int kq = c / (TM / 64), part = c % (TM / 64);
+      src[i] = A + (int64_t)kq * lda + min(m0 + part * 64 + lane, P - 1);
+      sstep[i] = 4 * lda;
+    } else {
+      const int kq = (c - nA) / (TN / 64), part = (c - nA) % (TN / 64);
+      src[i] = B + (int64_t)kq * ldb + min(n0 + part * 64 + lane, n - 1);
+      sstep[i] = 4 * ldb;
     }
-    flush();
+  }
+  auto issue = [&](int s) {
+    char* buf = smem + (s % kBfBuf) * kStep;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i)
+      __builtin_amdgcn_global_load_lds((const void*)(src[i] + (int64_t)s * sstep[i]),
+                                       (lds_void*)(buf + (wave + 4 * i) * 1024), 16, 0, 0);
+  };
+  const int S = (int)(K / 32);
+  if (S > 0) {
+    issue(0);
+    if (S > 1) issue(1);
+    for (int sc = 0; sc < S; sc += kBfFlush / 32) {
+      const int se = min(S, sc + kBfFlush / 32);
+      for (int s = sc; s < se; ++s) {
+        // this wave's copies of step s have landed once at most those of s + 1, s + 2 remain
+        if (s + 2 < S) {
+          issue(s + 2);
+          if constexpr (kPer == 3) asm volatile("s_waitcnt vmcnt(6)" ::: "memory");
+          else asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
+        } else if (s + 1 < S) {
+          if constexpr (kPer == 3) asm volatile("s_waitcnt vmcnt(3)" ::: "memory");
+          else asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
+        } else {
+          asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        }
+        asm volatile("s_barrier" ::: "memory");  // every wave's copies of step s have landed
+        const lds_v8bf* LA = (const lds_v8bf*)(smem + (s % kBfBuf) * kStep);
+        const lds_v8bf* LB = (const lds_v8bf*)(smem + (s % kBfBuf) * kStep + kAB);
+        v8bf a[kBfWM], b[kBfWN];
+#pragma unroll
+        for (int fm = 0; fm < kBfWM; ++fm) a[fm] = LA[g * TM + wm * (16 * kBfWM) + fm * 16 + li];
+#pragma unroll
+        for (int fn = 0; fn < kBfWN; ++fn) b[fn] = LB[g * TN + wn * (16 * kBfWN) + fn * 16 + li];
+#pragma unroll
+        for (int fm = 0; fm < kBfWM; ++fm)
+#pragma unroll
+          for (int fn = 0; fn < kBfWN; ++fn)
+            acc[fm][fn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[fm], b[fn], acc[fm][fn], 0, 0,
+                                                                  0);
+      }
+      flush();
+    }
   }
 
 #pragma unroll
@@ -992,8 +1054,9 @@ extern "C" int sbi_contract_bf16_k8(const double* A64, int64_t lda64, const doub
   }
   if (P == 0 || n == 0) return SBI_OK;
   // per-wave tiles: 64 x 32 (workgroup 128 x 64) while that gives at least 1024 workgroups, else
-  // 32 x 32 (64 x 64): smaller tiles hold fewer registers (2 and 3 waves per SIMD instead of one
-  // at 64 x 64) and spread small grids further; every element's sum is the same (same bits)
+  // 32 x 32 (64 x 64): smaller tiles hold fewer registers (2 and 4 waves per SIMD instead of one
+  // at 64 x 64; LDS 48 and 32 KB per workgroup) and spread small grids further; every element's
+  // sum is the same (same bits)
   const int64_t g42 = ((P + 127) / 128) * ((n + 63) / 64);
   if (g42 >= 1024) {
     const dim3 grid((unsigned)((P + 127) / 128), (unsigned)((n + 63) / 64));
