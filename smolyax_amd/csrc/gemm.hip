@@ -597,8 +597,8 @@ int contract_launch(const T* A, int64_t lda, const T* B, int64_t ldb, const doub
 // sums are added into FP64 registers every kBfFlush k.
 //
 // Work split: a workgroup of 4 waves (2 x 2) owns a tile of TM = 32 * kBfWM points x TN =
-// 32 * kBfWN outputs; each wave a sub-tile of kBfWM x kBfWN MFMA 16x16 fragments (4 x 2 for
-// large grids, 2 x 2 otherwise; 4 x 4 held one wave per SIMD).  Per 32-k step the workgroup
+// 32 * kBfWN outputs; each wave a sub-tile of kBfWM x kBfWN MFMA 16x16 fragments (4 x 4 for the
+// largest grids, 4 x 2 for large ones, 2 x 2 otherwise).  Per 32-k step the workgroup
 // copies the step's 16-byte A vectors (4 k-groups x TM points) and B vectors (4 x TN outputs)
 // into LDS with direct-to-LDS loads, kBfBuf buffers in rotation, the copies issued two steps
 // ahead and one barrier per step (a buffer is rewritten two steps after it was read, and every
@@ -608,7 +608,8 @@ int contract_launch(const T* A, int64_t lda, const T* B, int64_t ldb, const doub
 // FP32 accumulators stay in the matrix-core registers across the chunk (a flush test inside the
 // step loop made the compiler copy them to and from ordinary registers at every step).
 // Measured (same bits as each wave loading its own fragments from global memory): 4096 points x
-// 4096 outputs, K 10 016: 0.42 ms against 0.66-0.71 ms; 8192 x 256, K 5 024: 39 us against 66 us.
+// 4096 outputs, K 10 016: 0.42 ms against 0.66-0.71 ms (4 x 2 fragments; 0.32-0.35 ms with 4 x 4);
+// 8192 x 256, K 5 024: 39 us against 66 us.
 // Every workgroup is independent and each Y row depends only on its own column of A (fixed order).
 // ------------------------------------------------------------------------------------------
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
@@ -617,9 +618,10 @@ typedef __attribute__((address_space(3))) void lds_void;
 constexpr int kBfFlush = 1024;  // k per FP32 partial sum
 constexpr int kBfBuf = 4;       // LDS buffers of one 32-k step each
 
-// kBfWM x kBfWN fragments per wave (points, outputs); 2 x 2 waves per workgroup
+// kBfWM x kBfWN fragments per wave (points, outputs); 2 x 2 waves per workgroup (the body of the
+// kernels below; each has its own LDS buffers)
 template <int kBfWM, int kBfWN>
-__global__ __launch_bounds__(256) void contract_bf16_kernel(
+__device__ __forceinline__ void contract_bf16_body(
     const uint4* __restrict__ A, int64_t lda, const uint4* __restrict__ B, int64_t ldb,
     int64_t K, const double* __restrict__ A64, int64_t lda64, const double* __restrict__ B64,
     int64_t ldb64, int64_t K64, const double* __restrict__ offset, double* __restrict__ Y,
@@ -627,7 +629,7 @@ __global__ __launch_bounds__(256) void contract_bf16_kernel(
   constexpr int TM = 32 * kBfWM, TN = 32 * kBfWN;
   constexpr int kAB = 4 * TM * 16, kStep = kAB + 4 * TN * 16;  // bytes per step: A, then B
   constexpr int nA = kAB / 1024, nI = kStep / 1024, kPer = nI / 4;  // 1 KB copies
-  static_assert(TM % 64 == 0 && TN % 64 == 0 && nI % 4 == 0 && (kPer == 2 || kPer == 3),
+  static_assert(TM % 64 == 0 && TN % 64 == 0 && nI % 4 == 0 && kPer >= 2 && kPer <= 4,
                 "tile shape");
   __shared__ __attribute__((aligned(16))) char smem[kBfBuf * kStep];
   const int lane = threadIdx.x & 63;
@@ -728,10 +730,12 @@ __global__ __launch_bounds__(256) void contract_bf16_kernel(
         // this wave's copies of step s have landed once at most those of s + 1, s + 2 remain
         if (s + 2 < S) {
           issue(s + 2);
-          if constexpr (kPer == 3) asm volatile("s_waitcnt vmcnt(6)" ::: "memory");
+          if constexpr (kPer == 4) asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
+          else if constexpr (kPer == 3) asm volatile("s_waitcnt vmcnt(6)" ::: "memory");
           else asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
         } else if (s + 1 < S) {
-          if constexpr (kPer == 3) asm volatile("s_waitcnt vmcnt(3)" ::: "memory");
+          if constexpr (kPer == 4) asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
+          else if constexpr (kPer == 3) asm volatile("s_waitcnt vmcnt(3)" ::: "memory");
           else asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
         } else {
           asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -769,6 +773,26 @@ __global__ __launch_bounds__(256) void contract_bf16_kernel(
       }
   }
 }
+
+#define SBI_BF16_ARGS                                                                          \
+  const uint4 *__restrict__ A, int64_t lda, const uint4 *__restrict__ B, int64_t ldb, int64_t K, \
+      const double *__restrict__ A64, int64_t lda64, const double *__restrict__ B64,             \
+      int64_t ldb64, int64_t K64, const double *__restrict__ offset, double *__restrict__ Y,     \
+      int64_t ldy, int64_t P, int64_t n
+template <int kBfWM, int kBfWN>
+__global__ __launch_bounds__(256) void contract_bf16_kernel(SBI_BF16_ARGS) {
+  contract_bf16_body<kBfWM, kBfWN>(A, lda, B, ldb, K, A64, lda64, B64, ldb64, K64, offset, Y, ldy,
+                                   P, n);
+}
+// 4 x 4 fragments per wave (workgroup 128 x 128), asked for two waves per SIMD: 243 registers
+// and no scratch (one wave per SIMD otherwise).  4096 points x 4096 outputs, K 10 016: 323-350 us
+// against 416-485 us for 4 x 2 fragments; same bits.  (The request on the 4 x 2 and 2 x 2 forms
+// measured no gain, so it is this kernel's alone.)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void
+contract_bf16_wide_kernel(SBI_BF16_ARGS) {
+  contract_bf16_body<4, 4>(A, lda, B, ldb, K, A64, lda64, B64, ldb64, K64, offset, Y, ldy, P, n);
+}
+#undef SBI_BF16_ARGS
 
 // ------------------------------------------------------------------------------------------
 // Jacobian contraction: for each input dim z, J[p, c, z] = sum_{k in [zoff[z], zoff[z+1])}
@@ -1053,12 +1077,18 @@ extern "C" int sbi_contract_bf16_k8(const double* A64, int64_t lda64, const doub
     return SBI_ERR_INVALID;
   }
   if (P == 0 || n == 0) return SBI_OK;
-  // per-wave tiles: 64 x 32 (workgroup 128 x 64) while that gives at least 1024 workgroups, else
-  // 32 x 32 (64 x 64): smaller tiles hold fewer registers (2 and 4 waves per SIMD instead of one
-  // at 64 x 64; LDS 48 and 32 KB per workgroup) and spread small grids further; every element's
-  // sum is the same (same bits)
+  // per-wave tiles: 64 x 64 (workgroup 128 x 128, LDS 64 KB) while that gives at least 1024
+  // workgroups, else 64 x 32 (128 x 64, 48 KB) likewise, else 32 x 32 (64 x 64, 32 KB): the
+  // smaller tiles spread small grids further (2 and 4 waves per SIMD); every element's sum is the
+  // same (same bits)
+  const int64_t g44 = ((P + 127) / 128) * ((n + 127) / 128);
   const int64_t g42 = ((P + 127) / 128) * ((n + 63) / 64);
-  if (g42 >= 1024) {
+  if (g44 >= 1024) {
+    const dim3 grid((unsigned)((P + 127) / 128), (unsigned)((n + 127) / 128));
+    hipLaunchKernelGGL(contract_bf16_wide_kernel, grid, dim3(256), 0, (hipStream_t)stream,
+                       (const uint4*)A16, lda16, (const uint4*)B16, ldb16, K16, A64, lda64, B64,
+                       ldb64, K64, offset, Y, ldy, P, n);
+  } else if (g42 >= 1024) {
     const dim3 grid((unsigned)((P + 127) / 128), (unsigned)((n + 63) / 64));
     hipLaunchKernelGGL((contract_bf16_kernel<4, 2>), grid, dim3(256), 0, (hipStream_t)stream,
                        (const uint4*)A16, lda16, (const uint4*)B16, ldb16, K16, A64, lda64, B64,
