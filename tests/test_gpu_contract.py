@@ -4,6 +4,9 @@ Reference: the same product in FP64 on the device with torch (operands upcast ex
 per element, relative to sum_k |A[k,p]| |B[k,j]|: FP64 operands 1e-13 (FP64 MFMA, K <= 4096 terms);
 FP32 operands 2e-6 (products and 256-term partial sums in FP32, flushed into FP64).  Rows are
 independent: a point's row is identical bit for bit alone and inside a larger batch.
+sbi_contract_split_k4(_ws): the same bits as the plain layout and the unsplit kernel.
+sbi_contract_bf16_k8: against the FP64 product of the same BF16 operands (bound from the FP32
+partial sums of 1024 terms), and the same bits for its three tile shapes.
 """
 
 import ctypes
@@ -208,3 +211,65 @@ def test_contract_split_k4_ws_same_bits(gpu, P, n, K64, K32, split):
     torch.cuda.synchronize()
     assert torch.equal(Y0, Y1)
     assert torch.equal(Y0, Y2)
+
+
+def _k8(Mb, K):
+    """(K, m) BF16 -> sbi_contract_bf16_k8's k-interleaved layout [K/8][m][8]."""
+    return Mb.view(K // 8, 8, Mb.shape[1]).permute(0, 2, 1).contiguous()
+
+
+def _bf16_contract(A64, B64, K64, A8, B8, K16, off, P, n, lda, ldb):
+    Y = torch.full((P, n), float("nan"), dtype=torch.float64, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    N.check(N.fn["sbi_contract_bf16_k8"](A64.data_ptr() if K64 else None, lda,
+                                         B64.data_ptr() if K64 else None, ldb, K64,
+                                         A8.data_ptr() if K16 else None, lda,
+                                         B8.data_ptr() if K16 else None, ldb, K16,
+                                         off.data_ptr(), Y.data_ptr(), n, P, n, st), "bf16_k8")
+    torch.cuda.synchronize()
+    return Y
+
+
+@pytest.mark.parametrize("P,n,K64,K16", [(100, 33, 0, 64), (257, 300, 5, 1024), (1000, 130, 16, 2080),
+                                         (4096, 4096, 3, 1024), (64, 7, 9, 0), (5, 3, 0, 32)])
+def test_contract_bf16_matches_fp64_product(gpu, P, n, K64, K16):
+    """sbi_contract_bf16_k8 against the FP64 product of the same BF16 operands (and FP64 head
+    rows).  BF16 products are exact in FP32; each FP32 partial sum covers at most 1024 terms
+    before it is added into FP64, so per element |Y - ref| <= 1023 * 2^-24 * sum |a b| (+ the
+    FP64 head's rounding): bound 6.2e-5 relative to sum_k |A[k,p]| |B[k,j]|."""
+    print(f"bf16_k8 P={P} n={n} K64={K64} K16={K16}", flush=True)
+    g = torch.Generator(device="cuda").manual_seed(P * 13 + n + K16)
+    A64 = torch.randn((max(K64, 1), P), dtype=torch.float64, device="cuda", generator=g)[:K64].contiguous()
+    B64 = torch.randn((max(K64, 1), n), dtype=torch.float64, device="cuda", generator=g)[:K64].contiguous()
+    Ab = torch.randn((max(K16, 8), P), dtype=torch.float64, device="cuda", generator=g)[:K16].to(torch.bfloat16)
+    Bb = torch.randn((max(K16, 8), n), dtype=torch.float64, device="cuda", generator=g)[:K16].to(torch.bfloat16)
+    off = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    A8 = _k8(Ab, K16) if K16 else Ab
+    B8 = _k8(Bb, K16) if K16 else Bb
+    Y = _bf16_contract(A64, B64, K64, A8, B8, K16, off, P, n, P, n)
+    Ad = torch.cat([A64, Ab.double()]) if K16 else A64
+    Bd = torch.cat([B64, Bb.double()]) if K16 else B64
+    ref = Ad.t() @ Bd + off
+    scale = Ad.abs().t() @ Bd.abs()
+    assert torch.isfinite(Y).all()
+    err = ((Y - ref).abs() / (scale + 1e-300)).max().item()
+    assert err <= 6.2e-5, (P, n, K64, K16, err)
+
+
+def test_contract_bf16_tile_shapes_same_bits(gpu):
+    """The three tile shapes of sbi_contract_bf16_k8 (4 x 4 fragments per wave for grids of at
+    least 1024 workgroups of 128 x 128, else 4 x 2 for 1024 of 128 x 64, else 2 x 2) sum every
+    element in the same order: 4096, 2048 and 512 points of the same operands (one row stride)
+    give the same bits on their common rows, with and without FP64 head rows."""
+    n, K16 = 4096, 2080
+    g = torch.Generator(device="cuda").manual_seed(11)
+    Ab = torch.randn((K16, 4096), dtype=torch.float64, device="cuda", generator=g).to(torch.bfloat16)
+    Bb = torch.randn((K16, n), dtype=torch.float64, device="cuda", generator=g).to(torch.bfloat16)
+    A8, B8 = _k8(Ab, K16), _k8(Bb, K16)
+    off = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    for K64 in (0, 7):
+        A64 = torch.randn((max(K64, 1), 4096), dtype=torch.float64, device="cuda", generator=g)[:K64].contiguous()
+        B64 = torch.randn((max(K64, 1), n), dtype=torch.float64, device="cuda", generator=g)[:K64].contiguous()
+        Ys = [_bf16_contract(A64, B64, K64, A8, B8, K16, off, P, n, 4096, n) for P in (4096, 2048, 512)]
+        assert torch.equal(Ys[0][:2048], Ys[1]), K64
+        assert torch.equal(Ys[0][:512], Ys[2]), K64
