@@ -873,8 +873,8 @@ __global__ __launch_bounds__(64 * kWaves) void jac_contract_kernel(
     load(R0);
     if (R0 + kChunk < R1) load_idx(R0 + kChunk);
   }
-  for (int64_t k0 = R0; k0 < R1; k0 += kChunk) {
-    double ca[2][4], cb[FN][4];
+  double ca[2][4], cb[FN][4];
+  auto advance = [&](int64_t k0) {  // this chunk's operands; the next chunk's loads in flight
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       ca[0][s] = a[0][s];
@@ -882,10 +882,12 @@ __global__ __launch_bounds__(64 * kWaves) void jac_contract_kernel(
 #pragma unroll
       for (int fn = 0; fn < FN; ++fn) cb[fn][s] = b[fn][s];
     }
-    if (k0 + kChunk < R1) {  // next chunk in flight during this one
+    if (k0 + kChunk < R1) {
       load(k0 + kChunk);
       if (k0 + 2 * kChunk < R1) load_idx(k0 + 2 * kChunk);
     }
+  };
+  auto apply = [&](int64_t k0) {  // a chunk applied to every dim it touches
     const int64_t klast = min(k0 + kChunk, R1) - 1;
     // dims touched by this chunk: [first row's dim, last row's dim] (uniform)
 #pragma unroll
@@ -912,6 +914,32 @@ __global__ __launch_bounds__(64 * kWaves) void jac_contract_kernel(
           }
         }
       }
+    }
+  };
+  // The chunks in stream order: those wholly inside dim jj in a branch-free loop on acc[jj], then
+  // the chunk that ends dim jj inside it (or the stream's last, partial chunk) through apply.  At
+  // dim jj the next chunk starts at or after the dim's first row, so every chunk is applied once
+  // and each dim's sum sees the same MFMAs in the same order as a single chunk loop with a per-dim
+  // test (same bits; 320 points x 4096 outputs x 100 dims: 3.14-3.23 ms against 3.45-3.51 ms,
+  // 2560 x 256 x 200: 1.15-1.23 ms against 1.20-1.26 ms).
+  int64_t k0 = R0;
+#pragma unroll
+  for (int jj = 0; jj < JG; ++jj) {
+    while (k0 < R1 && zb[jj] <= k0 && k0 + kChunk <= zb[jj + 1]) {
+      advance(k0);
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+          for (int fn = 0; fn < FN; ++fn)
+            acc[jj][fm][fn] = mfma(ca[fm][s], cb[fn][s], acc[jj][fm][fn]);
+      k0 += kChunk;
+    }
+    if (k0 < R1 && k0 < zb[jj + 1]) {
+      advance(k0);
+      apply(k0);
+      k0 += kChunk;
     }
   }
 
