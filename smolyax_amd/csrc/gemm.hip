@@ -62,6 +62,44 @@ __device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// FP64 head rows k < K64 of A64 (K64 x P) and B64 (K64 x n) into h with the FP64 MFMA (lane group
+// g supplies row k0 + g of each 4-row step; lane li point ph[fm] and output jcol[fn]).  Four steps
+// per iteration: all their loads are issued before their MFMAs (one memory round trip per 16
+// rows instead of per 4), and the MFMAs run in row order, steps past K64 skipped.
+template <int WM, int WN>
+__device__ __forceinline__ void head_fp64(const double* __restrict__ A64, int64_t lda64,
+                                          const double* __restrict__ B64, int64_t ldb64,
+                                          int64_t K64, int g, const int64_t (&ph)[WM],
+                                          const int64_t (&jcol)[WN], v4d (&h)[WM][WN]) {
+  for (int64_t k0 = 0; k0 < K64; k0 += 16) {
+    double ha[4][WM], hb[4][WN];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t k = k0 + 4 * u + g;
+      const bool in = k < K64;
+      const int64_t kc = in ? k : K64 - 1;
+#pragma unroll
+      for (int fm = 0; fm < WM; ++fm) {
+        const double v = A64[kc * lda64 + ph[fm]];
+        ha[u][fm] = in ? v : 0.0;
+      }
+#pragma unroll
+      for (int fn = 0; fn < WN; ++fn) {
+        const double v = B64[kc * ldb64 + jcol[fn]];
+        hb[u][fn] = in ? v : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (k0 + 4 * u >= K64) break;
+#pragma unroll
+      for (int fm = 0; fm < WM; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < WN; ++fn) h[fm][fn] = mfma(ha[u][fm], hb[u][fn], h[fm][fn]);
+    }
+  }
+}
+
 // Optional FP64 head (FP32 operands only): rows k < K64 of A64 (K64 x P) and B64 (K64 x n) are
 // contracted first with the FP64 MFMA, straight into the FP64 accumulators.  Lane li supplies the
 // A row of point 4*(li & 3) + (li >> 2) of its 16-point block, which puts the FP64 MFMA's result
@@ -115,26 +153,7 @@ __global__ __launch_bounds__(64 * kWaves) void contract_kernel(
       for (int fm = 0; fm < 2; ++fm)
 #pragma unroll
         for (int fn = 0; fn < FN; ++fn) h[fm][fn] = v4d{0, 0, 0, 0};
-      for (int64_t k0 = 0; k0 < K64; k0 += 4) {
-        const int64_t k = k0 + g;
-        const bool in = k < K64;
-        const int64_t kc = in ? k : K64 - 1;
-        double ha[2], hb[FN];
-#pragma unroll
-        for (int fm = 0; fm < 2; ++fm) {
-          const double v = A64[kc * lda64 + ph[fm]];
-          ha[fm] = in ? v : 0.0;
-        }
-#pragma unroll
-        for (int fn = 0; fn < FN; ++fn) {
-          const double v = B64[kc * ldb64 + jcol[fn]];
-          hb[fn] = in ? v : 0.0;
-        }
-#pragma unroll
-        for (int fm = 0; fm < 2; ++fm)
-#pragma unroll
-          for (int fn = 0; fn < FN; ++fn) h[fm][fn] = mfma(ha[fm], hb[fn], h[fm][fn]);
-      }
+      head_fp64<2, FN>(A64, lda64, B64, ldb64, K64, g, ph, jcol, h);
 #pragma unroll
       for (int fm = 0; fm < 2; ++fm)
 #pragma unroll
@@ -329,26 +348,7 @@ __global__ __launch_bounds__(64 * kWaves) void contract_split_kernel(
       for (int fm = 0; fm < 2; ++fm)
 #pragma unroll
         for (int fn = 0; fn < FN; ++fn) h[fm][fn] = v4d{0, 0, 0, 0};
-      for (int64_t k0 = 0; k0 < K64; k0 += 4) {
-        const int64_t k = k0 + g;
-        const bool in = k < K64;
-        const int64_t kc = in ? k : K64 - 1;
-        double ha[2], hb[FN];
-#pragma unroll
-        for (int fm = 0; fm < 2; ++fm) {
-          const double v = A64[kc * lda64 + ph[fm]];
-          ha[fm] = in ? v : 0.0;
-        }
-#pragma unroll
-        for (int fn = 0; fn < FN; ++fn) {
-          const double v = B64[kc * ldb64 + jcol[fn]];
-          hb[fn] = in ? v : 0.0;
-        }
-#pragma unroll
-        for (int fm = 0; fm < 2; ++fm)
-#pragma unroll
-          for (int fn = 0; fn < FN; ++fn) h[fm][fn] = mfma(ha[fm], hb[fn], h[fm][fn]);
-      }
+      head_fp64<2, FN>(A64, lda64, B64, ldb64, K64, g, ph, jcol, h);
 #pragma unroll
       for (int fm = 0; fm < 2; ++fm)
 #pragma unroll
@@ -655,28 +655,7 @@ __device__ __forceinline__ void contract_bf16_body(
 #pragma unroll
     for (int fm = 0; fm < kBfWM; ++fm)
       ph[fm] = min(m_base + fm * 16 + 4 * (li & 3) + (li >> 2), P - 1);
-    for (int64_t k0 = 0; k0 < K64; k0 += 4) {
-      const int64_t k = k0 + g;
-      const bool in = k < K64;
-      const int64_t kc = in ? k : K64 - 1;
-      double ha[kBfWM], hb[kBfWN];
-#pragma unroll
-      for (int fm = 0; fm < kBfWM; ++fm) {
-        const double v = A64[kc * lda64 + ph[fm]];
-        ha[fm] = in ? v : 0.0;
-      }
-#pragma unroll
-      for (int fn = 0; fn < kBfWN; ++fn) {
-        const double v = B64[kc * ldb64 + jcol[fn]];
-        hb[fn] = in ? v : 0.0;
-      }
-#pragma unroll
-      for (int fm = 0; fm < kBfWM; ++fm)
-#pragma unroll
-        for (int fn = 0; fn < kBfWN; ++fn)
-          acc64[fm][fn] = __builtin_amdgcn_mfma_f64_16x16x4f64(ha[fm], hb[fn], acc64[fm][fn], 0,
-                                                               0, 0);
-    }
+    head_fp64<kBfWM, kBfWN>(A64, lda64, B64, ldb64, K64, g, ph, jcol, acc64);
   }
 
   v4f acc[kBfWM][kBfWN];
