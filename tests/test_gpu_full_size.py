@@ -23,7 +23,11 @@ exact interpolant):
 - the gradient on a slice of the points against p's analytic derivative to 1e-13 of max |J|
   (measured <= 1.3e-15);
 - the integral (offset + q^T F, the same zeta-weighted combination) against p's exact integral
-  over [-1, 1]^d with density 2^-d, to the same bound times max |p|.
+  over [-1, 1]^d with density 2^-d, to the same bound times max |p|;
+- the adjoint (vjp in FP64: G^T = S V^T on the matrix cores, then the Newton-form adjoint) on the
+  gradient slice against v . J of p to 1e-12 of max |v . J|, and against central differences
+  (h = 1e-6, FP64 evaluations) over every input dim on 4 points to 1e-6 (truncation h^2 p''' / 6
+  and rounding u |y| / h are both far below it).
 
 test_polynomial_property_on_oracle checks on the CPU, at a small size, that the oracle itself
 reproduces p, its derivative and its integral (so the property, not this file's host formulas,
@@ -205,3 +209,29 @@ def test_full_size_integral(workload):
     scale = float(np.max(np.abs(w["p"](w["x"][:256]))))
     print(f"{w['name']} integral: max error {e:.2e} (bound {w['tol64'] * scale:.2e})", flush=True)
     assert e <= w["tol64"] * scale
+
+
+@pytest.mark.gpu
+def test_full_size_vjp(workload):
+    w = workload
+    n = w["n_grad"]
+    ip, p, x = w["ip"], w["p"], w["x"]
+    d, d_out = x.shape[1], p.C.shape[1]
+    v = np.random.default_rng(5).standard_normal((n, d_out))
+    print(f"{w['name']} vjp on {n} points", flush=True)
+    g = ip.vjp(w["xt"][:n], torch.as_tensor(v, device="cuda"), "fp64").cpu().numpy()
+    gr = np.einsum("po,pod->pd", v, p.jacobian(x[:n]))
+    e = rel(g, gr)
+    print(f"{w['name']} vjp: max error {e:.2e} of max |v.J|", flush=True)
+    assert e <= 1e-12
+    # central differences of v . y over every input dim on 4 points (FP64 evaluations)
+    h, m = 1e-6, 4
+    E = np.eye(d) * h
+    Xp = (x[:m, None, :] + E[None]).reshape(-1, d)
+    Xm = (x[:m, None, :] - E[None]).reshape(-1, d)
+    yp = ip.evaluate(torch.as_tensor(Xp, device="cuda"), "newton-fp64").cpu().numpy()
+    ym = ip.evaluate(torch.as_tensor(Xm, device="cuda"), "newton-fp64").cpu().numpy()
+    fd = np.einsum("pjo,po->pj", ((yp - ym) / (2 * h)).reshape(m, d, d_out), v[:m])
+    e_fd = rel(g[:m], fd)
+    print(f"{w['name']} vjp vs central differences: {e_fd:.2e}", flush=True)
+    assert e_fd <= 1e-6
