@@ -231,10 +231,11 @@ def _bf16_contract(A64, B64, K64, A8, B8, K16, off, P, n, lda, ldb):
 
 
 @pytest.mark.parametrize("P,n,K64,K16", [(100, 33, 0, 64), (257, 300, 5, 1024), (1000, 130, 16, 2080),
-                                         (4096, 4096, 3, 1024), (64, 7, 9, 0), (5, 3, 0, 32)])
+                                         (4096, 4096, 3, 1024), (4000, 4100, 2, 96), (64, 7, 9, 0),
+                                         (5, 3, 0, 32)])
 def test_contract_bf16_matches_fp64_product(gpu, P, n, K64, K16):
     """sbi_contract_bf16_k8 against the FP64 product of the same BF16 operands (and FP64 head
-    rows).  BF16 products are exact in FP32; each FP32 partial sum covers at most 1024 terms
+    rows); 4000 x 4100 takes the 4 x 4 tiles with partial tiles on both edges.  BF16 products are exact in FP32; each FP32 partial sum covers at most 1024 terms
     before it is added into FP64, so per element |Y - ref| <= 1023 * 2^-24 * sum |a b| (+ the
     FP64 head's rounding): bound 6.2e-5 relative to sum_k |A[k,p]| |B[k,j]|."""
     print(f"bf16_k8 P={P} n={n} K64={K64} K16={K16}", flush=True)
